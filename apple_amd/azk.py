@@ -1,0 +1,524 @@
+"""ctypes binding of libazk.so (include/azk.h).
+
+Python-side mirror of the reference's reusable-plan API (AppleZKProver, Swift): the
+class names are the Swift plan names, methods take numpy arrays for host data and
+``Buffer`` objects for resident HBM data.  Errors raise ``AZKError`` carrying the
+status code that maps one-to-one onto ``AppleZKProverError`` (MetalTypes.swift:3-23).
+
+There is no CPU fallback: if libazk.so or a GPU is missing, constructors raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libazk.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "azk.h")
+
+STATUS = {
+    0: "ok", 1: "noDevice", 2: "allocationFailed", 3: "unsupportedOneBlockLength", 4: "invalidLeafCount",
+    5: "invalidPrehashedLeafLength", 6: "invalidInputLayout", 7: "correctnessValidationFailed",
+    8: "commandExecutionFailed", 9: "unavailableOnThisPlatform", 10: "invalidKernelConfiguration",
+    11: "invalidArgument",
+}
+
+
+class AZKError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS.get(code, code)}: {message}")
+        self.code = code
+        self.status = STATUS.get(code, str(code))
+
+
+class Stats(C.Structure):
+    _fields_ = [("wall_ns", C.c_uint64), ("gpu_ns", C.c_uint64), ("submit_ns", C.c_uint64)]
+
+
+class Capabilities(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int),
+                ("wavefront_size", C.c_int), ("lds_bytes_per_workgroup", C.c_size_t), ("hbm_bytes", C.c_size_t),
+                ("clock_khz", C.c_int)]
+
+
+class KeccakfDesc(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("input_stride", C.c_uint32), ("output_stride", C.c_uint32)]
+
+
+class OneBlockDesc(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("input_stride", C.c_uint32), ("input_length", C.c_uint32),
+                ("output_stride", C.c_uint32)]
+
+
+class MerkleDesc(C.Structure):
+    _fields_ = [("leaf_count", C.c_uint32), ("leaf_stride", C.c_uint32), ("leaf_length", C.c_uint32),
+                ("subtree_log", C.c_uint32)]
+
+
+class ProverDesc(C.Structure):
+    _fields_ = [("log_n", C.c_uint32), ("rounds", C.c_uint32), ("log_blowup", C.c_uint32),
+                ("query_count", C.c_uint32), ("folding_step", C.c_uint32), ("grinding_bits", C.c_uint32),
+                ("public_input_digest", C.c_uint8 * 32)]
+
+
+class PhaseTimes(C.Structure):
+    _fields_ = [("fft_ns", C.c_uint64), ("commit0_ns", C.c_uint64), ("fold_ns", C.c_uint64),
+                ("tree_ns", C.c_uint64), ("transcript_ns", C.c_uint64), ("emit_ns", C.c_uint64),
+                ("total_ns", C.c_uint64)]
+
+
+_lib = None
+vp = C.c_void_p
+sz = C.c_size_t
+u32 = C.c_uint32
+
+
+def declared_symbols() -> list[str]:
+    """Every function name declared in include/azk.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(azk_[a-z0-9_]+)\s*\(", text)))
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AZKError(9, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    st = C.c_int
+    sp = C.POINTER(Stats)
+    sigs = {
+        "azk_last_error": (C.c_char_p, []), "azk_version": (C.c_char_p, []),
+        "azk_context_create": (st, [C.c_int, C.POINTER(vp)]), "azk_context_destroy": (None, [vp]),
+        "azk_context_capabilities": (st, [vp, C.POINTER(Capabilities)]), "azk_context_synchronize": (st, [vp]),
+        "azk_buffer_alloc": (st, [vp, sz, C.POINTER(vp)]), "azk_buffer_wrap": (st, [vp, vp, sz, C.POINTER(vp)]),
+        "azk_buffer_free": (None, [vp]), "azk_buffer_size": (sz, [vp]), "azk_buffer_device_ptr": (vp, [vp]),
+        "azk_buffer_upload": (st, [vp, sz, vp, sz]), "azk_buffer_download": (st, [vp, sz, vp, sz]),
+        "azk_keccakf_plan_create": (st, [vp, C.POINTER(KeccakfDesc), C.POINTER(vp)]),
+        "azk_keccakf_plan_destroy": (None, [vp]),
+        "azk_keccakf_permute": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_keccakf_permute_resident": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_keccakf_permute_verified": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_keccakf_plan_clear": (st, [vp]),
+        "azk_sha3_256_plan_create": (st, [vp, C.POINTER(OneBlockDesc), C.POINTER(vp)]),
+        "azk_keccak_256_plan_create": (st, [vp, C.POINTER(OneBlockDesc), C.POINTER(vp)]),
+        "azk_hash_plan_destroy": (None, [vp]),
+        "azk_hash_execute": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_hash_execute_resident": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_hash_execute_verified": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_hash_plan_clear": (st, [vp]),
+        "azk_merkle_plan_create": (st, [vp, C.POINTER(MerkleDesc), C.POINTER(vp)]),
+        "azk_merkle_plan_destroy": (None, [vp]),
+        "azk_merkle_commit": (st, [vp, vp, sz, vp, sp]),
+        "azk_merkle_commit_resident": (st, [vp, vp, sz, vp, sp]),
+        "azk_merkle_commit_verified": (st, [vp, vp, sz, vp, sp]),
+        "azk_merkle_open": (st, [vp, u32, vp, sz, vp]),
+        "azk_merkle_plan_clear": (st, [vp]),
+        "azk_prehashed_plan_create": (st, [vp, u32, C.POINTER(vp)]),
+        "azk_prehashed_plan_destroy": (None, [vp]),
+        "azk_prehashed_commit_resident": (st, [vp, vp, sz, vp, sp]),
+        "azk_codeword_plan_create": (st, [vp, u32, C.POINTER(vp)]),
+        "azk_codeword_plan_destroy": (None, [vp]),
+        "azk_codeword_execute_resident": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_codeword_execute": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_codeword_read_twiddles": (st, [vp, vp, sz]),
+        "azk_fold_plan_create": (st, [vp, u32, u32, C.POINTER(vp)]),
+        "azk_fold_plan_destroy": (None, [vp]),
+        "azk_fold_execute_resident": (st, [vp, vp, sz, vp, vp, sz, sp]),
+        "azk_fold_execute": (st, [vp, vp, sz, vp, vp, sz, sp]),
+        "azk_circle_prover_create": (st, [vp, C.POINTER(ProverDesc), C.POINTER(vp)]),
+        "azk_circle_prover_destroy": (None, [vp]),
+        "azk_circle_prover_proof_size": (sz, [vp]),
+        "azk_circle_prove_fft_coefficients_resident": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_circle_prove_evaluations_resident": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_circle_prove_fft_coefficients": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_circle_prover_clear": (st, [vp]),
+        "azk_circle_prover_set_graph": (st, [vp, C.c_int]),
+        "azk_circle_prover_set_timing": (st, [vp, C.c_int]),
+        "azk_circle_prover_phase_times": (st, [vp, C.POINTER(PhaseTimes)]),
+        "azk_circle_prove_batch": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp),
+                                        C.POINTER(sz), C.POINTER(sz), sp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ck(rc: int) -> None:
+    if rc != 0:
+        raise AZKError(rc, load().azk_last_error().decode())
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _host(a) -> np.ndarray:
+    if isinstance(a, (bytes, bytearray)):
+        a = np.frombuffer(bytes(a), np.uint8)
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint8).reshape(-1) if a.size else np.zeros(1, np.uint8)
+
+
+def _stats_dict(s: Stats) -> dict:
+    return {"wall_ns": s.wall_ns, "gpu_ns": s.gpu_ns, "submit_ns": s.submit_ns}
+
+
+class Context:
+    """MetalContext (MetalContext.swift:76-112): one device, one stream."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = vp()
+        _ck(lib.azk_context_create(device, C.byref(h)))
+        self.h = h
+        self._lib = lib
+
+    def capabilities(self) -> dict:
+        c = Capabilities()
+        _ck(self._lib.azk_context_capabilities(self.h, C.byref(c)))
+        return {"name": c.name.decode(), "arch": c.arch.decode(), "compute_units": c.compute_units,
+                "wavefront_size": c.wavefront_size, "lds_bytes": c.lds_bytes_per_workgroup,
+                "hbm_bytes": c.hbm_bytes, "clock_khz": c.clock_khz}
+
+    def synchronize(self) -> None:
+        _ck(self._lib.azk_context_synchronize(self.h))
+
+    def buffer(self, nbytes: int) -> "Buffer":
+        return Buffer(self, nbytes)
+
+    def upload(self, data) -> "Buffer":
+        a = _host(data)
+        b = Buffer(self, a.size)
+        b.upload(a)
+        return b
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.azk_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Buffer:
+    """Resident HBM buffer (MTLBuffer + offset pairs in the reference)."""
+
+    def __init__(self, ctx: Context, nbytes: int, device_ptr: int | None = None):
+        self.ctx = ctx
+        h = vp()
+        if device_ptr is None:
+            _ck(ctx._lib.azk_buffer_alloc(ctx.h, nbytes, C.byref(h)))
+        else:
+            _ck(ctx._lib.azk_buffer_wrap(ctx.h, device_ptr, nbytes, C.byref(h)))
+        self.h = h
+        self.nbytes = nbytes
+
+    @property
+    def device_ptr(self) -> int:
+        return self.ctx._lib.azk_buffer_device_ptr(self.h)
+
+    def upload(self, data, offset: int = 0) -> None:
+        a = _host(data)
+        _ck(self.ctx._lib.azk_buffer_upload(self.h, offset, _ptr(a), a.size if len(data) else 0))
+
+    def download(self, nbytes: int | None = None, offset: int = 0) -> np.ndarray:
+        n = self.nbytes - offset if nbytes is None else nbytes
+        out = np.zeros(max(n, 1), np.uint8)
+        _ck(self.ctx._lib.azk_buffer_download(self.h, offset, _ptr(out), n))
+        return out[:n]
+
+    def free(self) -> None:
+        if self.h:
+            self.ctx._lib.azk_buffer_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _Plan:
+    _destroy = ""
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            getattr(self.ctx._lib, self._destroy)(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class KeccakF1600PermutationBatchPlan(_Plan):
+    """KeccakF1600PermutationBatcher.swift:73-294."""
+    _destroy = "azk_keccakf_plan_destroy"
+
+    def __init__(self, ctx: Context, count: int, input_stride: int = 200, output_stride: int = 200):
+        self.ctx, self.count, self.input_stride, self.output_stride = ctx, count, input_stride, output_stride
+        h = vp()
+        _ck(ctx._lib.azk_keccakf_plan_create(ctx.h, C.byref(KeccakfDesc(count, input_stride, output_stride)), C.byref(h)))
+        self.h = h
+        self.last_stats: dict = {}
+
+    def permute(self, states, verified: bool = False) -> np.ndarray:
+        a = _host(states)
+        out = np.zeros(self.count * self.output_stride, np.uint8)
+        s = Stats()
+        fn = self.ctx._lib.azk_keccakf_permute_verified if verified else self.ctx._lib.azk_keccakf_permute
+        _ck(fn(self.h, _ptr(a), a.size, _ptr(out), out.size, C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return out
+
+    def permute_resident(self, inp: Buffer, out: Buffer, in_offset: int = 0, out_offset: int = 0) -> dict:
+        s = Stats()
+        _ck(self.ctx._lib.azk_keccakf_permute_resident(self.h, inp.h, in_offset, out.h, out_offset, C.byref(s)))
+        return _stats_dict(s)
+
+    def clear(self) -> None:
+        _ck(self.ctx._lib.azk_keccakf_plan_clear(self.h))
+
+
+class _OneBlockPlan(_Plan):
+    _destroy = "azk_hash_plan_destroy"
+    _create = ""
+
+    def __init__(self, ctx: Context, count: int, input_stride: int, input_length: int, output_stride: int = 32):
+        self.ctx, self.count = ctx, count
+        self.input_stride, self.input_length, self.output_stride = input_stride, input_length, output_stride
+        h = vp()
+        d = OneBlockDesc(count, input_stride, input_length, output_stride)
+        _ck(getattr(ctx._lib, self._create)(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.last_stats: dict = {}
+
+    def hash(self, messages, verified: bool = False) -> np.ndarray:
+        a = _host(messages)
+        out = np.zeros(max(self.count * self.output_stride, 1), np.uint8)
+        s = Stats()
+        fn = self.ctx._lib.azk_hash_execute_verified if verified else self.ctx._lib.azk_hash_execute
+        _ck(fn(self.h, _ptr(a), a.size if self.count * self.input_stride else 0, _ptr(out),
+               self.count * self.output_stride, C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return out[: self.count * self.output_stride]
+
+    def hash_resident(self, inp: Buffer, out: Buffer, in_offset: int = 0, out_offset: int = 0) -> dict:
+        s = Stats()
+        _ck(self.ctx._lib.azk_hash_execute_resident(self.h, inp.h, in_offset, out.h, out_offset, C.byref(s)))
+        return _stats_dict(s)
+
+    def clear(self) -> None:
+        _ck(self.ctx._lib.azk_hash_plan_clear(self.h))
+
+
+class SHA3FixedOneBlockHashPlan(_OneBlockPlan):
+    """SHA3BatchHasher.swift:81-264."""
+    _create = "azk_sha3_256_plan_create"
+
+
+class Keccak256FixedOneBlockHashPlan(_OneBlockPlan):
+    """Keccak256BatchHasher.swift:73-257."""
+    _create = "azk_keccak_256_plan_create"
+
+
+class SHA3RawLeavesMerkleCommitPlan(_Plan):
+    """MerkleCommitter.swift:526-1182 (tree kept resident; openings gathered from it)."""
+    _destroy = "azk_merkle_plan_destroy"
+
+    def __init__(self, ctx: Context, leaf_count: int, leaf_stride: int, leaf_length: int, subtree_log: int = 0):
+        self.ctx, self.leaf_count, self.leaf_stride, self.leaf_length = ctx, leaf_count, leaf_stride, leaf_length
+        h = vp()
+        _ck(ctx._lib.azk_merkle_plan_create(ctx.h, C.byref(MerkleDesc(leaf_count, leaf_stride, leaf_length, subtree_log)),
+                                            C.byref(h)))
+        self.h = h
+        self.depth = leaf_count.bit_length() - 1
+        self.last_stats: dict = {}
+
+    def commit(self, leaves, verified: bool = False) -> bytes:
+        a = _host(leaves)
+        root = np.zeros(32, np.uint8)
+        s = Stats()
+        fn = self.ctx._lib.azk_merkle_commit_verified if verified else self.ctx._lib.azk_merkle_commit
+        _ck(fn(self.h, _ptr(a), a.size if self.leaf_count * self.leaf_stride else 0, _ptr(root), C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return root.tobytes()
+
+    def commit_resident(self, leaves: Buffer, offset: int = 0) -> bytes:
+        root = np.zeros(32, np.uint8)
+        s = Stats()
+        _ck(self.ctx._lib.azk_merkle_commit_resident(self.h, leaves.h, offset, _ptr(root), C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return root.tobytes()
+
+    def open(self, index: int) -> tuple[bytes, bytes]:
+        sib = np.zeros(max(32 * self.depth, 1), np.uint8)
+        root = np.zeros(32, np.uint8)
+        _ck(self.ctx._lib.azk_merkle_open(self.h, index, _ptr(sib), 32 * self.depth, _ptr(root)))
+        return sib[: 32 * self.depth].tobytes(), root.tobytes()
+
+    def clear(self) -> None:
+        _ck(self.ctx._lib.azk_merkle_plan_clear(self.h))
+
+
+class SHA3PrehashedLeavesMerkleCommitPlan(_Plan):
+    """MerkleCommitter.swift:314-524."""
+    _destroy = "azk_prehashed_plan_destroy"
+
+    def __init__(self, ctx: Context, leaf_count: int):
+        self.ctx = ctx
+        h = vp()
+        _ck(ctx._lib.azk_prehashed_plan_create(ctx.h, leaf_count, C.byref(h)))
+        self.h = h
+
+    def commit_resident(self, leaf_hashes: Buffer, offset: int = 0) -> bytes:
+        root = np.zeros(32, np.uint8)
+        s = Stats()
+        _ck(self.ctx._lib.azk_prehashed_commit_resident(self.h, leaf_hashes.h, offset, _ptr(root), C.byref(s)))
+        return root.tobytes()
+
+
+class CircleCodewordPlan(_Plan):
+    """CircleCodeword.swift:411-887 (input: interleaved line-basis coefficients)."""
+    _destroy = "azk_codeword_plan_destroy"
+
+    def __init__(self, ctx: Context, log_n: int):
+        self.ctx, self.log_n = ctx, log_n
+        h = vp()
+        _ck(ctx._lib.azk_codeword_plan_create(ctx.h, log_n, C.byref(h)))
+        self.h = h
+        self.last_stats: dict = {}
+
+    def execute(self, coeffs: np.ndarray) -> np.ndarray:
+        a = _host(np.ascontiguousarray(coeffs, dtype=np.uint32))
+        out = np.zeros(4 << self.log_n, np.uint32)
+        s = Stats()
+        _ck(self.ctx._lib.azk_codeword_execute(self.h, _ptr(a), a.size, _ptr(out), out.nbytes, C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return out
+
+    def execute_resident(self, coeffs: Buffer, out: Buffer, coeff_offset: int = 0, out_offset: int = 0) -> dict:
+        s = Stats()
+        _ck(self.ctx._lib.azk_codeword_execute_resident(self.h, coeffs.h, coeff_offset, out.h, out_offset, C.byref(s)))
+        return _stats_dict(s)
+
+    def read_twiddles(self) -> np.ndarray:
+        n = (1 << self.log_n) - 1
+        out = np.zeros(max(n, 1), np.uint32)
+        _ck(self.ctx._lib.azk_codeword_read_twiddles(self.h, _ptr(out), n))
+        return out[:n]
+
+
+class CircleFRIFoldChainPlan(_Plan):
+    """CircleFRIFoldPlan.swift:5-280 (explicit challenges)."""
+    _destroy = "azk_fold_plan_destroy"
+
+    def __init__(self, ctx: Context, log_n: int, rounds: int):
+        self.ctx, self.log_n, self.rounds = ctx, log_n, rounds
+        h = vp()
+        _ck(ctx._lib.azk_fold_plan_create(ctx.h, log_n, rounds, C.byref(h)))
+        self.h = h
+
+    def execute(self, evals: np.ndarray, challenges) -> np.ndarray:
+        a = _host(np.ascontiguousarray(evals, dtype=np.uint32))
+        ch = np.ascontiguousarray(np.asarray(challenges, dtype=np.uint32).reshape(-1))
+        out = np.zeros(4 << (self.log_n - self.rounds), np.uint32)
+        s = Stats()
+        _ck(self.ctx._lib.azk_fold_execute(self.h, _ptr(a), a.size, _ptr(ch), _ptr(out), out.nbytes, C.byref(s)))
+        return out
+
+    def execute_resident(self, evals: Buffer, challenges, out: Buffer, evals_offset: int = 0, out_offset: int = 0) -> dict:
+        ch = np.ascontiguousarray(np.asarray(challenges, dtype=np.uint32).reshape(-1))
+        s = Stats()
+        _ck(self.ctx._lib.azk_fold_execute_resident(self.h, evals.h, evals_offset, _ptr(ch), out.h, out_offset, C.byref(s)))
+        return _stats_dict(s)
+
+
+class CircleCodewordPCSFRIProverV1(_Plan):
+    """CircleCodeword.swift:920-1419 + CirclePCSFRIResidentProverV1 (CircleFRIFoldPlan.swift:693-1025)."""
+    _destroy = "azk_circle_prover_destroy"
+
+    def __init__(self, ctx: Context, log_n: int, rounds: int, security=(2, 32, 1, 0), public_input_digest: bytes = b""):
+        self.ctx, self.log_n, self.rounds = ctx, log_n, rounds
+        d = ProverDesc()
+        d.log_n, d.rounds = log_n, rounds
+        d.log_blowup, d.query_count, d.folding_step, d.grinding_bits = security
+        dg = public_input_digest or bytes(32)
+        if len(dg) != 32:
+            raise AZKError(6, "public input digest must be 32 bytes")
+        d.public_input_digest[:] = list(dg)
+        h = vp()
+        _ck(ctx._lib.azk_circle_prover_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.proof_size = int(ctx._lib.azk_circle_prover_proof_size(h))
+        self._out = np.zeros(self.proof_size, np.uint8)
+        self.last_stats: dict = {}
+
+    def _finish(self, rc, n, s) -> bytes:
+        _ck(rc)
+        self.last_stats = _stats_dict(s)
+        return self._out[: n.value].tobytes()
+
+    def prove_fft_coefficients_resident(self, coeffs: Buffer, offset: int = 0) -> bytes:
+        n, s = C.c_size_t(0), Stats()
+        rc = self.ctx._lib.azk_circle_prove_fft_coefficients_resident(self.h, coeffs.h, offset, _ptr(self._out),
+                                                                     self.proof_size, C.byref(n), C.byref(s))
+        return self._finish(rc, n, s)
+
+    def prove_evaluations_resident(self, evals: Buffer, offset: int = 0) -> bytes:
+        n, s = C.c_size_t(0), Stats()
+        rc = self.ctx._lib.azk_circle_prove_evaluations_resident(self.h, evals.h, offset, _ptr(self._out),
+                                                                self.proof_size, C.byref(n), C.byref(s))
+        return self._finish(rc, n, s)
+
+    def prove_fft_coefficients(self, coeffs: np.ndarray) -> bytes:
+        a = _host(np.ascontiguousarray(coeffs, dtype=np.uint32))
+        n, s = C.c_size_t(0), Stats()
+        rc = self.ctx._lib.azk_circle_prove_fft_coefficients(self.h, _ptr(a), a.size, _ptr(self._out), self.proof_size,
+                                                            C.byref(n), C.byref(s))
+        return self._finish(rc, n, s)
+
+    def set_graph(self, enable: bool) -> None:
+        _ck(self.ctx._lib.azk_circle_prover_set_graph(self.h, int(enable)))
+
+    def set_timing(self, enable: bool) -> None:
+        _ck(self.ctx._lib.azk_circle_prover_set_timing(self.h, int(enable)))
+
+    def phase_times(self) -> dict:
+        t = PhaseTimes()
+        _ck(self.ctx._lib.azk_circle_prover_phase_times(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in PhaseTimes._fields_}
+
+    def clear(self) -> None:
+        _ck(self.ctx._lib.azk_circle_prover_clear(self.h))
+
+
+def prove_batch(provers: list, coeff_buffers: list, offsets=None) -> tuple[list[bytes], dict]:
+    """Many independent commits in one captured graph (azk_circle_prove_batch)."""
+    n = len(provers)
+    lib = load()
+    offs = (sz * n)(*(offsets or [0] * n))
+    outs = [np.zeros(p.proof_size, np.uint8) for p in provers]
+    ph = (vp * n)(*[p.h for p in provers])
+    bh = (vp * n)(*[b.h for b in coeff_buffers])
+    op = (vp * n)(*[o.ctypes.data for o in outs])
+    caps = (sz * n)(*[p.proof_size for p in provers])
+    lens = (sz * n)()
+    s = Stats()
+    _ck(lib.azk_circle_prove_batch(ph, bh, offs, n, op, caps, lens, C.byref(s)))
+    return [outs[i][: lens[i]].tobytes() for i in range(n)], _stats_dict(s)

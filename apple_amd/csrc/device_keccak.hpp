@@ -1,0 +1,213 @@
+// Keccak-f[1600] and fixed-shape SHA3-256 sponges for gfx950.
+//
+// Two forms of the same permutation:
+//  * kf_v: one state per lane, each 64-bit lane as a VGPR pair.  Rotations are
+//    v_alignbit_b32 pairs, the 5-way column parities and chi are v_bitop3_b32
+//    (truth tables 0x96 = xor3, chi via the compiler's own bitop3 formation):
+//    ~190 VALU ops per round, 4555 per permutation.
+//  * kf_s: wave-uniform state; the compiler keeps it in SGPRs and runs 64-bit
+//    SALU ops.  Used where one serial sponge sits on the critical path (the
+//    Fiat-Shamir transcript): its latency is several times lower than one VALU wave.
+// Semantics follow the reference's SHA3 oracle (Sources/AppleZKProver/SHA3Oracle.swift:
+// 181-280) and its Metal keccak_f1600 (Resources/HashMerkleKernels.metal:308-348);
+// round constants and rho offsets are the FIPS 202 tables.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace azk {
+
+// ---------------------------------------------------------------- constants
+__constant__ static const uint32_t kRcLo[24] = {
+    0x00000001u, 0x00008082u, 0x0000808Au, 0x80008000u, 0x0000808Bu, 0x80000001u,
+    0x80008081u, 0x00008009u, 0x0000008Au, 0x00000088u, 0x80008009u, 0x8000000Au,
+    0x8000808Bu, 0x0000008Bu, 0x00008089u, 0x00008003u, 0x00008002u, 0x00000080u,
+    0x0000800Au, 0x8000000Au, 0x80008081u, 0x00008080u, 0x80000001u, 0x80008008u};
+__constant__ static const uint32_t kRcHi[24] = {
+    0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x00000000u,
+    0x80000000u, 0x80000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u,
+    0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u,
+    0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x80000000u};
+
+// ---------------------------------------------------------------- VALU form
+struct u2 { uint32_t lo, hi; };
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+__device__ __forceinline__ u2 x2(u2 a, u2 b) { return {a.lo ^ b.lo, a.hi ^ b.hi}; }
+__device__ __forceinline__ u2 x5(u2 a, u2 b, u2 c, u2 d, u2 e)
+{
+    return {xor3(xor3(a.lo, b.lo, c.lo), d.lo, e.lo), xor3(xor3(a.hi, b.hi, c.hi), d.hi, e.hi)};
+}
+// 64-bit rotate-left by a compile-time n, as alignbit pairs (n = 32 is a swap).
+template <int N>
+__device__ __forceinline__ u2 rot(u2 v)
+{
+    if constexpr (N == 0) return v;
+    else if constexpr (N == 32) return {v.hi, v.lo};
+    else if constexpr (N < 32)
+        return {__builtin_amdgcn_alignbit(v.lo, v.hi, 32 - N), __builtin_amdgcn_alignbit(v.hi, v.lo, 32 - N)};
+    else
+        return {__builtin_amdgcn_alignbit(v.hi, v.lo, 64 - N), __builtin_amdgcn_alignbit(v.lo, v.hi, 64 - N)};
+}
+__device__ __forceinline__ u2 chi(u2 a, u2 b, u2 c) { return {a.lo ^ (~b.lo & c.lo), a.hi ^ (~b.hi & c.hi)}; }
+
+__device__ __forceinline__ void kround_v(u2 (&a)[25], uint32_t rclo, uint32_t rchi)
+{
+    u2 c0 = x5(a[0], a[5], a[10], a[15], a[20]);
+    u2 c1 = x5(a[1], a[6], a[11], a[16], a[21]);
+    u2 c2 = x5(a[2], a[7], a[12], a[17], a[22]);
+    u2 c3 = x5(a[3], a[8], a[13], a[18], a[23]);
+    u2 c4 = x5(a[4], a[9], a[14], a[19], a[24]);
+    u2 d0 = x2(c4, rot<1>(c1)), d1 = x2(c0, rot<1>(c2)), d2 = x2(c1, rot<1>(c3));
+    u2 d3 = x2(c2, rot<1>(c4)), d4 = x2(c3, rot<1>(c0));
+    // theta + rho + pi: B[y, 2x+3y] = rot(A[x, y] ^ D[x], r[x, y])
+    u2 b[25];
+    b[0] = x2(a[0], d0);
+    b[10] = rot<1>(x2(a[1], d1));
+    b[20] = rot<62>(x2(a[2], d2));
+    b[5] = rot<28>(x2(a[3], d3));
+    b[15] = rot<27>(x2(a[4], d4));
+    b[16] = rot<36>(x2(a[5], d0));
+    b[1] = rot<44>(x2(a[6], d1));
+    b[11] = rot<6>(x2(a[7], d2));
+    b[21] = rot<55>(x2(a[8], d3));
+    b[6] = rot<20>(x2(a[9], d4));
+    b[7] = rot<3>(x2(a[10], d0));
+    b[17] = rot<10>(x2(a[11], d1));
+    b[2] = rot<43>(x2(a[12], d2));
+    b[12] = rot<25>(x2(a[13], d3));
+    b[22] = rot<39>(x2(a[14], d4));
+    b[23] = rot<41>(x2(a[15], d0));
+    b[8] = rot<45>(x2(a[16], d1));
+    b[18] = rot<15>(x2(a[17], d2));
+    b[3] = rot<21>(x2(a[18], d3));
+    b[13] = rot<8>(x2(a[19], d4));
+    b[14] = rot<18>(x2(a[20], d0));
+    b[24] = rot<2>(x2(a[21], d1));
+    b[9] = rot<61>(x2(a[22], d2));
+    b[19] = rot<56>(x2(a[23], d3));
+    b[4] = rot<14>(x2(a[24], d4));
+#pragma unroll
+    for (int y = 0; y < 25; y += 5) {
+        u2 t0 = b[y], t1 = b[y + 1], t2 = b[y + 2], t3 = b[y + 3], t4 = b[y + 4];
+        a[y] = chi(t0, t1, t2);
+        a[y + 1] = chi(t1, t2, t3);
+        a[y + 2] = chi(t2, t3, t4);
+        a[y + 3] = chi(t3, t4, t0);
+        a[y + 4] = chi(t4, t0, t1);
+    }
+    a[0].lo ^= rclo;
+    a[0].hi ^= rchi;
+}
+
+// First and last rounds are peeled so sparse inputs constant-fold and unused
+// output lanes are dropped; the middle 22 rounds stay rolled (two per trip) so a
+// kernel can hold several call sites without blowing the instruction cache.
+__device__ __forceinline__ void kf_v(u2 (&a)[25])
+{
+    kround_v(a, 0x00000001u, 0u);
+#pragma unroll 2
+    for (int r = 1; r < 23; ++r) kround_v(a, kRcLo[r], kRcHi[r]);
+    kround_v(a, 0x80008008u, 0x80000000u);
+}
+
+// ---------------------------------------------------------------- SALU form
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int n) { return n == 0 ? x : ((x << n) | (x >> (64 - n))); }
+
+__device__ __forceinline__ void kround_s(uint64_t (&a)[25], uint64_t rc)
+{
+    uint64_t c[5], d[5], b[25];
+#pragma unroll
+    for (int x = 0; x < 5; ++x) c[x] = a[x] ^ a[x + 5] ^ a[x + 10] ^ a[x + 15] ^ a[x + 20];
+#pragma unroll
+    for (int x = 0; x < 5; ++x) d[x] = c[(x + 4) % 5] ^ rotl64(c[(x + 1) % 5], 1);
+    constexpr int R[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
+#pragma unroll
+    for (int x = 0; x < 5; ++x)
+#pragma unroll
+        for (int y = 0; y < 5; ++y) b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(a[x + 5 * y] ^ d[x], R[x + 5 * y]);
+#pragma unroll
+    for (int y = 0; y < 25; y += 5)
+#pragma unroll
+        for (int x = 0; x < 5; ++x) a[y + x] = b[y + x] ^ (~b[y + (x + 1) % 5] & b[y + (x + 2) % 5]);
+    a[0] ^= rc;
+}
+
+__device__ __forceinline__ void kf_s(uint64_t (&a)[25])
+{
+#pragma unroll 1
+    for (int r = 0; r < 24; ++r) kround_s(a, (uint64_t)kRcLo[r] | ((uint64_t)kRcHi[r] << 32));
+}
+
+// ---------------------------------------------------------------- sponges
+__device__ __forceinline__ void zero_state(u2 (&a)[25])
+{
+#pragma unroll
+    for (int i = 0; i < 25; ++i) a[i] = {0u, 0u};
+}
+
+// SHA3-256 of a 16-byte QM31 leaf (LE u32 a, b, c, d): one permutation.
+// Reference: sha3_256_absorb_qm31_leaf, Resources/HashMerkleKernels.metal:733-740.
+__device__ __forceinline__ void sha3_leaf16(const uint32_t v[4], uint32_t out[8])
+{
+    u2 a[25];
+    zero_state(a);
+    a[0] = {v[0], v[1]};
+    a[1] = {v[2], v[3]};
+    a[2] = {0x06u, 0u};
+    a[16] = {0u, 0x80000000u};
+    kf_v(a);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { out[2 * i] = a[i].lo; out[2 * i + 1] = a[i].hi; }
+}
+
+// SHA3-256(left || right), two 32-byte digests: one permutation (Merkle parent).
+// Reference: MerkleOracle.swift:16-26, HashMerkleKernels.metal:1082-1098.
+__device__ __forceinline__ void sha3_parent(const uint32_t l[8], const uint32_t r[8], uint32_t out[8])
+{
+    u2 a[25];
+    zero_state(a);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        a[i] = {l[2 * i], l[2 * i + 1]};
+        a[4 + i] = {r[2 * i], r[2 * i + 1]};
+    }
+    a[8] = {0x06u, 0u};
+    a[16] = {0u, 0x80000000u};
+    kf_v(a);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { out[2 * i] = a[i].lo; out[2 * i + 1] = a[i].hi; }
+}
+
+// One-block sponge over 34 message words (already zeroed beyond `len` bytes),
+// len <= 136; a 136-byte message takes a second, padding-only block.
+// Reference: SHA3Oracle.swift:181-219 with the one-block bound of
+// SHA3BatchHasher.swift:161-166; Metal absorb_fixed_136 HashMerkleKernels.metal:633-657.
+__device__ __forceinline__ void sponge_oneblock(const uint32_t (&w)[34], uint32_t len, uint32_t suffix, uint32_t out[8])
+{
+    u2 a[25];
+    zero_state(a);
+#pragma unroll
+    for (int i = 0; i < 17; ++i) a[i] = {w[2 * i], w[2 * i + 1]};
+    if (len == 136) {
+        kf_v(a);
+        a[0].lo ^= suffix;
+    } else {
+        // byte len ^= suffix
+        uint32_t word = len >> 2, sh = (len & 3u) * 8u;
+#pragma unroll
+        for (int i = 0; i < 17; ++i) {
+            if (word == (uint32_t)(2 * i)) a[i].lo ^= suffix << sh;
+            if (word == (uint32_t)(2 * i + 1)) a[i].hi ^= suffix << sh;
+        }
+    }
+    a[16].hi ^= 0x80000000u;
+    kf_v(a);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { out[2 * i] = a[i].lo; out[2 * i + 1] = a[i].hi; }
+}
+
+} // namespace azk

@@ -1,0 +1,643 @@
+// gfx950 kernels for the Circle-PCS commit path.
+//
+// Reference kernels these replace (Sources/AppleZKProver/Resources/HashMerkleKernels.metal):
+//   keccak_f1600_permutation_scalar/simdgroup :428-478   -> k_keccakf_batch
+//   sha3_256/keccak_256_oneblock_specialized  :791-990   -> k_oneblock_hash
+//   sha3_256_merkle_treelet_leaves/parents/fuse_upper :1082-1218 -> k_merkle_raw_tiles,
+//                                                        k_tree_reduce
+//   circle_codeword_fft_stage(+_leaf_hash)    :1881-1943 -> k_fft_pass, k_fft_final
+//   qm31_fri_fold(_challenge_buffer)          :1987-2032 -> k_fold
+//   transcript_absorb_keccak / squeeze        :1324-1404 -> k_transcript
+//   circle domain / twiddle materialization   :1781-1879 -> k_fft_twiddles, k_inverse
+//   extract_sibling + query extraction        :1149-1165 -> k_emit_proof
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_field.hpp"
+#include "device_keccak.hpp"
+#include "kernels.hpp"
+
+namespace azk {
+
+// ============================================================== batch hashing
+
+// One Keccak-f[1600] per lane over strided 25 x u64 states; output padding zeroed.
+__global__ __launch_bounds__(256) void k_keccakf_batch(const uint8_t *__restrict__ in, uint32_t in_stride,
+                                                       uint8_t *__restrict__ out, uint32_t out_stride, uint32_t count)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint2 *src = reinterpret_cast<const uint2 *>(in + (size_t)i * in_stride);
+    u2 a[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+        uint2 v = src[k];
+        a[k] = {v.x, v.y};
+    }
+    kf_v(a);
+    uint2 *dst = reinterpret_cast<uint2 *>(out + (size_t)i * out_stride);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) dst[k] = make_uint2(a[k].lo, a[k].hi);
+    for (uint32_t b = 200; b < out_stride; b += 8) *reinterpret_cast<uint2 *>(out + (size_t)i * out_stride + b) = make_uint2(0, 0);
+}
+
+// Loads the first `len` bytes of a message into 34 LE words, zero beyond.
+template <bool kAligned>
+__device__ __forceinline__ void load_message(const uint8_t *__restrict__ p, uint32_t len, uint32_t (&w)[34])
+{
+    if constexpr (kAligned) {
+        const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+#pragma unroll
+        for (int k = 0; k < 34; ++k) {
+            uint32_t lo = 4u * k;
+            uint32_t v = lo < len ? q[k] : 0u;
+            if (lo < len && len - lo < 4) v &= (1u << (8 * (len - lo))) - 1u;
+            w[k] = v;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 34; ++k) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                uint32_t idx = 4u * k + b;
+                if (idx < len) v |= (uint32_t)p[idx] << (8 * b);
+            }
+            w[k] = v;
+        }
+    }
+}
+
+// One-block SHA3-256 (suffix 0x06) / Keccak-256 (suffix 0x01) per lane.
+template <bool kAligned>
+__global__ __launch_bounds__(256) void k_oneblock_hash(const uint8_t *__restrict__ in, uint32_t stride, uint32_t len,
+                                                       uint8_t *__restrict__ out, uint32_t out_stride, uint32_t count,
+                                                       uint32_t suffix)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t w[34];
+    load_message<kAligned>(in + (size_t)i * stride, len, w);
+    uint32_t d[8];
+    sponge_oneblock(w, len, suffix, d);
+    uint8_t *o = out + (size_t)i * out_stride;
+    if ((reinterpret_cast<uintptr_t>(o) & 15u) == 0) {
+        reinterpret_cast<uint4 *>(o)[0] = make_uint4(d[0], d[1], d[2], d[3]);
+        reinterpret_cast<uint4 *>(o)[1] = make_uint4(d[4], d[5], d[6], d[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) o[k] = (uint8_t)(d[k / 4] >> (8 * (k % 4)));
+    }
+    for (uint32_t b = 32; b < out_stride; ++b) o[b] = 0;
+}
+
+// ============================================================== Merkle tiles
+
+// Node `idx` of level `l` in a full-tree array of a tree with `m` leaves:
+// levels are stored bottom-up, level l starting at node 2m - (2m >> l).
+__device__ __forceinline__ uint32_t *node_ptr(uint32_t *tree, uint64_t m, int l, uint64_t idx)
+{
+    return tree + 8 * ((2 * m - ((2 * m) >> l)) + idx);
+}
+
+__device__ __forceinline__ void st_node(uint32_t *p, const uint32_t (&d)[8])
+{
+    reinterpret_cast<uint4 *>(p)[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    reinterpret_cast<uint4 *>(p)[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+__device__ __forceinline__ void ld_node(const uint32_t *p, uint32_t (&d)[8])
+{
+    uint4 x = reinterpret_cast<const uint4 *>(p)[0], y = reinterpret_cast<const uint4 *>(p)[1];
+    d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
+}
+
+// Reduces `cnt` level-`l` nodes held in LDS region A (node q at words 8q..8q+7),
+// whose first node has global index `g` in that level, up to one node.  Every
+// computed level is written to the tree in HBM.  Ping-pong A -> B -> A ... with one
+// barrier per level (B holds cnt/2 nodes).
+__device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t *B, uint32_t cnt, int l, uint64_t g, uint32_t *tree,
+                                           uint64_t m)
+{
+    uint32_t *src = A, *dst = B;
+    while (cnt > 1) {
+        uint32_t half = cnt >> 1;
+        for (uint32_t q = threadIdx.x; q < half; q += blockDim.x) {
+            uint32_t lft[8], rgt[8], o[8];
+            ld_node(src + 16 * q, lft);
+            ld_node(src + 16 * q + 8, rgt);
+            sha3_parent(lft, rgt, o);
+            st_node(dst + 8 * q, o);
+            st_node(node_ptr(tree, m, l + 1, (g >> 1) + q), o);
+        }
+        __syncthreads();
+        uint32_t *t = src; src = dst; dst = t;
+        cnt = half;
+        ++l;
+        g >>= 1;
+    }
+}
+
+// Hashes the T QM31 leaves held in LDS region A (leaf i at words 4i..4i+3, global
+// leaf index g0 + i) and builds the tile's subtree up to its root, writing every
+// node.  Leaf pair p's 32 bytes are overwritten in place by its parent.
+__device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t *B, uint32_t T, uint64_t g0, uint32_t *tree,
+                                               uint64_t m)
+{
+    if (T == 1) {
+        uint32_t v[4] = {A[0], A[1], A[2], A[3]}, h[8];
+        if (threadIdx.x == 0) {
+            sha3_leaf16(v, h);
+            st_node(node_ptr(tree, m, 0, g0), h);
+        }
+        return;
+    }
+    for (uint32_t p = threadIdx.x; p < T / 2; p += blockDim.x) {
+        uint4 e0 = reinterpret_cast<const uint4 *>(A)[2 * p], e1 = reinterpret_cast<const uint4 *>(A)[2 * p + 1];
+        uint32_t v0[4] = {e0.x, e0.y, e0.z, e0.w}, v1[4] = {e1.x, e1.y, e1.z, e1.w};
+        uint32_t h0[8], h1[8], par[8];
+        sha3_leaf16(v0, h0);
+        sha3_leaf16(v1, h1);
+        st_node(node_ptr(tree, m, 0, g0 + 2 * p), h0);
+        st_node(node_ptr(tree, m, 0, g0 + 2 * p + 1), h1);
+        sha3_parent(h0, h1, par);
+        st_node(node_ptr(tree, m, 1, (g0 >> 1) + p), par);
+        st_node(A + 8 * p, par);
+    }
+    __syncthreads();
+    reduce_lds(A, B, T / 2, 1, g0 >> 1, tree, m);
+}
+
+// Raw-leaf Merkle tile: leaves of `len` <= 136 bytes at `stride`, T leaves per
+// block.  Reference: SHA3RawLeavesMerkleCommitPlan.encodeCommitmentRoot
+// (MerkleCommitter.swift:801-894) with the treelet kernel :1167-1218.
+template <bool kAligned>
+__global__ __launch_bounds__(256) void k_merkle_raw_tiles(const uint8_t *__restrict__ leaves, uint32_t stride,
+                                                          uint32_t len, uint32_t log_t, uint32_t *tree, uint64_t m)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t T = 1u << log_t;
+    uint32_t *A = lds, *B = lds + 4 * T;
+    const uint64_t g0 = (uint64_t)blockIdx.x * T;
+    if (T == 1) {
+        if (threadIdx.x == 0) {
+            uint32_t w[34], h[8];
+            load_message<kAligned>(leaves + g0 * stride, len, w);
+            sponge_oneblock(w, len, 0x06u, h);
+            st_node(node_ptr(tree, m, 0, g0), h);
+        }
+        return;
+    }
+    for (uint32_t p = threadIdx.x; p < T / 2; p += blockDim.x) {
+        uint32_t h[2][8];
+#pragma unroll 1
+        for (int s = 0; s < 2; ++s) {
+            uint32_t w[34];
+            load_message<kAligned>(leaves + (g0 + 2 * p + s) * (uint64_t)stride, len, w);
+            sponge_oneblock(w, len, 0x06u, h[s]);
+        }
+        uint32_t par[8];
+        st_node(node_ptr(tree, m, 0, g0 + 2 * p), h[0]);
+        st_node(node_ptr(tree, m, 0, g0 + 2 * p + 1), h[1]);
+        sha3_parent(h[0], h[1], par);
+        st_node(node_ptr(tree, m, 1, (g0 >> 1) + p), par);
+        st_node(A + 8 * p, par);
+    }
+    __syncthreads();
+    reduce_lds(A, B, T / 2, 1, g0 >> 1, tree, m);
+}
+
+// Prehashed leaves (32-byte digests) copied into the tree's level 0 and reduced.
+// Reference: SHA3PrehashedLeavesMerkleCommitPlan (MerkleCommitter.swift:314-524).
+__global__ __launch_bounds__(256) void k_tree_reduce(uint32_t *tree, uint64_t m, int l, uint32_t log_c)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t C = 1u << log_c;
+    uint32_t *A = lds, *B = lds + 8 * C;
+    const uint64_t g = (uint64_t)blockIdx.x * C;
+    for (uint32_t q = threadIdx.x; q < C; q += blockDim.x) {
+        uint32_t d[8];
+        ld_node(node_ptr(tree, m, l, g + q), d);
+        st_node(A + 8 * q, d);
+    }
+    __syncthreads();
+    reduce_lds(A, B, C, l, g, tree, m);
+}
+
+__global__ __launch_bounds__(256) void k_copy_words(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// ============================================================== Circle domain
+
+// FFT twiddles, stage-major (N - 1 words): stage 0 holds y, stage s >= 1 holds x
+// doubled s - 1 times, of the domain point at storage index h << (s + 1).
+// Reference: CircleCodewordOracle.circleFFTTwiddles (CircleCodeword.swift:129-171),
+// canonical domain CircleDomain.swift:225-239,273-285.
+__global__ __launch_bounds__(256) void k_fft_twiddles(uint32_t *out, uint32_t log_n)
+{
+    const uint64_t n = 1ull << log_n;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    // locate stage s with offset(s) = n - (n >> s) <= i < offset(s + 1)
+    uint32_t s = 0;
+    while (i >= n - (n >> (s + 1))) ++s;
+    uint64_t h = i - (n - (n >> s));
+    uint64_t storage = h << (s + 1);
+    // bit-reverse storage -> natural index
+    uint64_t nat = 0, v = storage;
+    for (uint32_t b = 0; b < log_n; ++b) { nat = (nat << 1) | (v & 1); v >>= 1; }
+    const uint32_t init = 1u << (31 - (log_n + 1)), step = 1u << (31 - (log_n - 1));
+    const uint64_t half = n >> 1;
+    uint32_t idx;
+    if (nat < half) idx = (uint32_t)((init + (uint64_t)step * nat) & 0x7fffffffu);
+    else idx = (uint32_t)((0x80000000u - ((init + (uint64_t)step * (nat - half)) & 0x7fffffffu)) & 0x7fffffffu);
+    cpoint p = cp_from_index(idx);
+    uint32_t w;
+    if (s == 0) w = p.y;
+    else {
+        w = p.x;
+        for (uint32_t k = 1; k < s; ++k) w = double_x(w);
+    }
+    out[i] = w;
+}
+
+// Elementwise M31 inverse (fold twiddles are the inverses of the first FFT stages:
+// inverse y for the circle fold, inverse x for the line folds, CircleDomain.swift:399-511).
+__global__ __launch_bounds__(256) void k_inverse(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = m31_inv(in[i]);
+}
+
+// ============================================================== Circle FFT
+
+__device__ __forceinline__ qm31 ld_qm(const uint4 &v) { return {v.x, v.y, v.z, v.w}; }
+__device__ __forceinline__ uint4 st_qm(qm31 q) { return make_uint4(q.a, q.b, q.c, q.d); }
+
+__device__ __forceinline__ void butterfly(uint4 *s, uint32_t i1, uint32_t i2, uint32_t w)
+{
+    qm31 l = ld_qm(s[i1]), r = qm_scale(ld_qm(s[i2]), w);
+    s[i1] = st_qm(qm_add(l, r));
+    s[i2] = st_qm(qm_sub(l, r));
+}
+
+// Stages lo + k - 1 .. lo of the Circle FFT on a tile of T = 2^(k + m) elements:
+// 2^k strided rows (stride 2^lo) x 2^m contiguous columns, staged through LDS.
+// Reference stage loop: CircleCodewordOracle.evaluateWithCircleFFT
+// (CircleCodeword.swift:72-98), Metal circle_codeword_fft_stage :1881-1908.
+__device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *__restrict__ tw, uint64_t n, uint32_t lo,
+                                                uint32_t k, uint32_t m, uint64_t high)
+{
+    const uint32_t T = 1u << (k + m);
+    const uint32_t cmask = (1u << m) - 1u;
+    for (int t = (int)k - 1; t >= 0; --t) {
+        const uint32_t sg = lo + (uint32_t)t;
+        const uint64_t off = sg ? n - (n >> sg) : 0;
+        const uint32_t tmask = (1u << t) - 1u;
+        for (uint32_t b = threadIdx.x; b < T / 2; b += blockDim.x) {
+            uint32_t c = b & cmask, rest = b >> m;
+            uint32_t jlo = rest & tmask, jhi = rest >> t;
+            uint32_t j = (jhi << (t + 1)) | jlo;
+            uint64_t h = (high << (k - (uint32_t)t - 1)) | jhi;
+            uint32_t w = tw[off + h];
+            butterfly(s, (j << m) | c, ((j + (1u << t)) << m) | c, w);
+        }
+        __syncthreads();
+    }
+}
+
+// Non-final pass (lo >= 1): tile index -> (high bits, column chunk).
+__global__ __launch_bounds__(256) void k_fft_pass(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                  const uint32_t *__restrict__ tw, uint32_t log_n, uint32_t lo,
+                                                  uint32_t k, uint32_t m)
+{
+    extern __shared__ uint4 s[];
+    const uint32_t T = 1u << (k + m);
+    const uint64_t n = 1ull << log_n;
+    const uint32_t chunk_bits = lo - m;
+    const uint64_t high = (uint64_t)blockIdx.x >> chunk_bits;
+    const uint64_t chunk = (uint64_t)blockIdx.x & ((1ull << chunk_bits) - 1);
+    const uint64_t base = (high << (lo + k)) | (chunk << m);
+    const uint32_t cmask = (1u << m) - 1u;
+    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x)
+        s[L] = src[base | ((uint64_t)(L >> m) << lo) | (L & cmask)];
+    __syncthreads();
+    fft_tile_stages(s, tw, n, lo, k, m, high);
+    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x)
+        dst[base | ((uint64_t)(L >> m) << lo) | (L & cmask)] = s[L];
+}
+
+// Final pass: stages log_t - 1 .. 0 on a contiguous tile of T = 2^log_t elements,
+// then the first-layer leaf hashes and the tile's Merkle subtree, as the reference
+// fuses stage 0 with the leaf hash (circle_codeword_fft_stage_leaf_hash :1910-1943)
+// and commits the hashes (SHA3PrehashedLeavesMerkleCommitPlan).
+// With run_fft == 0 the tile is only hashed (a codeword supplied by the caller).
+__global__ __launch_bounds__(256) void k_fft_final(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                   const uint32_t *__restrict__ tw, uint32_t log_n, uint32_t log_t,
+                                                   uint32_t run_fft, uint32_t *tree)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t T = 1u << log_t;
+    uint4 *s = reinterpret_cast<uint4 *>(lds);
+    const uint64_t n = 1ull << log_n;
+    const uint64_t base = (uint64_t)blockIdx.x * T;
+    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) s[L] = src[base + L];
+    __syncthreads();
+    if (run_fft) {
+        fft_tile_stages(s, tw, n, 0, log_t, 0, blockIdx.x);
+        for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) dst[base + L] = s[L];
+    }
+    if (tree) tile_tree_qm31(lds, lds + 4 * T, T, base, tree, n);
+}
+
+// ============================================================== FRI fold
+
+// Layer r (2 n_out elements) -> layer r + 1 (n_out elements) with the challenge
+// read from device memory, then (if `tree`) the new layer's leaf hashes and tile
+// subtrees.  Reference: qm31_fri_fold_challenge_buffer (:2013-2032) and
+// QM31FRIFoldOracle.fold (QM31FRIFold.swift:11-44).
+__global__ __launch_bounds__(256) void k_fold(const uint4 *__restrict__ in, uint4 *__restrict__ out,
+                                              const uint32_t *__restrict__ tw, const uint32_t *__restrict__ ch,
+                                              uint32_t log_t, uint32_t *tree, uint64_t n_out)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t T = 1u << log_t;
+    const uint64_t base = (uint64_t)blockIdx.x * T;
+    const qm31 c = {ch[0], ch[1], ch[2], ch[3]};
+    uint4 *s = reinterpret_cast<uint4 *>(lds);
+    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) {
+        uint64_t g = base + L;
+        qm31 o = fri_fold_pair(ld_qm(in[2 * g]), ld_qm(in[2 * g + 1]), tw[g], c);
+        uint4 ov = st_qm(o);
+        out[g] = ov;
+        if (tree) s[L] = ov;
+    }
+    if (!tree) return;
+    __syncthreads();
+    tile_tree_qm31(lds, lds + 4 * T, T, base, tree, n_out);
+}
+
+// ============================================================== transcript
+
+// Absorbs `len` bytes at `src` (8-byte aligned, readable up to the next multiple
+// of 8) as one SHA3-padded message.  Reference: TranscriptState.absorb
+// (SHA3Oracle.swift:19-39), Metal transcript_absorb_keccak :1324-1366.
+__device__ __forceinline__ void t_absorb(uint64_t (&a)[25], const uint64_t *src, uint64_t len, uint32_t suffix)
+{
+    uint64_t off = 0;
+    while (off + 136 <= len) {
+#pragma unroll
+        for (int i = 0; i < 17; ++i) a[i] ^= src[off / 8 + i];
+        kf_s(a);
+        off += 136;
+    }
+    const uint32_t tc = (uint32_t)(len - off);
+#pragma unroll
+    for (int i = 0; i < 17; ++i) {
+        uint32_t lo = 8u * i;
+        uint64_t w = 0;
+        if (lo < tc) {
+            w = src[off / 8 + i];
+            if (tc - lo < 8) w &= (1ull << (8 * (tc - lo))) - 1;
+        }
+        if (lo <= tc && tc < lo + 8) w ^= (uint64_t)suffix << (8 * (tc - lo));
+        if (i == 16) w ^= 0x8000000000000000ull;
+        a[i] ^= w;
+    }
+    kf_s(a);
+}
+
+__device__ __forceinline__ uint64_t pick_lane(const uint64_t (&b)[25], uint32_t idx)
+{
+    uint64_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 17; ++i)
+        if ((uint32_t)i == idx) w = b[i];
+    return w;
+}
+
+// Rejection-sampled u32 words mod `modulus` from a copy of the state, 34 words per
+// block.  Reference: TranscriptState.squeezeUInt32 (SHA3Oracle.swift:41-70).
+__device__ __forceinline__ void t_squeeze(const uint64_t (&a)[25], uint32_t count, uint32_t modulus, uint32_t *dst)
+{
+    uint64_t b[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) b[i] = a[i];
+    const uint64_t limit = (1ull << 32) - ((1ull << 32) % modulus);
+    uint32_t got = 0, cand = 0;
+    while (got < count) {
+        if (cand > 0 && cand % 34 == 0) kf_s(b);
+        uint32_t wi = cand % 34;
+        uint64_t lane = pick_lane(b, wi >> 1);
+        uint64_t w = (wi & 1) ? (lane >> 32) : (lane & 0xffffffffull);
+        ++cand;
+        if (w < limit) {
+            if (threadIdx.x == 0) dst[got] = (uint32_t)(w % modulus);
+            ++got;
+        }
+    }
+}
+
+// Runs a list of transcript operations on the persistent 200-byte state with one
+// wave executing wave-uniform code (SALU Keccak).
+__global__ __launch_bounds__(64) void k_transcript(const TranscriptOp *__restrict__ ops, uint32_t nops, uint64_t *state)
+{
+    uint64_t a[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) a[i] = state[i];
+    for (uint32_t k = 0; k < nops; ++k) {
+        const TranscriptOp op = ops[k];
+        if (op.kind == TOP_RESET) {
+#pragma unroll
+            for (int i = 0; i < 25; ++i) a[i] = 0;
+        } else if (op.kind == TOP_ABSORB) {
+            t_absorb(a, reinterpret_cast<const uint64_t *>(op.src), op.n, op.suffix);
+        } else if (op.kind == TOP_SQUEEZE_U32) {
+            t_squeeze(a, (uint32_t)op.n, op.modulus, reinterpret_cast<uint32_t *>(op.dst));
+        }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < 25; ++i) state[i] = a[i];
+    }
+}
+
+// ============================================================== proof emission
+
+// Block 0: commitments and the final layer into their slots of the proof words.
+// Block 1 + q * R + r: layer r of query q (and the query head when r == 0): layer
+// index, pair index, and both openings {leaf index, value, sibling count, siblings}.
+// Layout: CirclePCSFRIProofCodecV1.encode (CircleProofFormat.swift:475-535,617-628).
+__global__ __launch_bounds__(256) void k_emit_proof(EmitParams p)
+{
+    uint32_t *w = p.proof;
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < 8 * p.rounds; i += blockDim.x) w[p.roots_off + i] = p.roots[i / 8][i % 8];
+        for (uint64_t i = threadIdx.x; i < 4 * (uint64_t)p.final_count; i += blockDim.x) w[p.final_off + i] = p.final_layer[i];
+        return;
+    }
+    const uint32_t q = (blockIdx.x - 1) / p.rounds, r = (blockIdx.x - 1) % p.rounds;
+    // indices come from the device transcript (mod N/2); the mask keeps every read in bounds
+    const uint64_t pair0 = (uint64_t)p.query_pairs[q] & ((1ull << (p.log_n - 1)) - 1);
+    const uint64_t pair = pair0 >> r;
+    const uint32_t depth = p.log_n - r;  // siblings per opening
+    uint32_t *qs = w + p.queries_off + (uint64_t)q * p.query_words;
+    uint32_t *ls = qs + p.layer_off[r];
+    const uint64_t m = 1ull << depth;
+    const uint32_t *tree = p.trees[r];
+    const uint4 *layer = p.layers[r];
+    const uint32_t open_words = 7 + 8 * depth;
+    if (threadIdx.x == 0) {
+        if (r == 0) {
+            qs[0] = (uint32_t)pair0; qs[1] = (uint32_t)(pair0 >> 32); qs[2] = p.rounds;
+        }
+        ls[0] = r; ls[1] = (uint32_t)pair; ls[2] = (uint32_t)(pair >> 32);
+        for (int side = 0; side < 2; ++side) {
+            uint32_t *o = ls + 3 + side * open_words;
+            uint64_t leaf = 2 * pair + side;
+            uint4 v = layer[leaf];
+            o[0] = (uint32_t)leaf; o[1] = (uint32_t)(leaf >> 32);
+            o[2] = v.x; o[3] = v.y; o[4] = v.z; o[5] = v.w;
+            o[6] = depth;
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < 2 * 8 * depth; i += blockDim.x) {
+        uint32_t side = i / (8 * depth), rem = i % (8 * depth), lv = rem / 8, wd = rem % 8;
+        uint64_t leaf = 2 * pair + side;
+        uint64_t sib = (leaf >> lv) ^ 1ull;
+        ls[3 + side * open_words + 7 + rem] = tree[8 * ((2 * m - ((2 * m) >> lv)) + sib) + wd];
+    }
+}
+
+// Sibling path of leaf `leaf` (bottom-up, 32 bytes per level) from a resident tree.
+// Reference: MerkleOracle.openingSHA3_256 (MerkleOracle.swift:53-110).
+__global__ __launch_bounds__(256) void k_merkle_path(const uint32_t *__restrict__ tree, uint64_t m, uint32_t depth,
+                                                     uint64_t leaf, uint32_t *__restrict__ out)
+{
+    uint32_t t = threadIdx.x + blockIdx.x * blockDim.x;
+    if (t >= 8 * depth + 8) return;
+    if (t >= 8 * depth) {  // root
+        out[t] = tree[8 * (2 * m - 2) + (t - 8 * depth)];
+        return;
+    }
+    uint32_t lv = t / 8, wd = t % 8;
+    out[t] = tree[8 * ((2 * m - ((2 * m) >> lv)) + ((leaf >> lv) ^ 1ull)) + wd];
+}
+
+// ============================================================== host launchers
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+void launch_keccakf_batch(hipStream_t st, const void *in, uint32_t in_stride, void *out, uint32_t out_stride, uint32_t count)
+{
+    if (!count) return;
+    hipLaunchKernelGGL(k_keccakf_batch, dim3(blocks_for(count, 256)), dim3(256), 0, st, (const uint8_t *)in, in_stride,
+                       (uint8_t *)out, out_stride, count);
+}
+
+void launch_oneblock_hash(hipStream_t st, const void *in, uint32_t stride, uint32_t len, void *out, uint32_t out_stride,
+                          uint32_t count, uint32_t suffix)
+{
+    if (!count) return;
+    bool aligned = ((stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(in) & 3u) == 0);
+    if (aligned)
+        hipLaunchKernelGGL(k_oneblock_hash<true>, dim3(blocks_for(count, 256)), dim3(256), 0, st, (const uint8_t *)in,
+                           stride, len, (uint8_t *)out, out_stride, count, suffix);
+    else
+        hipLaunchKernelGGL(k_oneblock_hash<false>, dim3(blocks_for(count, 256)), dim3(256), 0, st, (const uint8_t *)in,
+                           stride, len, (uint8_t *)out, out_stride, count, suffix);
+}
+
+static inline size_t tile_lds_bytes(uint32_t log_t) { return (size_t)24 << log_t; }  // 16T + 8T bytes
+
+void launch_merkle_raw(hipStream_t st, const void *leaves, uint32_t stride, uint32_t len, uint64_t m, uint32_t log_t,
+                       uint32_t *tree)
+{
+    uint32_t tiles = (uint32_t)(m >> log_t);
+    bool aligned = ((stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(leaves) & 3u) == 0);
+    size_t lds = tile_lds_bytes(log_t);
+    if (aligned)
+        hipLaunchKernelGGL(k_merkle_raw_tiles<true>, dim3(tiles), dim3(256), lds, st, (const uint8_t *)leaves, stride, len,
+                           log_t, tree, m);
+    else
+        hipLaunchKernelGGL(k_merkle_raw_tiles<false>, dim3(tiles), dim3(256), lds, st, (const uint8_t *)leaves, stride,
+                           len, log_t, tree, m);
+}
+
+void launch_tree_upper(hipStream_t st, uint32_t *tree, uint64_t m, uint32_t level, uint32_t max_log_c)
+{
+    // level `level` holds m >> level nodes; reduce in chunks of up to 2^max_log_c.
+    uint32_t log_m = 0;
+    while ((1ull << log_m) < m) ++log_m;
+    while (level < log_m) {
+        uint32_t remaining = log_m - level;
+        uint32_t log_c = remaining < max_log_c ? remaining : max_log_c;
+        uint64_t nodes = m >> level;
+        uint32_t blocks = (uint32_t)(nodes >> log_c);
+        size_t lds = (size_t)(8u << log_c) * 4 + (size_t)(4u << log_c) * 4;
+        hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(256), lds, st, tree, m, (int)level, log_c);
+        level += log_c;
+    }
+}
+
+void launch_copy_words(hipStream_t st, const uint32_t *src, uint32_t *dst, uint64_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy_words, dim3(blocks_for(n, 256)), dim3(256), 0, st, src, dst, n);
+}
+
+void launch_fft_twiddles(hipStream_t st, uint32_t *out, uint32_t log_n)
+{
+    uint64_t n = (1ull << log_n) - 1;
+    if (!n) return;
+    hipLaunchKernelGGL(k_fft_twiddles, dim3(blocks_for(n, 256)), dim3(256), 0, st, out, log_n);
+}
+
+void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_inverse, dim3(blocks_for(n, 256)), dim3(256), 0, st, in, out, n);
+}
+
+void launch_fft_pass(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t lo,
+                     uint32_t k, uint32_t m)
+{
+    uint32_t tiles = (uint32_t)(1ull << (log_n - k - m));
+    hipLaunchKernelGGL(k_fft_pass, dim3(tiles), dim3(256), (size_t)16 << (k + m), st, (const uint4 *)src, (uint4 *)dst, tw,
+                       log_n, lo, k, m);
+}
+
+void launch_fft_final(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
+                      bool run_fft, uint32_t *tree)
+{
+    uint32_t tiles = (uint32_t)(1ull << (log_n - log_t));
+    hipLaunchKernelGGL(k_fft_final, dim3(tiles), dim3(256), tile_lds_bytes(log_t), st, (const uint4 *)src, (uint4 *)dst,
+                       tw, log_n, log_t, (uint32_t)run_fft, tree);
+}
+
+void launch_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
+                 uint32_t log_t, uint32_t *tree)
+{
+    uint32_t tiles = (uint32_t)(n_out >> log_t);
+    hipLaunchKernelGGL(k_fold, dim3(tiles), dim3(256), tile_lds_bytes(log_t), st, (const uint4 *)in, (uint4 *)out, tw, ch,
+                       log_t, tree, n_out);
+}
+
+void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out)
+{
+    hipLaunchKernelGGL(k_merkle_path, dim3(blocks_for(8 * depth + 8, 256)), dim3(256), 0, st, tree, m, depth, leaf, out);
+}
+
+void launch_transcript(hipStream_t st, const TranscriptOp *ops, uint32_t nops, uint64_t *state)
+{
+    hipLaunchKernelGGL(k_transcript, dim3(1), dim3(64), 0, st, ops, nops, state);
+}
+
+void launch_emit_proof(hipStream_t st, const EmitParams &p, uint32_t queries)
+{
+    hipLaunchKernelGGL(k_emit_proof, dim3(1 + queries * p.rounds), dim3(256), 0, st, p);
+}
+
+} // namespace azk

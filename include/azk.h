@@ -1,0 +1,238 @@
+/*
+ * azk.h -- C ABI of the MI355X (gfx950) Circle-PCS commit engine.
+ *
+ * Drop-in boundary for the reference's reusable-plan API (AppleZKProver, Swift over
+ * Metal).  Every entry point names the reference interface it replaces
+ * (paths relative to the reference repository, Sources/AppleZKProver/).
+ * Plain pointers and sizes only; no exceptions cross the ABI; every call returns an
+ * azk_status and leaves a thread-local message for azk_last_error().
+ *
+ * Threading (MerkleCommitter.swift:548,723 executionLock): concurrent calls on one
+ * plan are serialized by a per-plan mutex; separate plans may be used concurrently.
+ * Calls are synchronous unless named *_async.
+ */
+#ifndef AZK_H
+#define AZK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* AppleZKProverError, MetalTypes.swift:3-23 (one code per case). */
+typedef enum azk_status {
+    AZK_OK = 0,
+    AZK_ERR_NO_DEVICE = 1,                      /* noMetalDevice */
+    AZK_ERR_ALLOCATION = 2,                     /* failedToCreateBuffer / CommandQueue */
+    AZK_ERR_UNSUPPORTED_ONE_BLOCK_LENGTH = 3,   /* unsupportedOneBlockLength(n) */
+    AZK_ERR_INVALID_LEAF_COUNT = 4,             /* invalidLeafCount(n) */
+    AZK_ERR_INVALID_PREHASHED_LEAF_LENGTH = 5,  /* invalidPrehashedLeafLength(n) */
+    AZK_ERR_INVALID_LAYOUT = 6,                 /* invalidInputLayout */
+    AZK_ERR_CORRECTNESS = 7,                    /* correctnessValidationFailed(msg) */
+    AZK_ERR_COMMAND_FAILED = 8,                 /* commandExecutionFailed(msg) */
+    AZK_ERR_UNAVAILABLE = 9,                    /* unavailableOnThisPlatform */
+    AZK_ERR_INVALID_KERNEL_CONFIG = 10,         /* invalidKernelConfiguration(msg) */
+    AZK_ERR_INVALID_ARGUMENT = 11               /* null handle or pointer (no Swift analogue) */
+} azk_status;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *azk_last_error(void);
+const char *azk_version(void);
+
+typedef struct azk_context azk_context;
+typedef struct azk_buffer azk_buffer;
+
+/* GPUCapabilities, MetalContext.swift:11-44. */
+typedef struct azk_capabilities {
+    char name[64];
+    char arch[32];
+    int compute_units;
+    int wavefront_size;
+    size_t lds_bytes_per_workgroup;
+    size_t hbm_bytes;
+    int clock_khz;
+} azk_capabilities;
+
+/* GPUExecutionStats, MetalTypes.swift:71-79 (+ submit time, MerkleCommitter.swift:782-797). */
+typedef struct azk_stats {
+    uint64_t wall_ns;    /* host wall clock of the call */
+    uint64_t gpu_ns;     /* HIP-event time of the device work */
+    uint64_t submit_ns;  /* host time spent enqueueing */
+} azk_stats;
+
+/* ---- context: MetalContext(device:pipelineCacheConfiguration:), MetalContext.swift:76-112 */
+azk_status azk_context_create(int device_ordinal, azk_context **out);
+void azk_context_destroy(azk_context *ctx);
+azk_status azk_context_capabilities(azk_context *ctx, azk_capabilities *caps);
+azk_status azk_context_synchronize(azk_context *ctx);
+
+/* ---- resident buffers: MTLBuffer + offset pairs (MetalBufferFactory.swift:1-159) */
+azk_status azk_buffer_alloc(azk_context *ctx, size_t bytes, azk_buffer **out);
+/* Wraps caller-owned device memory (not freed by azk_buffer_free). */
+azk_status azk_buffer_wrap(azk_context *ctx, void *device_ptr, size_t bytes, azk_buffer **out);
+void azk_buffer_free(azk_buffer *buf);
+size_t azk_buffer_size(const azk_buffer *buf);
+void *azk_buffer_device_ptr(const azk_buffer *buf);
+azk_status azk_buffer_upload(azk_buffer *buf, size_t offset, const void *host, size_t bytes);
+azk_status azk_buffer_download(const azk_buffer *buf, size_t offset, void *host, size_t bytes);
+
+/* ---- Keccak-f[1600] permutation batch
+ * KeccakF1600PermutationBatcher.makePermutationPlan / KeccakF1600PermutationBatchPlan
+ * (KeccakF1600PermutationBatcher.swift:31-42,73-294); descriptor MetalTypes.swift:172-188. */
+typedef struct azk_keccakf_desc {
+    uint32_t count;
+    uint32_t input_stride;   /* bytes, multiple of 8, >= 200 */
+    uint32_t output_stride;  /* bytes, multiple of 8, >= 200; padding is zeroed */
+} azk_keccakf_desc;
+typedef struct azk_keccakf_plan azk_keccakf_plan;
+azk_status azk_keccakf_plan_create(azk_context *ctx, const azk_keccakf_desc *desc, azk_keccakf_plan **out);
+void azk_keccakf_plan_destroy(azk_keccakf_plan *plan);
+/* permute(states:) :157-197 -- host arrays in and out. */
+azk_status azk_keccakf_permute(azk_keccakf_plan *plan, const void *states, size_t in_bytes, void *out, size_t out_bytes,
+                               azk_stats *stats);
+/* resident variant: inputs already in HBM, nothing crosses PCIe. */
+azk_status azk_keccakf_permute_resident(azk_keccakf_plan *plan, const azk_buffer *in, size_t in_offset, azk_buffer *out,
+                                        size_t out_offset, azk_stats *stats);
+/* permuteVerified :199-203 -- CPU recomputation gate, AZK_ERR_CORRECTNESS on mismatch. */
+azk_status azk_keccakf_permute_verified(azk_keccakf_plan *plan, const void *states, size_t in_bytes, void *out,
+                                        size_t out_bytes, azk_stats *stats);
+azk_status azk_keccakf_plan_clear(azk_keccakf_plan *plan);
+
+/* ---- one-block SHA3-256 / Keccak-256 batch hashing
+ * SHA3FixedOneBlockHashPlan (SHA3BatchHasher.swift:81-264), Keccak256FixedOneBlockHashPlan
+ * (Keccak256BatchHasher.swift:73-257). */
+typedef struct azk_oneblock_desc {
+    uint32_t count;
+    uint32_t input_stride;   /* bytes >= input_length */
+    uint32_t input_length;   /* 0..136 */
+    uint32_t output_stride;  /* bytes >= 32; padding is zeroed */
+} azk_oneblock_desc;
+typedef struct azk_hash_plan azk_hash_plan;
+azk_status azk_sha3_256_plan_create(azk_context *ctx, const azk_oneblock_desc *desc, azk_hash_plan **out);
+azk_status azk_keccak_256_plan_create(azk_context *ctx, const azk_oneblock_desc *desc, azk_hash_plan **out);
+void azk_hash_plan_destroy(azk_hash_plan *plan);
+azk_status azk_hash_execute(azk_hash_plan *plan, const void *messages, size_t in_bytes, void *digests, size_t out_bytes,
+                            azk_stats *stats);
+azk_status azk_hash_execute_resident(azk_hash_plan *plan, const azk_buffer *in, size_t in_offset, azk_buffer *out,
+                                     size_t out_offset, azk_stats *stats);
+azk_status azk_hash_execute_verified(azk_hash_plan *plan, const void *messages, size_t in_bytes, void *digests,
+                                     size_t out_bytes, azk_stats *stats);
+azk_status azk_hash_plan_clear(azk_hash_plan *plan);
+
+/* ---- SHA3-256 Merkle commitment over raw leaves (<= 136 bytes)
+ * SHA3MerkleCommitter.makeRawLeavesCommitPlan / SHA3RawLeavesMerkleCommitPlan
+ * (MerkleCommitter.swift:152-165,526-1182).  The whole tree stays resident after a
+ * commit; openings gather siblings from it (openRawLeaf :896-1063 rebuilt it). */
+typedef struct azk_merkle_desc {
+    uint32_t leaf_count;     /* power of two */
+    uint32_t leaf_stride;
+    uint32_t leaf_length;    /* 0..136 */
+    uint32_t subtree_log;    /* leaves per LDS treelet = 2^subtree_log; 0 = default */
+} azk_merkle_desc;
+typedef struct azk_merkle_plan azk_merkle_plan;
+azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_desc *desc, azk_merkle_plan **out);
+void azk_merkle_plan_destroy(azk_merkle_plan *plan);
+azk_status azk_merkle_commit(azk_merkle_plan *plan, const void *leaves, size_t bytes, uint8_t root[32], azk_stats *stats);
+azk_status azk_merkle_commit_resident(azk_merkle_plan *plan, const azk_buffer *leaves, size_t offset, uint8_t root[32],
+                                      azk_stats *stats);
+azk_status azk_merkle_commit_verified(azk_merkle_plan *plan, const void *leaves, size_t bytes, uint8_t root[32],
+                                      azk_stats *stats);
+/* Opening of leaf `index` from the last commit: siblings bottom-up, log2(leaf_count) x 32 bytes.
+ * MerkleOracle.openingSHA3_256 (MerkleOracle.swift:53-110). */
+azk_status azk_merkle_open(azk_merkle_plan *plan, uint32_t index, uint8_t *siblings, size_t siblings_bytes,
+                           uint8_t root[32]);
+azk_status azk_merkle_plan_clear(azk_merkle_plan *plan);
+
+/* ---- Merkle root over prehashed 32-byte leaves
+ * SHA3PrehashedLeavesMerkleCommitPlan(context:leafCount:) (MerkleCommitter.swift:314-524). */
+typedef struct azk_prehashed_plan azk_prehashed_plan;
+azk_status azk_prehashed_plan_create(azk_context *ctx, uint32_t leaf_count, azk_prehashed_plan **out);
+void azk_prehashed_plan_destroy(azk_prehashed_plan *plan);
+azk_status azk_prehashed_commit_resident(azk_prehashed_plan *plan, const azk_buffer *leaf_hashes, size_t offset,
+                                         uint8_t root[32], azk_stats *stats);
+
+/* ---- Circle codeword (Circle FFT) on the canonical bit-reversed domain
+ * CircleCodewordPlan(context:domain:) (CircleCodeword.swift:411-887).  Input: interleaved
+ * line-basis coefficients (coeff[2i] = x_i, coeff[2i+1] = y_i; CircleCodeword.swift:100-127),
+ * N QM31 = N x 16 bytes.  Output: N QM31 evaluations in storage order. */
+typedef struct azk_codeword_plan azk_codeword_plan;
+azk_status azk_codeword_plan_create(azk_context *ctx, uint32_t log_n, azk_codeword_plan **out);
+void azk_codeword_plan_destroy(azk_codeword_plan *plan);
+azk_status azk_codeword_execute_resident(azk_codeword_plan *plan, const azk_buffer *coeffs, size_t coeff_offset,
+                                         azk_buffer *out, size_t out_offset, azk_stats *stats);
+azk_status azk_codeword_execute(azk_codeword_plan *plan, const void *coeffs, size_t in_bytes, void *evals,
+                                size_t out_bytes, azk_stats *stats);
+/* FFT twiddles (N - 1 words) as materialized on the device (readCodewordTwiddles :818-820). */
+azk_status azk_codeword_read_twiddles(azk_codeword_plan *plan, uint32_t *out, size_t words);
+
+/* ---- Circle FRI fold chain with explicit challenges
+ * CircleFRIFoldPlan / CircleFRIFoldChainPlan (CircleFRIFoldPlan.swift:5-280). */
+typedef struct azk_fold_plan azk_fold_plan;
+azk_status azk_fold_plan_create(azk_context *ctx, uint32_t log_n, uint32_t rounds, azk_fold_plan **out);
+void azk_fold_plan_destroy(azk_fold_plan *plan);
+/* challenges: rounds x 4 LE u32; output: N >> rounds QM31. */
+azk_status azk_fold_execute_resident(azk_fold_plan *plan, const azk_buffer *evals, size_t evals_offset,
+                                     const uint32_t *challenges, azk_buffer *out, size_t out_offset, azk_stats *stats);
+azk_status azk_fold_execute(azk_fold_plan *plan, const void *evals, size_t in_bytes, const uint32_t *challenges,
+                            void *out, size_t out_bytes, azk_stats *stats);
+
+/* ---- Circle PCS/FRI commit superstep (the prover)
+ * CircleCodewordPCSFRIProverV1 (CircleCodeword.swift:920-1419) and
+ * CirclePCSFRIResidentProverV1 (CircleFRIFoldPlan.swift:693-1025), security parameters
+ * CircleFRISecurityParametersV1 (CircleProofFormat.swift:129-158).  Output: the proof
+ * in CirclePCSFRIProofCodecV1 bytes (CircleProofFormat.swift:475-535).  Only roots, the
+ * final layer and the opened leaves/paths leave HBM. */
+typedef struct azk_circle_prover_desc {
+    uint32_t log_n;            /* codeword log size, 2..28 */
+    uint32_t rounds;           /* FRI rounds, 1..min(log_n, 28) */
+    uint32_t log_blowup;
+    uint32_t query_count;
+    uint32_t folding_step;     /* must be 1 */
+    uint32_t grinding_bits;    /* must be 0 on the device path */
+    uint8_t public_input_digest[32];
+} azk_circle_prover_desc;
+typedef struct azk_circle_prover azk_circle_prover;
+azk_status azk_circle_prover_create(azk_context *ctx, const azk_circle_prover_desc *desc, azk_circle_prover **out);
+void azk_circle_prover_destroy(azk_circle_prover *prover);
+size_t azk_circle_prover_proof_size(const azk_circle_prover *prover);
+/* proveCircleFFTCoefficientsResident (CircleCodeword.swift:1007-1018). */
+azk_status azk_circle_prove_fft_coefficients_resident(azk_circle_prover *prover, const azk_buffer *coeffs,
+                                                      size_t offset, uint8_t *proof, size_t capacity,
+                                                      size_t *proof_len, azk_stats *stats);
+/* prove(evaluationsBuffer:offset:) (CircleFRIFoldPlan.swift:782-848). */
+azk_status azk_circle_prove_evaluations_resident(azk_circle_prover *prover, const azk_buffer *evals, size_t offset,
+                                                 uint8_t *proof, size_t capacity, size_t *proof_len,
+                                                 azk_stats *stats);
+/* Host-array variant: uploads the line-basis coefficients, then proves. */
+azk_status azk_circle_prove_fft_coefficients(azk_circle_prover *prover, const void *coeffs, size_t bytes,
+                                             uint8_t *proof, size_t capacity, size_t *proof_len, azk_stats *stats);
+/* clearReusableBuffers (CircleCodeword.swift:1238-1253). */
+azk_status azk_circle_prover_clear(azk_circle_prover *prover);
+/* 1 = replay the superstep as a captured hipGraph (default), 0 = eager stream launches. */
+azk_status azk_circle_prover_set_graph(azk_circle_prover *prover, int enable);
+/* Per-phase device times of the last prove when timing is on (HIP events). */
+typedef struct azk_phase_times {
+    uint64_t fft_ns;        /* Circle FFT passes (non-final) */
+    uint64_t commit0_ns;    /* final FFT pass fused with leaf hashes + first-layer tree */
+    uint64_t fold_ns;       /* fold kernels fused with next-layer leaf hashes + tiles */
+    uint64_t tree_ns;       /* upper Merkle levels */
+    uint64_t transcript_ns; /* device Fiat-Shamir */
+    uint64_t emit_ns;       /* proof gather */
+    uint64_t total_ns;
+} azk_phase_times;
+azk_status azk_circle_prover_set_timing(azk_circle_prover *prover, int enable);
+azk_status azk_circle_prover_phase_times(const azk_circle_prover *prover, azk_phase_times *out);
+
+/* ---- Batched form: many independent commits of uneven sizes in one captured graph.
+ * No reference counterpart (the reference runs one command buffer per job); each
+ * job's proof is byte-identical to its plain single-job prove. */
+azk_status azk_circle_prove_batch(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
+                                  const size_t *offsets, uint32_t jobs, uint8_t *const *proofs,
+                                  const size_t *capacities, size_t *proof_lens, azk_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AZK_H */
