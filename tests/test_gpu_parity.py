@@ -299,4 +299,3 @@ def test_batch_uneven_jobs_match_plain(ctx):
         coeffs = inputs.circle_fft_coefficients(seed=500 + j, log_n=ln)
         digest = inputs.random_bytes(1000 + j, 32).tobytes()
         assert proofs[j] == _oracle_proof_from_coeffs(coeffs, ln, (2, 8, 1, 0), digest, min(10, ln - 2))
-EOF
