@@ -67,7 +67,7 @@ class ProverDesc(C.Structure):
 class PhaseTimes(C.Structure):
     _fields_ = [("fft_ns", C.c_uint64), ("commit0_ns", C.c_uint64), ("fold_ns", C.c_uint64),
                 ("tree_ns", C.c_uint64), ("transcript_ns", C.c_uint64), ("emit_ns", C.c_uint64),
-                ("total_ns", C.c_uint64)]
+                ("host_ns", C.c_uint64), ("total_ns", C.c_uint64)]
 
 
 _lib = None
@@ -140,6 +140,9 @@ def load() -> C.CDLL:
         "azk_circle_prover_set_graph": (st, [vp, C.c_int]),
         "azk_circle_prover_set_timing": (st, [vp, C.c_int]),
         "azk_circle_prover_phase_times": (st, [vp, C.POINTER(PhaseTimes)]),
+        "azk_circle_batch_create": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp)]),
+        "azk_circle_batch_destroy": (None, [vp]),
+        "azk_circle_batch_prove": (st, [vp, C.POINTER(vp), C.POINTER(sz), C.POINTER(sz), sp]),
         "azk_circle_prove_batch": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp),
                                         C.POINTER(sz), C.POINTER(sz), sp]),
     }
@@ -522,3 +525,40 @@ def prove_batch(provers: list, coeff_buffers: list, offsets=None) -> tuple[list[
     s = Stats()
     _ck(lib.azk_circle_prove_batch(ph, bh, offs, n, op, caps, lens, C.byref(s)))
     return [outs[i][: lens[i]].tobytes() for i in range(n)], _stats_dict(s)
+
+
+class CircleBatchPlan:
+    """A fixed set of jobs captured once as one graph (azk_circle_batch_*)."""
+
+    def __init__(self, provers: list, coeff_buffers: list, offsets=None):
+        self.lib = load()
+        self.provers, self.buffers = list(provers), list(coeff_buffers)
+        n = len(self.provers)
+        offs = (sz * n)(*(offsets or [0] * n))
+        ph = (vp * n)(*[p.h for p in self.provers])
+        bh = (vp * n)(*[b.h for b in self.buffers])
+        h = vp()
+        _ck(self.lib.azk_circle_batch_create(ph, bh, offs, n, C.byref(h)))
+        self.h = h
+        self._outs = [np.zeros(p.proof_size, np.uint8) for p in self.provers]
+        self._op = (vp * n)(*[o.ctypes.data for o in self._outs])
+        self._caps = (sz * n)(*[p.proof_size for p in self.provers])
+        self._lens = (sz * n)()
+        self.last_stats: dict = {}
+
+    def prove(self) -> list[bytes]:
+        s = Stats()
+        _ck(self.lib.azk_circle_batch_prove(self.h, self._op, self._caps, self._lens, C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return [self._outs[i][: self._lens[i]].tobytes() for i in range(len(self.provers))]
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.azk_circle_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

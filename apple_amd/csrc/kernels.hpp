@@ -7,7 +7,7 @@ namespace azk {
 
 constexpr int kMaxRounds = 30;
 
-enum TranscriptOpKind : uint32_t { TOP_RESET = 0, TOP_ABSORB = 1, TOP_SQUEEZE_U32 = 2 };
+enum TranscriptOpKind : uint32_t { TOP_RESET = 0, TOP_ABSORB = 1, TOP_SQUEEZE_U32 = 2, TOP_LOAD_STATE = 3 };
 
 // One step of a device transcript program.  `src` must be 8-byte aligned and
 // readable up to the next multiple of 8 bytes past `n`.
@@ -50,6 +50,8 @@ void launch_fft_final(hipStream_t st, const void *src, void *dst, const uint32_t
 void launch_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
                  uint32_t log_t, uint32_t *tree);
 void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out);
+void launch_fold_leafwise(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch,
+                         uint64_t n_out, uint32_t log_t, uint32_t *tree);
 void launch_transcript(hipStream_t st, const TranscriptOp *ops, uint32_t nops, uint64_t *state);
 void launch_emit_proof(hipStream_t st, const EmitParams &p, uint32_t queries);
 

@@ -1,29 +1,20 @@
-// Host runtime behind include/azk.h: context, resident buffers, reusable plans, the
-// Circle PCS/FRI prover superstep (captured once per shape as a hipGraph and replayed),
-// and the batched multi-job form.
+// Host runtime behind include/azk.h: context, resident buffers and the reusable plans
+// (Keccak-f, one-block hashes, Merkle, prehashed Merkle, Circle codeword, fold chain).
+// The Circle PCS prover and the batched form live in prover.hip.
 //
 // Plan semantics mirror the reference's Swift plans (cited per entry point in azk.h);
 // buffers, streams and graphs are HIP-native.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <chrono>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <mutex>
-#include <string>
-#include <vector>
-
-#include "../../include/azk.h"
+#include "runtime_common.hpp"
 #include "host_sha3.hpp"
-#include "kernels.hpp"
 
 using namespace azk;
+using namespace azk::rt;
 
 namespace {
-
 thread_local std::string g_err;
+}
+
+namespace azk::rt {
 
 azk_status fail(azk_status s, const char *fmt, ...)
 {
@@ -36,59 +27,6 @@ azk_status fail(azk_status s, const char *fmt, ...)
     return s;
 }
 
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess) return fail(AZK_ERR_COMMAND_FAILED, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-#define CHECK_ARG(c)                                                                               \
-    do {                                                                                           \
-        if (!(c)) return fail(AZK_ERR_INVALID_ARGUMENT, "invalid argument: %s", #c);               \
-    } while (0)
-
-inline uint64_t now_ns()
-{
-    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
-               std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
-inline bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
-inline uint32_t ilog2(uint64_t v)
-{
-    uint32_t r = 0;
-    while (v > 1) { v >>= 1; ++r; }
-    return r;
-}
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-constexpr uint32_t kP = 0x7fffffffu;
-constexpr uint32_t kDefaultLogTile = 11;   // 2048-element tiles: 48 KB LDS per block
-constexpr uint32_t kUpperLogChunk = 10;    // upper-tree reduction chunk
-
-} // namespace
-
-// ====================================================================== context
-
-struct azk_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipDeviceProp_t props{};
-    std::mutex mu;  // serializes use of `stream` across plans
-};
-
-struct azk_buffer {
-    azk_context *ctx = nullptr;
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    bool owned = false;
-};
-
-namespace {
-
-// Range check of a (buffer, offset, bytes) triple: validateBufferRange,
-// CircleCodeword.swift:845-857.
 azk_status check_range(const azk_buffer *b, size_t off, size_t bytes)
 {
     if (!b) return fail(AZK_ERR_INVALID_ARGUMENT, "null buffer");
@@ -97,52 +35,75 @@ azk_status check_range(const azk_buffer *b, size_t off, size_t bytes)
     return AZK_OK;
 }
 
-// Aliasing of two ranges: rangesOverlap, CircleCodeword.swift:863-879.
 bool overlaps(const azk_buffer *a, size_t ao, size_t an, const azk_buffer *b, size_t bo, size_t bn)
 {
     if (!a || !b || !an || !bn) return false;
     uintptr_t a0 = (uintptr_t)a->ptr + ao, b0 = (uintptr_t)b->ptr + bo;
     return a0 < b0 + bn && b0 < a0 + an;
 }
+std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t)
+{
+    std::vector<FftPass> v;
+    uint32_t lt = std::min(log_t, log_n);
+    uint32_t S = log_n - lt;
+    if (!S) return v;
+    uint32_t kmax = log_t - 3;
+    uint32_t P = (S + kmax - 1) / kmax, hi = log_n;
+    for (uint32_t p = 0; p < P; ++p) {
+        uint32_t k = S / P + (p < S % P ? 1 : 0);
+        uint32_t lo = hi - k;
+        v.push_back({lo, k, log_t - k});
+        hi = lo;
+    }
+    return v;
+}
 
-struct DeviceMem {
-    void *p = nullptr;
-    size_t n = 0;
-    ~DeviceMem() { if (p) (void)hipFree(p); }
-    azk_status ensure(size_t bytes)
-    {
-        if (bytes <= n) return AZK_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return fail(AZK_ERR_ALLOCATION, "hipMalloc(%zu) failed", bytes);
-        n = bytes;
-        return AZK_OK;
+void enqueue_fft(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
+                 uint32_t *tree)
+{
+    const void *cur = src;
+    for (const FftPass &p : fft_passes(log_n, log_t)) {
+        launch_fft_pass(st, cur, dst, tw, log_n, p.lo, p.k, p.m);
+        cur = dst;
     }
-};
+    launch_fft_final(st, cur, dst, tw, log_n, std::min(log_t, log_n), true, tree);
+}
 
-struct Timer {
-    hipEvent_t a = nullptr, b = nullptr;
-    ~Timer()
-    {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
+azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw)
+{
+    const uint64_t n = 1ull << log_n;
+    azk_status s = fft_tw.ensure(4 * n);
+    if (s) return s;
+    launch_fft_twiddles(ctx->stream, (uint32_t *)fft_tw.p, log_n);
+    if (fold_tw) {
+        const uint64_t total = n - (n >> rounds);
+        s = fold_tw->ensure(4 * std::max<uint64_t>(total, 1));
+        if (s) return s;
+        launch_inverse(ctx->stream, (const uint32_t *)fft_tw.p, (uint32_t *)fold_tw->p, total);
     }
-    azk_status init()
-    {
-        if (!a) HIP_TRY(hipEventCreate(&a));
-        if (!b) HIP_TRY(hipEventCreate(&b));
-        return AZK_OK;
-    }
-    uint64_t elapsed_ns()
-    {
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
-        return (uint64_t)(ms * 1e6);
-    }
-};
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return AZK_OK;
+}
 
-} // namespace
+
+
+uint32_t enqueue_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
+                      uint32_t log_t, uint32_t *tree)
+{
+    if (tree && n_out < kLeafwiseBelow) {
+        uint32_t lt = std::min<uint32_t>(kLeafwiseLogTile, ilog2(n_out));
+        launch_fold_leafwise(st, in, out, tw, ch, n_out, lt, tree);
+        return lt;
+    }
+    uint32_t lt = std::min<uint32_t>(log_t, ilog2(n_out));
+    launch_fold(st, in, out, tw, ch, n_out, lt, tree);
+    return lt;
+}
+
+} // namespace azk::rt
+
+// ====================================================================== context / buffers
 
 extern "C" {
 
@@ -253,35 +214,6 @@ azk_status azk_buffer_download(const azk_buffer *b, size_t off, void *host, size
 }
 
 } // extern "C"
-
-// ====================================================================== common plan helpers
-
-namespace {
-
-// Runs `enqueue` on the context stream bracketed by HIP events and waits.
-template <class F>
-azk_status run_timed(azk_context *ctx, azk_stats *stats, F &&enqueue)
-{
-    Timer t;
-    azk_status s = t.init();
-    if (s) return s;
-    uint64_t w0 = now_ns();
-    HIP_TRY(hipEventRecord(t.a, ctx->stream));
-    s = enqueue();
-    if (s) return s;
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(t.b, ctx->stream));
-    uint64_t w1 = now_ns();
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (stats) {
-        stats->submit_ns = w1 - w0;
-        stats->wall_ns = now_ns() - w0;
-        stats->gpu_ns = t.elapsed_ns();
-    }
-    return AZK_OK;
-}
-
-} // namespace
 
 // ====================================================================== Keccak-f batch
 
@@ -637,63 +569,6 @@ extern "C" azk_status azk_prehashed_commit_resident(azk_prehashed_plan *p, const
     return AZK_OK;
 }
 
-// ====================================================================== Circle FFT helpers
-
-namespace {
-
-struct FftPass { uint32_t lo, k, m; };
-
-// Stages log_n-1 .. lt (lt = min(log_t, log_n)) split into passes of at most
-// log_t - 3 stages, each on tiles of 2^log_t elements with >= 8 contiguous columns.
-std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t)
-{
-    std::vector<FftPass> v;
-    uint32_t lt = std::min(log_t, log_n);
-    uint32_t S = log_n - lt;
-    if (!S) return v;
-    uint32_t kmax = log_t - 3;
-    uint32_t P = (S + kmax - 1) / kmax, hi = log_n;
-    for (uint32_t p = 0; p < P; ++p) {
-        uint32_t k = S / P + (p < S % P ? 1 : 0);
-        uint32_t lo = hi - k;
-        v.push_back({lo, k, log_t - k});
-        hi = lo;
-    }
-    return v;
-}
-
-// Enqueues the Circle FFT of `src` (interleaved line-basis coefficients) into `dst`;
-// with `tree`, the final pass also hashes the codeword and builds its tile subtrees.
-void enqueue_fft(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
-                 uint32_t *tree)
-{
-    const void *cur = src;
-    for (const FftPass &p : fft_passes(log_n, log_t)) {
-        launch_fft_pass(st, cur, dst, tw, log_n, p.lo, p.k, p.m);
-        cur = dst;
-    }
-    launch_fft_final(st, cur, dst, tw, log_n, std::min(log_t, log_n), true, tree);
-}
-
-azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw)
-{
-    const uint64_t n = 1ull << log_n;
-    azk_status s = fft_tw.ensure(4 * n);
-    if (s) return s;
-    launch_fft_twiddles(ctx->stream, (uint32_t *)fft_tw.p, log_n);
-    if (fold_tw) {
-        const uint64_t total = n - (n >> rounds);
-        s = fold_tw->ensure(4 * std::max<uint64_t>(total, 1));
-        if (s) return s;
-        launch_inverse(ctx->stream, (const uint32_t *)fft_tw.p, (uint32_t *)fold_tw->p, total);
-    }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return AZK_OK;
-}
-
-} // namespace
-
 // ====================================================================== codeword plan
 
 struct azk_codeword_plan {
@@ -803,9 +678,8 @@ static azk_status fold_enqueue(azk_fold_plan *p, const void *in, void *out)
     for (uint32_t r = 0; r < p->rounds; ++r) {
         void *dst = (r + 1 == p->rounds) ? out : (r % 2 ? p->b.p : p->a.p);
         uint64_t n_out = m / 2;
-        uint32_t lt = std::min<uint32_t>(kDefaultLogTile, ilog2(n_out));
-        launch_fold(p->ctx->stream, cur, dst, (const uint32_t *)p->fold_tw.p + off, (const uint32_t *)p->ch.p + 4 * r, n_out,
-                    lt, nullptr);
+        enqueue_fold(p->ctx->stream, cur, dst, (const uint32_t *)p->fold_tw.p + off, (const uint32_t *)p->ch.p + 4 * r, n_out,
+                     kDefaultLogTile, nullptr);
         cur = dst;
         off += n_out;
         m = n_out;
@@ -850,543 +724,3 @@ extern "C" azk_status azk_fold_execute(azk_fold_plan *p, const void *evals, size
     return s;
 }
 
-// ====================================================================== Circle PCS prover
-
-namespace {
-
-void put32(std::vector<uint8_t> &v, uint32_t x) { for (int k = 0; k < 4; ++k) v.push_back((uint8_t)(x >> (8 * k))); }
-void put64(std::vector<uint8_t> &v, uint64_t x) { for (int k = 0; k < 8; ++k) v.push_back((uint8_t)(x >> (8 * k))); }
-
-// Circle transcript V1 frames, CircleProofFormat.swift:906-975.
-std::vector<uint8_t> base_frame(uint8_t type)
-{
-    static const char dom[] = "AppleZKProver.CircleFRI.PCS.V1";
-    std::vector<uint8_t> f;
-    put32(f, (uint32_t)(sizeof dom - 1));
-    f.insert(f.end(), dom, dom + sizeof dom - 1);
-    put32(f, 1);
-    f.push_back(type);
-    return f;
-}
-
-// 44-byte domain descriptor, CircleDomainDescriptorCodecV1.encode (CircleProofFormat.swift:75-93).
-std::vector<uint8_t> descriptor(uint32_t log_n)
-{
-    std::vector<uint8_t> d = {0x41, 0x5a, 0x4b, 0x43, 0x44, 0x56, 0x31, 0x00};
-    for (uint32_t w : {1u, kP, 31u, 2u, 1268011823u, log_n, 1u << (31 - (log_n + 1)), log_n - 1, 1u}) put32(d, w);
-    return d;
-}
-
-} // namespace
-
-struct azk_circle_prover {
-    azk_context *ctx;
-    azk_circle_prover_desc d;
-    uint32_t log_t;
-    uint64_t n;
-    // device state
-    DeviceMem fft_tw, fold_tw, arena, frames, ops, proof, coeff_staging;
-    std::vector<uint4 *> layers;      // rounds + 1 layers (layer 0 = codeword)
-    std::vector<uint32_t *> trees;    // rounds trees
-    uint32_t *challenges = nullptr, *qidx = nullptr;
-    uint64_t *tstate = nullptr;
-    std::vector<std::pair<uint32_t, uint32_t>> programs;  // (first op, count), rounds + 1 programs
-    // proof layout (words)
-    uint64_t roots_off, final_off, queries_off, query_words, proof_words;
-    std::vector<uint32_t> layer_off;
-    uint8_t *host_proof = nullptr;   // pinned staging
-    // graph cache
-    bool use_graph = true, timing = false;
-    hipGraphExec_t graph = nullptr;
-    const void *graph_src = nullptr;
-    int graph_mode = -1;
-    azk_phase_times phases{};
-    std::mutex mu;
-    ~azk_circle_prover()
-    {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        if (host_proof) (void)hipHostFree(host_proof);
-    }
-};
-
-namespace {
-
-azk_status prover_build(azk_circle_prover *p)
-{
-    const azk_circle_prover_desc &d = p->d;
-    const uint64_t n = p->n;
-    azk_context *ctx = p->ctx;
-    azk_status s = materialize_twiddles(ctx, d.log_n, d.rounds, p->fft_tw, &p->fold_tw);
-    if (s) return s;
-
-    // ---- arena: layers, trees, transcript scratch
-    std::vector<size_t> layer_at(d.rounds + 1), tree_at(d.rounds);
-    size_t at = 0;
-    for (uint32_t r = 0; r <= d.rounds; ++r) {
-        layer_at[r] = at;
-        at = align_up(at + ((size_t)16 << (d.log_n - r)), 256);
-    }
-    for (uint32_t r = 0; r < d.rounds; ++r) {
-        tree_at[r] = at;
-        at = align_up(at + (((size_t)2 << (d.log_n - r)) - 1) * 32, 256);
-    }
-    size_t ch_at = at; at = align_up(at + 16 * (size_t)d.rounds, 256);
-    size_t q_at = at; at = align_up(at + 4 * (size_t)d.query_count, 256);
-    size_t st_at = at; at = align_up(at + 200, 256);
-    s = p->arena.ensure(at);
-    if (s) return s;
-    uint8_t *base = (uint8_t *)p->arena.p;
-    p->layers.resize(d.rounds + 1);
-    p->trees.resize(d.rounds);
-    for (uint32_t r = 0; r <= d.rounds; ++r) p->layers[r] = (uint4 *)(base + layer_at[r]);
-    for (uint32_t r = 0; r < d.rounds; ++r) p->trees[r] = (uint32_t *)(base + tree_at[r]);
-    p->challenges = (uint32_t *)(base + ch_at);
-    p->qidx = (uint32_t *)(base + q_at);
-    p->tstate = (uint64_t *)(base + st_at);
-
-    // ---- transcript frames (8-byte aligned, zero padded) and programs
-    std::vector<uint8_t> blob;
-    auto add = [&](const std::vector<uint8_t> &b) {
-        size_t o = blob.size();
-        blob.insert(blob.end(), b.begin(), b.end());
-        blob.resize(align_up(blob.size(), 8) + 8, 0);
-        return std::make_pair(o, b.size());
-    };
-    std::vector<uint8_t> f;
-    f = base_frame(0);
-    for (uint32_t w : {1u, 1u, kP, 2u, 1268011823u}) put32(f, w);
-    auto hdr = add(f);
-    auto desc = add(descriptor(d.log_n));
-    f = base_frame(1);
-    for (uint32_t w : {d.log_blowup, d.query_count, d.folding_step, d.grinding_bits, d.log_blowup * d.query_count + d.grinding_bits})
-        put32(f, w);
-    auto sec = add(f);
-    f = base_frame(2);
-    put32(f, 32);
-    auto pin = add(f);
-    auto dig = add(std::vector<uint8_t>(d.public_input_digest, d.public_input_digest + 32));
-    std::vector<std::pair<size_t, size_t>> cfr(d.rounds), chf(d.rounds);
-    for (uint32_t r = 0; r < d.rounds; ++r) {
-        f = base_frame(3);
-        put32(f, r);
-        put32(f, 32);
-        cfr[r] = add(f);
-        f = base_frame(4);
-        put32(f, r);
-        put32(f, kP);
-        put32(f, 4);
-        chf[r] = add(f);
-    }
-    const uint64_t nfin = n >> d.rounds;
-    f = base_frame(5);
-    put64(f, nfin);
-    put64(f, nfin * 16);
-    auto ffr = add(f);
-    f = base_frame(6);
-    put32(f, d.query_count);
-    put64(f, n / 2);
-    auto qfr = add(f);
-    s = p->frames.ensure(blob.size());
-    if (s) return s;
-    HIP_TRY(hipMemcpy(p->frames.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
-    const uint8_t *fb = (const uint8_t *)p->frames.p;
-
-    std::vector<TranscriptOp> ops;
-    auto absorb = [&](const void *src, uint64_t len) { ops.push_back({TOP_ABSORB, 0x06, 0, 0, len, src, nullptr}); };
-    auto absorb_f = [&](std::pair<size_t, size_t> fr) { absorb(fb + fr.first, fr.second); };
-    for (uint32_t r = 0; r < d.rounds; ++r) {
-        uint32_t first = (uint32_t)ops.size();
-        if (r == 0) {  // prefixTranscriptAndChallenges, CircleProofFormat.swift:796-818
-            ops.push_back({TOP_RESET, 0, 0, 0, 0, nullptr, nullptr});
-            absorb_f(hdr);
-            absorb_f(desc);
-            absorb_f(sec);
-            absorb_f(pin);
-            absorb_f(dig);
-        }
-        const uint64_t m = n >> r;  // per round: commitment frame, root, challenge frame, squeeze 4 (:819-828)
-        absorb_f(cfr[r]);
-        absorb(p->trees[r] + 8 * (2 * m - 2), 32);
-        absorb_f(chf[r]);
-        ops.push_back({TOP_SQUEEZE_U32, 0, kP, 0, 4, nullptr, p->challenges + 4 * r});
-        p->programs.push_back({first, (uint32_t)ops.size() - first});
-    }
-    {   // final layer + query squeeze, CircleProofFormat.swift:699-717,852-859
-        uint32_t first = (uint32_t)ops.size();
-        absorb_f(ffr);
-        absorb(p->layers[d.rounds], nfin * 16);
-        absorb_f(qfr);
-        ops.push_back({TOP_SQUEEZE_U32, 0, (uint32_t)(n / 2), 0, d.query_count, nullptr, p->qidx});
-        p->programs.push_back({first, (uint32_t)ops.size() - first});
-    }
-    s = p->ops.ensure(ops.size() * sizeof(TranscriptOp));
-    if (s) return s;
-    HIP_TRY(hipMemcpy(p->ops.p, ops.data(), ops.size() * sizeof(TranscriptOp), hipMemcpyHostToDevice));
-
-    // ---- proof layout (all fields are 4-byte multiples; CircleProofFormat.swift:478-535)
-    std::vector<uint8_t> head = {0x41, 0x5a, 0x4b, 0x50, 0x46, 0x56, 0x31, 0x00};
-    put32(head, 1);
-    put32(head, 1);
-    put32(head, 44);
-    auto dd = descriptor(d.log_n);
-    head.insert(head.end(), dd.begin(), dd.end());
-    for (uint32_t w : {d.log_blowup, d.query_count, d.folding_step, d.grinding_bits}) put32(head, w);
-    put32(head, 32);
-    head.insert(head.end(), d.public_input_digest, d.public_input_digest + 32);
-    put32(head, d.rounds);
-    p->roots_off = head.size() / 4;
-    const uint64_t nfin_at = p->roots_off + 8 * (uint64_t)d.rounds;
-    p->final_off = nfin_at + 2;
-    const uint64_t q_count_at = p->final_off + 4 * nfin;
-    p->queries_off = q_count_at + 1;
-    p->layer_off.assign(d.rounds, 0);
-    uint64_t qw = 3;
-    for (uint32_t r = 0; r < d.rounds; ++r) {
-        p->layer_off[r] = (uint32_t)qw;
-        qw += 3 + 2 * (7 + 8 * (uint64_t)(d.log_n - r));
-    }
-    p->query_words = qw;
-    p->proof_words = p->queries_off + (uint64_t)d.query_count * qw;
-    std::vector<uint32_t> tmpl(p->proof_words, 0);
-    memcpy(tmpl.data(), head.data(), head.size());
-    tmpl[nfin_at] = (uint32_t)nfin;
-    tmpl[nfin_at + 1] = (uint32_t)(nfin >> 32);
-    tmpl[q_count_at] = d.query_count;
-    s = p->proof.ensure(4 * p->proof_words);
-    if (s) return s;
-    HIP_TRY(hipMemcpy(p->proof.p, tmpl.data(), 4 * p->proof_words, hipMemcpyHostToDevice));
-    HIP_TRY(hipHostMalloc((void **)&p->host_proof, 4 * p->proof_words, hipHostMallocDefault));
-    return AZK_OK;
-}
-
-EmitParams emit_params(azk_circle_prover *p, const uint4 *layer0)
-{
-    EmitParams e{};
-    e.proof = (uint32_t *)p->proof.p;
-    e.roots_off = p->roots_off;
-    e.final_off = p->final_off;
-    e.queries_off = p->queries_off;
-    e.query_words = p->query_words;
-    e.rounds = p->d.rounds;
-    e.log_n = p->d.log_n;
-    e.final_count = p->n >> p->d.rounds;
-    for (uint32_t r = 0; r < p->d.rounds; ++r) {
-        e.layer_off[r] = p->layer_off[r];
-        e.roots[r] = p->trees[r] + 8 * (2 * (p->n >> r) - 2);
-        e.trees[r] = p->trees[r];
-        e.layers[r] = r == 0 ? layer0 : p->layers[r];
-    }
-    e.final_layer = (const uint32_t *)p->layers[p->d.rounds];
-    e.query_pairs = p->qidx;
-    return e;
-}
-
-// Event slots for per-phase timing (eager mode only).
-struct PhaseClock {
-    std::vector<hipEvent_t> ev;
-    std::vector<int> tag;
-    hipStream_t st;
-    bool on;
-    PhaseClock(hipStream_t s, bool enabled) : st(s), on(enabled) {}
-    ~PhaseClock() { for (auto e : ev) (void)hipEventDestroy(e); }
-    void mark(int t)
-    {
-        if (!on) return;
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
-        (void)hipEventRecord(e, st);
-        ev.push_back(e);
-        tag.push_back(t);
-    }
-};
-enum { PH_START = 0, PH_FFT, PH_COMMIT0, PH_FOLD, PH_TREE, PH_TRANSCRIPT, PH_EMIT };
-
-// The commit superstep, stream-ordered: FFT -> first-layer leaf hashes + tree ->
-// per round {transcript -> fold fused with next-layer hashing -> upper tree} ->
-// final-layer absorb + query squeeze -> proof gather -> one D2H copy.
-// mode 0: src = line-basis coefficients (FFT first); mode 1: src = codeword.
-void prover_enqueue(azk_circle_prover *p, hipStream_t st, const void *src, int mode, PhaseClock &clk)
-{
-    const azk_circle_prover_desc &d = p->d;
-    const uint32_t lt0 = std::min(p->log_t, d.log_n);
-    const uint4 *layer0 = mode == 0 ? p->layers[0] : (const uint4 *)src;
-    clk.mark(PH_START);
-    if (mode == 0) {
-        const void *cur = src;
-        for (const FftPass &fp : fft_passes(d.log_n, p->log_t)) {
-            launch_fft_pass(st, cur, p->layers[0], (const uint32_t *)p->fft_tw.p, d.log_n, fp.lo, fp.k, fp.m);
-            cur = p->layers[0];
-        }
-        clk.mark(PH_FFT);
-        launch_fft_final(st, cur, p->layers[0], (const uint32_t *)p->fft_tw.p, d.log_n, lt0, true, p->trees[0]);
-    } else {
-        clk.mark(PH_FFT);
-        launch_fft_final(st, src, nullptr, nullptr, d.log_n, lt0, false, p->trees[0]);
-    }
-    clk.mark(PH_COMMIT0);
-    launch_tree_upper(st, p->trees[0], p->n, lt0, kUpperLogChunk);
-    clk.mark(PH_TREE);
-    const TranscriptOp *ops = (const TranscriptOp *)p->ops.p;
-    uint64_t off = 0, m = p->n;
-    for (uint32_t r = 0; r < d.rounds; ++r) {
-        launch_transcript(st, ops + p->programs[r].first, p->programs[r].second, p->tstate);
-        clk.mark(PH_TRANSCRIPT);
-        const uint64_t n_out = m / 2;
-        const bool commit_next = r + 1 < d.rounds;
-        const uint32_t lt = std::min<uint32_t>(p->log_t, ilog2(n_out));
-        launch_fold(st, r == 0 ? (const void *)layer0 : (const void *)p->layers[r], p->layers[r + 1],
-                    (const uint32_t *)p->fold_tw.p + off, p->challenges + 4 * r, n_out, lt,
-                    commit_next ? p->trees[r + 1] : nullptr);
-        clk.mark(PH_FOLD);
-        if (commit_next) {
-            launch_tree_upper(st, p->trees[r + 1], n_out, lt, kUpperLogChunk);
-            clk.mark(PH_TREE);
-        }
-        off += n_out;
-        m = n_out;
-    }
-    launch_transcript(st, ops + p->programs[d.rounds].first, p->programs[d.rounds].second, p->tstate);
-    clk.mark(PH_TRANSCRIPT);
-    launch_emit_proof(st, emit_params(p, layer0), d.query_count);
-    (void)hipMemcpyAsync(p->host_proof, p->proof.p, 4 * p->proof_words, hipMemcpyDeviceToHost, st);
-    clk.mark(PH_EMIT);
-}
-
-} // namespace
-
-extern "C" azk_status azk_circle_prover_create(azk_context *ctx, const azk_circle_prover_desc *desc,
-                                               azk_circle_prover **out)
-{
-    CHECK_ARG(ctx && desc && out);
-    const azk_circle_prover_desc &d = *desc;
-    // CirclePCSFRIProofV1.init / CircleFRISecurityParametersV1.init (CircleProofFormat.swift:129-158,355-393)
-    if (d.log_n < 2 || d.log_n > 28 || d.rounds < 1 || d.rounds > d.log_n || d.rounds > (uint32_t)kMaxRounds)
-        return fail(AZK_ERR_INVALID_LAYOUT, "log_n %u / rounds %u outside the supported shape", d.log_n, d.rounds);
-    if (d.log_blowup == 0 || d.query_count == 0 || d.query_count > 65536 || d.folding_step != 1 ||
-        (uint64_t)d.log_blowup * d.query_count + d.grinding_bits > 0xffffffffull)
-        return fail(AZK_ERR_INVALID_LAYOUT, "invalid security parameters");
-    if (d.grinding_bits != 0)
-        return fail(AZK_ERR_INVALID_KERNEL_CONFIG, "grinding is not implemented on the device path (grinding_bits = %u)",
-                    d.grinding_bits);
-    auto *p = new azk_circle_prover();
-    p->ctx = ctx;
-    p->d = d;
-    p->n = 1ull << d.log_n;
-    p->log_t = kDefaultLogTile;
-    HIP_TRY(hipSetDevice(ctx->device));
-    std::lock_guard<std::mutex> gc(ctx->mu);
-    azk_status s = prover_build(p);
-    if (s) { delete p; return s; }
-    *out = p;
-    return AZK_OK;
-}
-
-extern "C" void azk_circle_prover_destroy(azk_circle_prover *p)
-{
-    if (!p) return;
-    (void)hipSetDevice(p->ctx->device);
-    delete p;
-}
-
-extern "C" size_t azk_circle_prover_proof_size(const azk_circle_prover *p) { return p ? 4 * p->proof_words : 0; }
-
-extern "C" azk_status azk_circle_prover_set_graph(azk_circle_prover *p, int enable)
-{
-    CHECK_ARG(p);
-    std::lock_guard<std::mutex> g(p->mu);
-    p->use_graph = enable != 0;
-    return AZK_OK;
-}
-
-extern "C" azk_status azk_circle_prover_set_timing(azk_circle_prover *p, int enable)
-{
-    CHECK_ARG(p);
-    std::lock_guard<std::mutex> g(p->mu);
-    p->timing = enable != 0;
-    return AZK_OK;
-}
-
-extern "C" azk_status azk_circle_prover_phase_times(const azk_circle_prover *p, azk_phase_times *out)
-{
-    CHECK_ARG(p && out);
-    *out = p->phases;
-    return AZK_OK;
-}
-
-namespace {
-
-azk_status prove_locked(azk_circle_prover *p, const void *src, int mode, uint8_t *proof, size_t cap, size_t *len,
-                        azk_stats *stats)
-{
-    const size_t bytes = 4 * p->proof_words;
-    if (cap < bytes || !proof) return fail(AZK_ERR_INVALID_LAYOUT, "proof capacity %zu < %zu", cap, bytes);
-    hipStream_t st = p->ctx->stream;
-    const bool graph = p->use_graph && !p->timing;
-    PhaseClock clk(st, p->timing);
-    azk_status s = run_timed(p->ctx, stats, [&]() -> azk_status {
-        if (graph) {
-            if (!p->graph || p->graph_src != src || p->graph_mode != mode) {
-                if (p->graph) (void)hipGraphExecDestroy(p->graph);
-                p->graph = nullptr;
-                hipGraph_t gr;
-                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                PhaseClock none(st, false);
-                prover_enqueue(p, st, src, mode, none);
-                HIP_TRY(hipStreamEndCapture(st, &gr));
-                hipError_t e = hipGraphInstantiate(&p->graph, gr, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(gr);
-                if (e != hipSuccess) return fail(AZK_ERR_COMMAND_FAILED, "hipGraphInstantiate: %s", hipGetErrorString(e));
-                p->graph_src = src;
-                p->graph_mode = mode;
-            }
-            HIP_TRY(hipGraphLaunch(p->graph, st));
-        } else {
-            prover_enqueue(p, st, src, mode, clk);
-        }
-        return AZK_OK;
-    });
-    if (s) return s;
-    if (p->timing && clk.ev.size() > 1) {
-        azk_phase_times t{};
-        uint64_t *slot[] = {nullptr, &t.fft_ns, &t.commit0_ns, &t.fold_ns, &t.tree_ns, &t.transcript_ns, &t.emit_ns};
-        for (size_t i = 1; i < clk.ev.size(); ++i) {
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, clk.ev[i - 1], clk.ev[i]);
-            *slot[clk.tag[i]] += (uint64_t)(ms * 1e6);
-        }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, clk.ev.front(), clk.ev.back());
-        t.total_ns = (uint64_t)(ms * 1e6);
-        p->phases = t;
-    }
-    memcpy(proof, p->host_proof, bytes);
-    if (len) *len = bytes;
-    return AZK_OK;
-}
-
-} // namespace
-
-extern "C" azk_status azk_circle_prove_fft_coefficients_resident(azk_circle_prover *p, const azk_buffer *coeffs,
-                                                                 size_t off, uint8_t *proof, size_t cap, size_t *len,
-                                                                 azk_stats *stats)
-{
-    CHECK_ARG(p && coeffs);
-    std::lock_guard<std::mutex> g(p->mu);
-    azk_status s = check_range(coeffs, off, (size_t)16 << p->d.log_n);
-    if (s) return s;
-    if (((uintptr_t)coeffs->ptr + off) % 16) return fail(AZK_ERR_INVALID_LAYOUT, "coefficients must be 16-byte aligned");
-    HIP_TRY(hipSetDevice(p->ctx->device));
-    std::lock_guard<std::mutex> gc(p->ctx->mu);
-    return prove_locked(p, (uint8_t *)coeffs->ptr + off, 0, proof, cap, len, stats);
-}
-
-extern "C" azk_status azk_circle_prove_evaluations_resident(azk_circle_prover *p, const azk_buffer *evals, size_t off,
-                                                            uint8_t *proof, size_t cap, size_t *len, azk_stats *stats)
-{
-    CHECK_ARG(p && evals);
-    std::lock_guard<std::mutex> g(p->mu);
-    azk_status s = check_range(evals, off, (size_t)16 << p->d.log_n);
-    if (s) return s;
-    if (((uintptr_t)evals->ptr + off) % 16) return fail(AZK_ERR_INVALID_LAYOUT, "evaluations must be 16-byte aligned");
-    HIP_TRY(hipSetDevice(p->ctx->device));
-    std::lock_guard<std::mutex> gc(p->ctx->mu);
-    return prove_locked(p, (uint8_t *)evals->ptr + off, 1, proof, cap, len, stats);
-}
-
-extern "C" azk_status azk_circle_prove_fft_coefficients(azk_circle_prover *p, const void *coeffs, size_t bytes,
-                                                        uint8_t *proof, size_t cap, size_t *len, azk_stats *stats)
-{
-    CHECK_ARG(p && coeffs);
-    std::lock_guard<std::mutex> g(p->mu);
-    const size_t need = (size_t)16 << p->d.log_n;
-    if (bytes < need) return fail(AZK_ERR_INVALID_LAYOUT, "coefficient bytes %zu < %zu", bytes, need);
-    HIP_TRY(hipSetDevice(p->ctx->device));
-    azk_status s = p->coeff_staging.ensure(need);
-    if (s) return s;
-    std::lock_guard<std::mutex> gc(p->ctx->mu);
-    HIP_TRY(hipMemcpyAsync(p->coeff_staging.p, coeffs, need, hipMemcpyHostToDevice, p->ctx->stream));
-    return prove_locked(p, p->coeff_staging.p, 0, proof, cap, len, stats);
-}
-
-extern "C" azk_status azk_circle_prover_clear(azk_circle_prover *p)
-{
-    CHECK_ARG(p);
-    std::lock_guard<std::mutex> g(p->mu);
-    HIP_TRY(hipSetDevice(p->ctx->device));
-    // zero everything that held witness-derived data; the proof template is rebuilt
-    std::vector<uint32_t> tmpl(p->proof_words);
-    HIP_TRY(hipMemcpy(tmpl.data(), p->proof.p, 4 * p->proof_words, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemset(p->arena.p, 0, p->arena.n));
-    if (p->coeff_staging.p) HIP_TRY(hipMemset(p->coeff_staging.p, 0, p->coeff_staging.n));
-    for (uint32_t r = 0; r < p->d.rounds; ++r)
-        memset(tmpl.data() + p->roots_off + 8 * r, 0, 32);
-    memset(tmpl.data() + p->final_off, 0, 16 * (p->n >> p->d.rounds));
-    memset(tmpl.data() + p->queries_off, 0, 4 * p->d.query_count * p->query_words);
-    HIP_TRY(hipMemcpy(p->proof.p, tmpl.data(), 4 * p->proof_words, hipMemcpyHostToDevice));
-    memset(p->host_proof, 0, 4 * p->proof_words);
-    return AZK_OK;
-}
-
-// ====================================================================== batched form
-
-extern "C" azk_status azk_circle_prove_batch(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
-                                             const size_t *offsets, uint32_t jobs, uint8_t *const *proofs,
-                                             const size_t *caps, size_t *lens, azk_stats *stats)
-{
-    CHECK_ARG(provers && coeffs && offsets && proofs && caps);
-    if (!jobs) return AZK_OK;
-    azk_context *ctx = provers[0]->ctx;
-    for (uint32_t j = 0; j < jobs; ++j) {
-        CHECK_ARG(provers[j] && coeffs[j] && proofs[j]);
-        if (provers[j]->ctx != ctx) return fail(AZK_ERR_INVALID_ARGUMENT, "all provers must share one context");
-        for (uint32_t k = 0; k < j; ++k)
-            if (provers[k] == provers[j]) return fail(AZK_ERR_INVALID_ARGUMENT, "a prover appears twice in one batch");
-        azk_status s = check_range(coeffs[j], offsets[j], (size_t)16 << provers[j]->d.log_n);
-        if (s) return s;
-        if (caps[j] < 4 * provers[j]->proof_words) return fail(AZK_ERR_INVALID_LAYOUT, "proof capacity of job %u", j);
-    }
-    std::vector<std::unique_lock<std::mutex>> locks;
-    for (uint32_t j = 0; j < jobs; ++j) locks.emplace_back(provers[j]->mu);
-    HIP_TRY(hipSetDevice(ctx->device));
-    std::lock_guard<std::mutex> gc(ctx->mu);
-    hipStream_t st = ctx->stream;
-    // One graph: a fork from the capture stream into one branch per job, joined at the end.
-    std::vector<hipStream_t> side(jobs);
-    std::vector<hipEvent_t> done(jobs);
-    hipEvent_t fork;
-    HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    for (uint32_t j = 0; j < jobs; ++j) {
-        HIP_TRY(hipStreamCreateWithFlags(&side[j], hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&done[j], hipEventDisableTiming));
-    }
-    hipGraph_t gr = nullptr;
-    hipGraphExec_t ge = nullptr;
-    azk_status s = run_timed(ctx, stats, [&]() -> azk_status {
-        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        HIP_TRY(hipEventRecord(fork, st));
-        for (uint32_t j = 0; j < jobs; ++j) {
-            HIP_TRY(hipStreamWaitEvent(side[j], fork, 0));
-            PhaseClock none(side[j], false);
-            prover_enqueue(provers[j], side[j], (const uint8_t *)coeffs[j]->ptr + offsets[j], 0, none);
-            HIP_TRY(hipEventRecord(done[j], side[j]));
-            HIP_TRY(hipStreamWaitEvent(st, done[j], 0));
-        }
-        HIP_TRY(hipStreamEndCapture(st, &gr));
-        HIP_TRY(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
-        HIP_TRY(hipGraphLaunch(ge, st));
-        return AZK_OK;
-    });
-    if (ge) (void)hipGraphExecDestroy(ge);
-    if (gr) (void)hipGraphDestroy(gr);
-    for (uint32_t j = 0; j < jobs; ++j) {
-        (void)hipStreamDestroy(side[j]);
-        (void)hipEventDestroy(done[j]);
-    }
-    (void)hipEventDestroy(fork);
-    if (s) return s;
-    for (uint32_t j = 0; j < jobs; ++j) {
-        memcpy(proofs[j], provers[j]->host_proof, 4 * provers[j]->proof_words);
-        if (lens) lens[j] = 4 * provers[j]->proof_words;
-    }
-    return AZK_OK;
-}

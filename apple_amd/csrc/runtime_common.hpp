@@ -1,0 +1,176 @@
+// Shared host-side pieces of the runtime: errors, device memory, timing, layout checks,
+// the context/buffer objects and the Circle FFT pass schedule.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/azk.h"
+#include "kernels.hpp"
+
+struct azk_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    std::mutex mu;  // serializes use of `stream` across plans
+};
+
+struct azk_buffer {
+    azk_context *ctx = nullptr;
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    bool owned = false;
+};
+
+namespace azk::rt {
+
+azk_status fail(azk_status s, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return ::azk::rt::fail(AZK_ERR_COMMAND_FAILED, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_ARG(c)                                                                               \
+    do {                                                                                           \
+        if (!(c)) return ::azk::rt::fail(AZK_ERR_INVALID_ARGUMENT, "invalid argument: %s", #c);    \
+    } while (0)
+
+inline uint64_t now_ns()
+{
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+inline bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
+inline uint32_t ilog2(uint64_t v)
+{
+    uint32_t r = 0;
+    while (v > 1) { v >>= 1; ++r; }
+    return r;
+}
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+constexpr uint32_t kP = 0x7fffffffu;
+constexpr uint32_t kDefaultLogTile = 11;    // 2048-element tiles: 48 KB LDS per block
+constexpr uint32_t kUpperLogChunk = 10;     // upper-tree reduction chunk
+constexpr uint32_t kLeafwiseLogTile = 8;    // latency form: one leaf per thread, 256 per block
+constexpr uint64_t kLeafwiseBelow = 1ull << 19;  // layers smaller than this use the latency form
+
+struct DeviceMem {
+    void *p = nullptr;
+    size_t n = 0;
+    DeviceMem() = default;
+    DeviceMem(const DeviceMem &) = delete;
+    DeviceMem &operator=(const DeviceMem &) = delete;
+    ~DeviceMem() { if (p) (void)hipFree(p); }
+    azk_status ensure(size_t bytes)
+    {
+        if (bytes <= n && p) return AZK_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return fail(AZK_ERR_ALLOCATION, "hipMalloc(%zu) failed", bytes);
+        n = bytes;
+        return AZK_OK;
+    }
+};
+
+struct PinnedMem {
+    void *p = nullptr;
+    size_t n = 0;
+    PinnedMem() = default;
+    PinnedMem(const PinnedMem &) = delete;
+    PinnedMem &operator=(const PinnedMem &) = delete;
+    ~PinnedMem() { if (p) (void)hipHostFree(p); }
+    azk_status ensure(size_t bytes)
+    {
+        if (bytes <= n && p) return AZK_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+            return fail(AZK_ERR_ALLOCATION, "hipHostMalloc(%zu) failed", bytes);
+        n = bytes;
+        return AZK_OK;
+    }
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Timer()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    azk_status init()
+    {
+        if (!a) HIP_TRY(hipEventCreate(&a));
+        if (!b) HIP_TRY(hipEventCreate(&b));
+        return AZK_OK;
+    }
+    uint64_t elapsed_ns()
+    {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+        return (uint64_t)(ms * 1e6);
+    }
+};
+
+// Range check of a (buffer, offset, bytes) triple: validateBufferRange,
+// CircleCodeword.swift:845-857.
+azk_status check_range(const azk_buffer *b, size_t off, size_t bytes);
+// Aliasing of two ranges: rangesOverlap, CircleCodeword.swift:863-879.
+bool overlaps(const azk_buffer *a, size_t ao, size_t an, const azk_buffer *b, size_t bo, size_t bn);
+
+// Runs `enqueue` on the context stream bracketed by HIP events and waits.
+template <class F>
+azk_status run_timed(azk_context *ctx, azk_stats *stats, F &&enqueue)
+{
+    Timer t;
+    azk_status s = t.init();
+    if (s) return s;
+    uint64_t w0 = now_ns();
+    HIP_TRY(hipEventRecord(t.a, ctx->stream));
+    s = enqueue();
+    if (s) return s;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(t.b, ctx->stream));
+    uint64_t w1 = now_ns();
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        stats->submit_ns = w1 - w0;
+        stats->wall_ns = now_ns() - w0;
+        stats->gpu_ns = t.elapsed_ns();
+    }
+    return AZK_OK;
+}
+
+struct FftPass { uint32_t lo, k, m; };
+
+// Stages log_n-1 .. lt (lt = min(log_t, log_n)) split into passes of at most
+// log_t - 3 stages, each on tiles of 2^log_t elements with >= 8 contiguous columns.
+std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t);
+
+// Enqueues the Circle FFT of `src` (interleaved line-basis coefficients) into `dst`;
+// with `tree`, the final pass also hashes the codeword and builds its tile subtrees.
+void enqueue_fft(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
+                 uint32_t *tree);
+
+// FFT twiddles (and, with `fold_tw`, the first `rounds` stages inverted = FRI fold twiddles).
+azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw);
+
+// One FRI fold round (layer of 2 n_out -> n_out), choosing the tile form by size;
+// returns the log tile size whose roots remain for the upper tree (0 if no tree).
+uint32_t enqueue_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
+                      uint32_t log_t, uint32_t *tree);
+
+} // namespace azk::rt

@@ -218,16 +218,25 @@ typedef struct azk_phase_times {
     uint64_t commit0_ns;    /* final FFT pass fused with leaf hashes + first-layer tree */
     uint64_t fold_ns;       /* fold kernels fused with next-layer leaf hashes + tiles */
     uint64_t tree_ns;       /* upper Merkle levels */
-    uint64_t transcript_ns; /* device Fiat-Shamir */
-    uint64_t emit_ns;       /* proof gather */
-    uint64_t total_ns;
+    uint64_t transcript_ns; /* device Fiat-Shamir rounds (critical-path part) */
+    uint64_t emit_ns;       /* readback of the tail, index upload, proof gather, proof readback */
+    uint64_t host_ns;       /* host final-layer absorb + query squeeze */
+    uint64_t total_ns;      /* first to last event, including the host step */
 } azk_phase_times;
 azk_status azk_circle_prover_set_timing(azk_circle_prover *prover, int enable);
 azk_status azk_circle_prover_phase_times(const azk_circle_prover *prover, azk_phase_times *out);
 
 /* ---- Batched form: many independent commits of uneven sizes in one captured graph.
  * No reference counterpart (the reference runs one command buffer per job); each
- * job's proof is byte-identical to its plain single-job prove. */
+ * job's proof is byte-identical to its plain single-job prove.  A batch plan captures
+ * the jobs once (provers, coefficient buffers and offsets are fixed) and replays. */
+typedef struct azk_circle_batch azk_circle_batch;
+azk_status azk_circle_batch_create(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
+                                   const size_t *offsets, uint32_t jobs, azk_circle_batch **out);
+void azk_circle_batch_destroy(azk_circle_batch *batch);
+azk_status azk_circle_batch_prove(azk_circle_batch *batch, uint8_t *const *proofs, const size_t *capacities,
+                                  size_t *proof_lens, azk_stats *stats);
+/* One-shot form: create, prove, destroy. */
 azk_status azk_circle_prove_batch(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
                                   const size_t *offsets, uint32_t jobs, uint8_t *const *proofs,
                                   const size_t *capacities, size_t *proof_lens, azk_stats *stats);

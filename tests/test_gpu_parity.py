@@ -299,3 +299,22 @@ def test_batch_uneven_jobs_match_plain(ctx):
         coeffs = inputs.circle_fft_coefficients(seed=500 + j, log_n=ln)
         digest = inputs.random_bytes(1000 + j, 32).tobytes()
         assert proofs[j] == _oracle_proof_from_coeffs(coeffs, ln, (2, 8, 1, 0), digest, min(10, ln - 2))
+
+
+def test_batch_plan_replays_and_evaluation_jobs(ctx):
+    logs = [9, 12, 10, 13, 9]
+    provers, bufs, wants = [], [], []
+    for j, ln in enumerate(logs):
+        digest = inputs.random_bytes(2000 + j, 32).tobytes()
+        coeffs = inputs.circle_fft_coefficients(seed=900 + j, log_n=ln)
+        pr = azk.CircleCodewordPCSFRIProverV1(ctx, ln, ln - 3, (2, 6, 1, 0), digest)
+        b = ctx.upload(coeffs)
+        provers.append(pr)
+        bufs.append(b)
+        wants.append(_oracle_proof_from_coeffs(coeffs, ln, (2, 6, 1, 0), digest, ln - 3))
+    plan = azk.CircleBatchPlan(provers, bufs)
+    for _ in range(3):
+        assert plan.prove() == wants
+    # plain proves still agree after batch replays
+    assert [p.prove_fft_coefficients_resident(b) for p, b in zip(provers, bufs)] == wants
+    plan.close()
