@@ -91,6 +91,24 @@ __global__ __launch_bounds__(256) void k_oneblock_hash(const uint8_t *__restrict
     for (uint32_t b = 32; b < out_stride; ++b) o[b] = 0;
 }
 
+
+// ============================================================== descriptor lookup
+
+// Finds the descriptor whose block range holds blockIdx.x (tables are sorted by
+// first_block; the search is wave-uniform) and the block's index inside it.
+template <class D>
+__device__ __forceinline__ const D &find_desc(const D *__restrict__ t, uint32_t n, uint32_t &local)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (t[mid].first_block <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    local = blockIdx.x - t[lo].first_block;
+    return t[lo];
+}
+
 // ============================================================== Merkle tiles
 
 // Node `idx` of level `l` in a full-tree array of a tree with `m` leaves:
@@ -171,17 +189,21 @@ __device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t *B, uint32_
 // block.  Reference: SHA3RawLeavesMerkleCommitPlan.encodeCommitmentRoot
 // (MerkleCommitter.swift:801-894) with the treelet kernel :1167-1218.
 template <bool kAligned>
-__global__ __launch_bounds__(256) void k_merkle_raw_tiles(const uint8_t *__restrict__ leaves, uint32_t stride,
-                                                          uint32_t len, uint32_t log_t, uint32_t *tree, uint64_t m)
+__global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
-    const uint32_t T = 1u << log_t;
+    uint32_t tile;
+    const MerkleRawDesc &d = find_desc(tab, ntab, tile);
+    const uint32_t T = 1u << d.log_t;
+    const uint32_t stride = d.stride, len = d.len;
+    uint32_t *tree = d.tree;
+    const uint64_t m = d.m;
     uint32_t *A = lds, *B = lds + 4 * T;
-    const uint64_t g0 = (uint64_t)blockIdx.x * T;
+    const uint64_t g0 = (uint64_t)tile * T;
     if (T == 1) {
         if (threadIdx.x == 0) {
             uint32_t w[34], h[8];
-            load_message<kAligned>(leaves + g0 * stride, len, w);
+            load_message<kAligned>(d.leaves + g0 * stride, len, w);
             sponge_oneblock(w, len, 0x06u, h);
             st_node(node_ptr(tree, m, 0, g0), h);
         }
@@ -192,7 +214,7 @@ __global__ __launch_bounds__(256) void k_merkle_raw_tiles(const uint8_t *__restr
 #pragma unroll 1
         for (int s = 0; s < 2; ++s) {
             uint32_t w[34];
-            load_message<kAligned>(leaves + (g0 + 2 * p + s) * (uint64_t)stride, len, w);
+            load_message<kAligned>(d.leaves + (g0 + 2 * p + s) * (uint64_t)stride, len, w);
             sponge_oneblock(w, len, 0x06u, h[s]);
         }
         uint32_t par[8];
@@ -206,27 +228,36 @@ __global__ __launch_bounds__(256) void k_merkle_raw_tiles(const uint8_t *__restr
     reduce_lds(A, B, T / 2, 1, g0 >> 1, tree, m);
 }
 
-// Prehashed leaves (32-byte digests) copied into the tree's level 0 and reduced.
-// Reference: SHA3PrehashedLeavesMerkleCommitPlan (MerkleCommitter.swift:314-524).
-__global__ __launch_bounds__(256) void k_tree_reduce(uint32_t *tree, uint64_t m, int l, uint32_t log_c)
+// Upper Merkle levels: each block loads 2^log_c nodes of `level` and reduces them
+// by log_c levels (MerkleCommitter.swift:853-893: parents per level + fused upper tree).
+__global__ __launch_bounds__(256) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
-    const uint32_t C = 1u << log_c;
+    uint32_t chunk;
+    const TreeDesc &d = find_desc(tab, ntab, chunk);
+    const uint32_t C = 1u << d.log_c;
+    uint32_t *tree = d.tree;
+    const uint64_t m = d.m;
+    const int l = (int)d.level;
     uint32_t *A = lds, *B = lds + 8 * C;
-    const uint64_t g = (uint64_t)blockIdx.x * C;
+    const uint64_t g = (uint64_t)chunk * C;
     for (uint32_t q = threadIdx.x; q < C; q += blockDim.x) {
-        uint32_t d[8];
-        ld_node(node_ptr(tree, m, l, g + q), d);
-        st_node(A + 8 * q, d);
+        uint32_t v[8];
+        ld_node(node_ptr(tree, m, l, g + q), v);
+        st_node(A + 8 * q, v);
     }
     __syncthreads();
     reduce_lds(A, B, C, l, g, tree, m);
 }
 
-__global__ __launch_bounds__(256) void k_copy_words(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint64_t n)
+// Word copies, 1024 words per block.
+__global__ __launch_bounds__(256) void k_copy(const CopyDesc *__restrict__ tab, uint32_t ntab)
 {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    uint32_t blk;
+    const CopyDesc &d = find_desc(tab, ntab, blk);
+    const uint64_t base = (uint64_t)blk * 1024;
+    for (uint32_t i = threadIdx.x; i < 1024; i += blockDim.x)
+        if (base + i < d.words) d.dst[base + i] = d.src[base + i];
 }
 
 // ============================================================== Circle domain
@@ -309,22 +340,25 @@ __device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *__rest
 }
 
 // Non-final pass (lo >= 1): tile index -> (high bits, column chunk).
-__global__ __launch_bounds__(256) void k_fft_pass(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
-                                                  const uint32_t *__restrict__ tw, uint32_t log_n, uint32_t lo,
-                                                  uint32_t k, uint32_t m)
+__global__ __launch_bounds__(256) void k_fft_pass(const FftPassDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint4 s[];
+    uint32_t tile;
+    const FftPassDesc &d = find_desc(tab, ntab, tile);
+    const uint32_t lo = d.lo, k = d.k, m = d.m;
     const uint32_t T = 1u << (k + m);
-    const uint64_t n = 1ull << log_n;
+    const uint64_t n = 1ull << d.log_n;
     const uint32_t chunk_bits = lo - m;
-    const uint64_t high = (uint64_t)blockIdx.x >> chunk_bits;
-    const uint64_t chunk = (uint64_t)blockIdx.x & ((1ull << chunk_bits) - 1);
+    const uint64_t high = (uint64_t)tile >> chunk_bits;
+    const uint64_t chunk = (uint64_t)tile & ((1ull << chunk_bits) - 1);
     const uint64_t base = (high << (lo + k)) | (chunk << m);
     const uint32_t cmask = (1u << m) - 1u;
+    const uint4 *src = d.src;
+    uint4 *dst = d.dst;
     for (uint32_t L = threadIdx.x; L < T; L += blockDim.x)
         s[L] = src[base | ((uint64_t)(L >> m) << lo) | (L & cmask)];
     __syncthreads();
-    fft_tile_stages(s, tw, n, lo, k, m, high);
+    fft_tile_stages(s, d.tw, n, lo, k, m, high);
     for (uint32_t L = threadIdx.x; L < T; L += blockDim.x)
         dst[base | ((uint64_t)(L >> m) << lo) | (L & cmask)] = s[L];
 }
@@ -332,21 +366,25 @@ __global__ __launch_bounds__(256) void k_fft_pass(const uint4 *__restrict__ src,
 // Final pass: stages log_t - 1 .. 0 on a contiguous tile of T = 2^log_t elements,
 // then the first-layer leaf hashes and the tile's Merkle subtree, as the reference
 // fuses stage 0 with the leaf hash (circle_codeword_fft_stage_leaf_hash :1910-1943)
-// and commits the hashes (SHA3PrehashedLeavesMerkleCommitPlan).
-// With run_fft == 0 the tile is only hashed (a codeword supplied by the caller).
-__global__ __launch_bounds__(256) void k_fft_final(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
-                                                   const uint32_t *__restrict__ tw, uint32_t log_n, uint32_t log_t,
-                                                   uint32_t run_fft, uint32_t *tree)
+// and commits the hashes (SHA3PrehashedLeavesMerkleCommitPlan).  With run_fft == 0
+// the tile is only hashed (a codeword supplied by the caller).
+__global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
+    uint32_t tile;
+    const FftFinalDesc &d = find_desc(tab, ntab, tile);
+    const uint32_t log_t = d.log_t;
     const uint32_t T = 1u << log_t;
     uint4 *s = reinterpret_cast<uint4 *>(lds);
-    const uint64_t n = 1ull << log_n;
-    const uint64_t base = (uint64_t)blockIdx.x * T;
+    const uint64_t n = 1ull << d.log_n;
+    const uint64_t base = (uint64_t)tile * T;
+    const uint4 *src = d.src;
+    uint32_t *tree = d.tree;
     for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) s[L] = src[base + L];
     __syncthreads();
-    if (run_fft) {
-        fft_tile_stages(s, tw, n, 0, log_t, 0, blockIdx.x);
+    if (d.run_fft) {
+        fft_tile_stages(s, d.tw, n, 0, log_t, 0, tile);
+        uint4 *dst = d.dst;
         for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) dst[base + L] = s[L];
     }
     if (tree) tile_tree_qm31(lds, lds + 4 * T, T, base, tree, n);
@@ -358,14 +396,19 @@ __global__ __launch_bounds__(256) void k_fft_final(const uint4 *__restrict__ src
 // read from device memory, then (if `tree`) the new layer's leaf hashes and tile
 // subtrees.  Reference: qm31_fri_fold_challenge_buffer (:2013-2032) and
 // QM31FRIFoldOracle.fold (QM31FRIFold.swift:11-44).
-__global__ __launch_bounds__(256) void k_fold(const uint4 *__restrict__ in, uint4 *__restrict__ out,
-                                              const uint32_t *__restrict__ tw, const uint32_t *__restrict__ ch,
-                                              uint32_t log_t, uint32_t *tree, uint64_t n_out)
+__global__ __launch_bounds__(256) void k_fold(const FoldDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
-    const uint32_t T = 1u << log_t;
-    const uint64_t base = (uint64_t)blockIdx.x * T;
+    uint32_t tile;
+    const FoldDesc &d = find_desc(tab, ntab, tile);
+    const uint32_t T = 1u << d.log_t;
+    const uint64_t base = (uint64_t)tile * T;
+    const uint32_t *ch = d.ch;
     const qm31 c = {ch[0], ch[1], ch[2], ch[3]};
+    const uint4 *in = d.in;
+    uint4 *out = d.out;
+    const uint32_t *tw = d.tw;
+    uint32_t *tree = d.tree;
     uint4 *s = reinterpret_cast<uint4 *>(lds);
     for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) {
         uint64_t g = base + L;
@@ -376,25 +419,28 @@ __global__ __launch_bounds__(256) void k_fold(const uint4 *__restrict__ in, uint
     }
     if (!tree) return;
     __syncthreads();
-    tile_tree_qm31(lds, lds + 4 * T, T, base, tree, n_out);
+    tile_tree_qm31(lds, lds + 4 * T, T, base, tree, d.n_out);
 }
 
-// Latency form for small layers: one output element per thread, T = blockDim
-// outputs per block; each thread folds its pair, hashes the new leaf and the block
-// reduces its T leaf hashes in LDS (serial chain: one leaf permutation + log2 T levels).
-__global__ __launch_bounds__(256) void k_fold_leafwise(const uint4 *__restrict__ in, uint4 *__restrict__ out,
-                                                       const uint32_t *__restrict__ tw, const uint32_t *__restrict__ ch,
-                                                       uint32_t log_t, uint32_t *tree, uint64_t n_out)
+// Latency form for small layers: one output element per thread, T <= 256 outputs
+// per block; each thread folds its pair, hashes the new leaf and the block reduces
+// its leaf hashes in LDS (serial chain: one leaf permutation + log2 T levels).
+__global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restrict__ tab, uint32_t ntab)
 {
     __shared__ uint32_t A[8 * 256], B[4 * 256];
-    const uint32_t T = 1u << log_t;
-    const uint64_t base = (uint64_t)blockIdx.x * T;
+    uint32_t tile;
+    const FoldDesc &d = find_desc(tab, ntab, tile);
+    const uint32_t T = 1u << d.log_t;
+    const uint64_t base = (uint64_t)tile * T;
+    const uint32_t *ch = d.ch;
     const qm31 c = {ch[0], ch[1], ch[2], ch[3]};
+    uint32_t *tree = d.tree;
+    const uint64_t n_out = d.n_out;
     const uint32_t L = threadIdx.x;
     if (L < T) {
         const uint64_t g = base + L;
-        qm31 o = fri_fold_pair(ld_qm(in[2 * g]), ld_qm(in[2 * g + 1]), tw[g], c);
-        out[g] = st_qm(o);
+        qm31 o = fri_fold_pair(ld_qm(d.in[2 * g]), ld_qm(d.in[2 * g + 1]), d.tw[g], c);
+        d.out[g] = st_qm(o);
         if (tree) {
             uint32_t v[4] = {o.a, o.b, o.c, o.d}, h[8];
             sha3_leaf16(v, h);
@@ -477,10 +523,15 @@ __device__ __forceinline__ void t_squeeze(const uint64_t (&a)[25], uint32_t coun
     }
 }
 
-// Runs a list of transcript operations on the persistent 200-byte state with one
-// wave executing wave-uniform code (SALU Keccak).
-__global__ __launch_bounds__(64) void k_transcript(const TranscriptOp *__restrict__ ops, uint32_t nops, uint64_t *state)
+// Runs a transcript program on its job's persistent 200-byte state with one wave
+// executing wave-uniform code (SALU Keccak); one block per job.
+__global__ __launch_bounds__(64) void k_transcript(const TranscriptDesc *__restrict__ tab, uint32_t ntab)
 {
+    uint32_t unused;
+    const TranscriptDesc &d = find_desc(tab, ntab, unused);
+    const TranscriptOp *ops = d.ops;
+    uint64_t *state = d.state;
+    const uint32_t nops = d.nops;
     uint64_t a[25];
 #pragma unroll
     for (int i = 0; i < 25; ++i) a[i] = ld_uniform(state + i);
@@ -506,20 +557,24 @@ __global__ __launch_bounds__(64) void k_transcript(const TranscriptOp *__restric
 
 // ============================================================== proof emission
 
-// Block 0: commitments and the final layer into their slots of the proof words.
-// Block 1 + q * R + r: layer r of query q (and the query head when r == 0): layer
-// index, pair index, and both openings {leaf index, value, sibling count, siblings}.
-// Layout: CirclePCSFRIProofCodecV1.encode (CircleProofFormat.swift:475-535,617-628).
-__global__ __launch_bounds__(256) void k_emit_proof(EmitParams p)
+// Per job: block 0 writes the commitments and the final layer into their slots of
+// the proof words; block 1 + q * R + r writes layer r of query q (and the query head
+// when r == 0): layer index, pair index, and both openings {leaf index, value,
+// sibling count, siblings}.  Layout: CirclePCSFRIProofCodecV1.encode
+// (CircleProofFormat.swift:475-535,617-628).
+__global__ __launch_bounds__(256) void k_emit_proof(const EmitParams *__restrict__ tab, uint32_t ntab)
 {
+    uint32_t blk;
+    const EmitParams &p = find_desc(tab, ntab, blk);
     uint32_t *w = p.proof;
-    if (blockIdx.x == 0) {
-        for (uint32_t i = threadIdx.x; i < 8 * p.rounds; i += blockDim.x) w[p.roots_off + i] = p.roots[i / 8][i % 8];
-        for (uint64_t i = threadIdx.x; i < 4 * (uint64_t)p.final_count; i += blockDim.x) w[p.final_off + i] = p.final_layer[i];
+    const uint32_t rounds = p.rounds;
+    if (blk == 0) {
+        for (uint32_t i = threadIdx.x; i < 8 * rounds; i += blockDim.x) w[p.roots_off + i] = p.roots[i / 8][i % 8];
+        for (uint64_t i = threadIdx.x; i < 4 * p.final_count; i += blockDim.x) w[p.final_off + i] = p.final_layer[i];
         return;
     }
-    const uint32_t q = (blockIdx.x - 1) / p.rounds, r = (blockIdx.x - 1) % p.rounds;
-    // indices come from the device transcript (mod N/2); the mask keeps every read in bounds
+    const uint32_t q = (blk - 1) / rounds, r = (blk - 1) % rounds;
+    // indices come from the transcript (mod N/2); the mask keeps every read in bounds
     const uint64_t pair0 = (uint64_t)p.query_pairs[q] & ((1ull << (p.log_n - 1)) - 1);
     const uint64_t pair = pair0 >> r;
     const uint32_t depth = p.log_n - r;  // siblings per opening
@@ -531,7 +586,7 @@ __global__ __launch_bounds__(256) void k_emit_proof(EmitParams p)
     const uint32_t open_words = 7 + 8 * depth;
     if (threadIdx.x == 0) {
         if (r == 0) {
-            qs[0] = (uint32_t)pair0; qs[1] = (uint32_t)(pair0 >> 32); qs[2] = p.rounds;
+            qs[0] = (uint32_t)pair0; qs[1] = (uint32_t)(pair0 >> 32); qs[2] = rounds;
         }
         ls[0] = r; ls[1] = (uint32_t)pair; ls[2] = (uint32_t)(pair >> 32);
         for (int side = 0; side < 2; ++side) {
@@ -569,6 +624,7 @@ __global__ __launch_bounds__(256) void k_merkle_path(const uint32_t *__restrict_
 // ============================================================== host launchers
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+static inline size_t tile_lds_bytes(uint32_t log_t) { return (size_t)24 << log_t; }  // 16T + 8T bytes
 
 void launch_keccakf_batch(hipStream_t st, const void *in, uint32_t in_stride, void *out, uint32_t out_stride, uint32_t count)
 {
@@ -590,42 +646,9 @@ void launch_oneblock_hash(hipStream_t st, const void *in, uint32_t stride, uint3
                            stride, len, (uint8_t *)out, out_stride, count, suffix);
 }
 
-static inline size_t tile_lds_bytes(uint32_t log_t) { return (size_t)24 << log_t; }  // 16T + 8T bytes
-
-void launch_merkle_raw(hipStream_t st, const void *leaves, uint32_t stride, uint32_t len, uint64_t m, uint32_t log_t,
-                       uint32_t *tree)
+void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out)
 {
-    uint32_t tiles = (uint32_t)(m >> log_t);
-    bool aligned = ((stride & 3u) == 0) && ((reinterpret_cast<uintptr_t>(leaves) & 3u) == 0);
-    size_t lds = tile_lds_bytes(log_t);
-    if (aligned)
-        hipLaunchKernelGGL(k_merkle_raw_tiles<true>, dim3(tiles), dim3(256), lds, st, (const uint8_t *)leaves, stride, len,
-                           log_t, tree, m);
-    else
-        hipLaunchKernelGGL(k_merkle_raw_tiles<false>, dim3(tiles), dim3(256), lds, st, (const uint8_t *)leaves, stride,
-                           len, log_t, tree, m);
-}
-
-void launch_tree_upper(hipStream_t st, uint32_t *tree, uint64_t m, uint32_t level, uint32_t max_log_c)
-{
-    // level `level` holds m >> level nodes; reduce in chunks of up to 2^max_log_c.
-    uint32_t log_m = 0;
-    while ((1ull << log_m) < m) ++log_m;
-    while (level < log_m) {
-        uint32_t remaining = log_m - level;
-        uint32_t log_c = remaining < max_log_c ? remaining : max_log_c;
-        uint64_t nodes = m >> level;
-        uint32_t blocks = (uint32_t)(nodes >> log_c);
-        size_t lds = (size_t)(8u << log_c) * 4 + (size_t)(4u << log_c) * 4;
-        hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(256), lds, st, tree, m, (int)level, log_c);
-        level += log_c;
-    }
-}
-
-void launch_copy_words(hipStream_t st, const uint32_t *src, uint32_t *dst, uint64_t n)
-{
-    if (!n) return;
-    hipLaunchKernelGGL(k_copy_words, dim3(blocks_for(n, 256)), dim3(256), 0, st, src, dst, n);
+    hipLaunchKernelGGL(k_merkle_path, dim3(blocks_for(8 * depth + 8, 256)), dim3(256), 0, st, tree, m, depth, leaf, out);
 }
 
 void launch_fft_twiddles(hipStream_t st, uint32_t *out, uint32_t log_n)
@@ -641,51 +664,62 @@ void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t 
     hipLaunchKernelGGL(k_inverse, dim3(blocks_for(n, 256)), dim3(256), 0, st, in, out, n);
 }
 
-void launch_fft_pass(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t lo,
-                     uint32_t k, uint32_t m)
+void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t)
 {
-    uint32_t tiles = (uint32_t)(1ull << (log_n - k - m));
-    hipLaunchKernelGGL(k_fft_pass, dim3(tiles), dim3(256), (size_t)16 << (k + m), st, (const uint4 *)src, (uint4 *)dst, tw,
-                       log_n, lo, k, m);
+    if (!blocks) return;
+    if (aligned)
+        hipLaunchKernelGGL(k_merkle_raw_tiles<true>, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
+    else
+        hipLaunchKernelGGL(k_merkle_raw_tiles<false>, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
 }
 
-void launch_fft_final(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
-                      bool run_fft, uint32_t *tree)
+void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
 {
-    uint32_t tiles = (uint32_t)(1ull << (log_n - log_t));
-    hipLaunchKernelGGL(k_fft_final, dim3(tiles), dim3(256), tile_lds_bytes(log_t), st, (const uint4 *)src, (uint4 *)dst,
-                       tw, log_n, log_t, (uint32_t)run_fft, tree);
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), (size_t)16 << max_log_t, st, t, n);
 }
 
-void launch_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
-                 uint32_t log_t, uint32_t *tree)
+void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
 {
-    uint32_t tiles = (uint32_t)(n_out >> log_t);
-    hipLaunchKernelGGL(k_fold, dim3(tiles), dim3(256), tile_lds_bytes(log_t), st, (const uint4 *)in, (uint4 *)out, tw, ch,
-                       log_t, tree, n_out);
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_fft_final, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
 }
 
-void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out)
+void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
 {
-    hipLaunchKernelGGL(k_merkle_path, dim3(blocks_for(8 * depth + 8, 256)), dim3(256), 0, st, tree, m, depth, leaf, out);
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
 }
 
-void launch_fold_leafwise(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch,
-                         uint64_t n_out, uint32_t log_t, uint32_t *tree)
+void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks)
 {
-    uint32_t tiles = (uint32_t)(n_out >> log_t);
-    hipLaunchKernelGGL(k_fold_leafwise, dim3(tiles), dim3(256), 0, st, (const uint4 *)in, (uint4 *)out, tw, ch, log_t,
-                       tree, n_out);
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_fold_leafwise, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
-void launch_transcript(hipStream_t st, const TranscriptOp *ops, uint32_t nops, uint64_t *state)
+void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c)
 {
-    hipLaunchKernelGGL(k_transcript, dim3(1), dim3(64), 0, st, ops, nops, state);
+    if (!blocks) return;
+    size_t lds = (size_t)(8u << max_log_c) * 4 + (size_t)(4u << max_log_c) * 4;
+    hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(256), lds, st, t, n);
 }
 
-void launch_emit_proof(hipStream_t st, const EmitParams &p, uint32_t queries)
+void launch_transcript(hipStream_t st, const TranscriptDesc *t, uint32_t n)
 {
-    hipLaunchKernelGGL(k_emit_proof, dim3(1 + queries * p.rounds), dim3(256), 0, st, p);
+    if (!n) return;
+    hipLaunchKernelGGL(k_transcript, dim3(n), dim3(64), 0, st, t, n);
+}
+
+void launch_copy(hipStream_t st, const CopyDesc *t, uint32_t n, uint32_t blocks)
+{
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, t, n);
+}
+
+void launch_emit_proof(hipStream_t st, const EmitParams *t, uint32_t n, uint32_t blocks)
+{
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_emit_proof, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
 } // namespace azk

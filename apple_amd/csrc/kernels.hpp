@@ -1,4 +1,9 @@
 // Host-visible kernel parameter blocks and launchers (implemented in kernels.hip).
+//
+// Every tiled kernel is driven by a table of descriptors in device memory: one
+// descriptor per (job, phase) with the index of its first block; a block finds its
+// descriptor by binary search over `first_block`.  A single job is a table of one
+// entry; a batch of uneven jobs is one launch over all of their tiles.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -21,6 +26,67 @@ struct TranscriptOp {
     void *dst;
 };
 
+// Stages lo + k - 1 .. lo of one job's Circle FFT, tiles of 2^(k + m) elements.
+struct FftPassDesc {
+    const uint4 *src;
+    uint4 *dst;
+    const uint32_t *tw;
+    uint32_t log_n, lo, k, m;
+    uint32_t first_block, pad;
+};
+
+// Final FFT stages (run_fft) on contiguous tiles of 2^log_t elements, then leaf
+// hashes and tile subtrees into `tree` (if set).
+struct FftFinalDesc {
+    const uint4 *src;
+    uint4 *dst;
+    const uint32_t *tw;
+    uint32_t *tree;
+    uint32_t log_n, log_t, run_fft, first_block;
+};
+
+// One FRI fold round of one job (2 n_out -> n_out), fused with the new layer's
+// leaf hashes and tile subtrees (if `tree`).
+struct FoldDesc {
+    const uint4 *in;
+    uint4 *out;
+    const uint32_t *tw;
+    const uint32_t *ch;
+    uint32_t *tree;
+    uint64_t n_out;
+    uint32_t log_t, first_block;
+};
+
+// Reduction of chunks of 2^log_c nodes of `level` up by log_c levels.
+struct TreeDesc {
+    uint32_t *tree;
+    uint64_t m;  // leaves of the tree
+    uint32_t level, log_c, first_block, pad;
+};
+
+// A transcript program run by one wave.
+struct TranscriptDesc {
+    const TranscriptOp *ops;
+    uint64_t *state;
+    uint32_t nops, first_block;
+};
+
+// Word copy of `words` words, 1024 words per block.
+struct CopyDesc {
+    const uint32_t *src;
+    uint32_t *dst;
+    uint64_t words;
+    uint32_t first_block, pad;
+};
+
+// Raw-leaf Merkle tiles.
+struct MerkleRawDesc {
+    const uint8_t *leaves;
+    uint32_t *tree;
+    uint64_t m;
+    uint32_t stride, len, log_t, first_block;
+};
+
 struct EmitParams {
     uint32_t *proof;                 // proof as LE u32 words
     uint64_t roots_off, final_off, queries_off, query_words;
@@ -32,27 +98,25 @@ struct EmitParams {
     const uint32_t *trees[kMaxRounds];
     const uint4 *layers[kMaxRounds];
     const uint32_t *query_pairs;
+    uint32_t queries, first_block;   // blocks: 1 + queries * rounds
 };
 
 void launch_keccakf_batch(hipStream_t st, const void *in, uint32_t in_stride, void *out, uint32_t out_stride, uint32_t count);
 void launch_oneblock_hash(hipStream_t st, const void *in, uint32_t stride, uint32_t len, void *out, uint32_t out_stride,
                           uint32_t count, uint32_t suffix);
-void launch_merkle_raw(hipStream_t st, const void *leaves, uint32_t stride, uint32_t len, uint64_t m, uint32_t log_t,
-                       uint32_t *tree);
-void launch_tree_upper(hipStream_t st, uint32_t *tree, uint64_t m, uint32_t level, uint32_t max_log_c);
-void launch_copy_words(hipStream_t st, const uint32_t *src, uint32_t *dst, uint64_t n);
+void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out);
 void launch_fft_twiddles(hipStream_t st, uint32_t *out, uint32_t log_n);
 void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n);
-void launch_fft_pass(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t lo,
-                     uint32_t k, uint32_t m);
-void launch_fft_final(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
-                      bool run_fft, uint32_t *tree);
-void launch_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
-                 uint32_t log_t, uint32_t *tree);
-void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out);
-void launch_fold_leafwise(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch,
-                         uint64_t n_out, uint32_t log_t, uint32_t *tree);
-void launch_transcript(hipStream_t st, const TranscriptOp *ops, uint32_t nops, uint64_t *state);
-void launch_emit_proof(hipStream_t st, const EmitParams &p, uint32_t queries);
+
+// Table-driven launches: `blocks` = total blocks over all descriptors.
+void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t);
+void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
+void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
+void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
+void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
+void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c);
+void launch_transcript(hipStream_t st, const TranscriptDesc *t, uint32_t n);
+void launch_copy(hipStream_t st, const CopyDesc *t, uint32_t n, uint32_t blocks);
+void launch_emit_proof(hipStream_t st, const EmitParams *t, uint32_t n, uint32_t blocks);
 
 } // namespace azk

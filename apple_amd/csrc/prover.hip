@@ -1,23 +1,26 @@
-// Circle PCS/FRI commit superstep on one device, and its batched multi-job form.
+// Circle PCS/FRI commit superstep, for one job or a batch of uneven jobs.
 //
 // Reference: CircleCodewordPCSFRIProverV1 (Sources/AppleZKProver/CircleCodeword.swift:920-1419),
 // CirclePCSFRIResidentProverV1 + CircleFRIResidentQueryExtractorV1
 // (CircleFRIFoldPlan.swift:693-1231), transcript CircleFRITranscriptV1 and codec
 // CirclePCSFRIProofCodecV1 (CircleProofFormat.swift:475-983).
 //
-// Flow of one prove (two captured graphs around one host step):
-//   graph A: Circle FFT -> layer-0 leaf hashes + tile subtrees -> upper tree ->
-//            per round r: transcript A(r) {root r, challenge frame r, squeeze 4}
-//                         || transcript B(r+1) {commitment frame r+1} on a side stream,
-//                         fold r fused with layer r+1 hashing/tiles -> upper tree
-//            -> D2H of the transcript state and the final layer
-//   host:    final-layer frame + final layer + query frame absorbed, Q indices
-//            squeezed (the reference also derives these on the CPU after readback,
-//            CircleFRIFoldPlan.swift:875-897; a single-wave device sponge costs ~9 us
-//            per permutation, the 32 KiB final layer of the 2^21 shape is 241 of them)
-//   graph B: H2D of the indices -> proof gather (roots, final layer, openings) -> D2H
+// A Runner turns a list of jobs into a fixed program of table-driven launches, where
+// each launch covers that phase for every job (so small jobs' latency-bound tails
+// overlap), captured as two graphs around one host step:
+//   graph A: Circle FFT passes -> final pass fused with layer-0 leaf hashes + tile
+//            subtrees -> upper trees -> per round r: transcript A(r) {root r,
+//            challenge frame r, squeeze 4} || transcript B(r+1) {commitment frame r+1}
+//            on a side stream; fold r fused with layer r+1 hashing/tiles; upper trees
+//            -> gather of every job's transcript state + final layer -> one D2H copy
+//   host:    per job, final-layer frame + final layer + query frame absorbed and the
+//            Q indices squeezed (the reference also derives these on the CPU after
+//            readback, CircleFRIFoldPlan.swift:875-897; one wave runs a serial sponge
+//            at ~9 us per permutation, the 2^21 shape's final layer is 241 of them)
+//   graph B: one H2D copy of all indices -> proof gather for every job -> one D2H copy
 // The transcript prefix (header, domain, security, public-input frames, digest,
-// commitment frame 0) is constant per plan and absorbed once on the host at creation.
+// commitment frame 0) is constant per prover and absorbed once on the host.
+#include <memory>
 #include <thread>
 
 #include "host_sha3.hpp"
@@ -70,6 +73,8 @@ struct PhaseClock {
 };
 enum { PH_START = 0, PH_FFT, PH_COMMIT0, PH_FOLD, PH_TREE, PH_TRANSCRIPT, PH_TAIL, PH_HOST };
 
+struct Runner;
+
 } // namespace
 
 struct azk_circle_prover {
@@ -77,54 +82,71 @@ struct azk_circle_prover {
     azk_circle_prover_desc d{};
     uint32_t log_t = kDefaultLogTile;
     uint64_t n = 0, nfin = 0;
-    DeviceMem fft_tw, fold_tw, arena, frames, ops, proof, coeff_staging;
-    PinnedMem host_proof, host_tail, host_qidx;
+    DeviceMem fft_tw, fold_tw, arena, frames, ops, coeff_staging;
     std::vector<uint4 *> layers;    // rounds + 1 layers (layer 0 = codeword, layer R = final)
     std::vector<uint32_t *> trees;  // one full tree per committed layer
-    uint32_t *challenges = nullptr, *qidx = nullptr;
+    uint32_t *challenges = nullptr;
     uint64_t *tstate = nullptr, *tstate_init = nullptr;
     std::vector<std::pair<uint32_t, uint32_t>> prog_a, prog_b;  // (first op, count) per round
     std::vector<uint8_t> final_frame, query_frame;
+    std::vector<uint32_t> proof_template;
     uint64_t roots_off = 0, final_off = 0, queries_off = 0, query_words = 0, proof_words = 0;
     std::vector<uint32_t> layer_off;
-    hipStream_t own = nullptr, side = nullptr;
-    std::vector<hipEvent_t> ev_a, ev_b;
     bool use_graph = true, timing = false;
-    hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
-    const void *graph_src = nullptr;
-    int graph_mode = -1;
+    std::unique_ptr<Runner> runner;  // cached single-job program
     azk_phase_times phases{};
     std::mutex mu;
-    ~azk_circle_prover()
+    ~azk_circle_prover();
+};
+
+struct azk_circle_batch;
+
+namespace {
+
+struct Job {
+    azk_circle_prover *p;
+    const void *src;
+    int mode;  // 0: src = line-basis coefficients, 1: src = codeword
+    const uint4 *layer0() const { return mode == 0 ? p->layers[0] : (const uint4 *)src; }
+};
+
+struct Runner {
+    azk_context *ctx = nullptr;
+    std::vector<Job> jobs;
+    TableArena tables;
+    std::vector<Launch<FftPassDesc>> fft_pass;
+    Launch<FftFinalDesc> fft_final;
+    std::vector<Launch<TreeDesc>> tree0;
+    uint32_t rounds = 0;
+    std::vector<Launch<TranscriptDesc>> tr_a, tr_b;
+    std::vector<Launch<FoldDesc>> fold_big, fold_leaf;
+    std::vector<std::vector<Launch<TreeDesc>>> tree_r;
+    Launch<CopyDesc> tail;
+    Launch<EmitParams> emit;
+    DeviceMem tail_dev, qidx_dev, proof_dev;
+    PinnedMem tail_host, qidx_host, proof_host;
+    std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job
+    hipStream_t side = nullptr;
+    std::vector<hipEvent_t> ev_a, ev_b;
+    hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
+    ~Runner()
     {
         if (graph_a) (void)hipGraphExecDestroy(graph_a);
         if (graph_b) (void)hipGraphExecDestroy(graph_b);
         for (auto e : ev_a) (void)hipEventDestroy(e);
         for (auto e : ev_b) (void)hipEventDestroy(e);
-        if (own) (void)hipStreamDestroy(own);
         if (side) (void)hipStreamDestroy(side);
     }
 };
-
-namespace {
 
 azk_status prover_build(azk_circle_prover *p)
 {
     const azk_circle_prover_desc &d = p->d;
     const uint64_t n = p->n, nfin = p->nfin;
-    azk_context *ctx = p->ctx;
-    azk_status s = materialize_twiddles(ctx, d.log_n, d.rounds, p->fft_tw, &p->fold_tw);
+    azk_status s = materialize_twiddles(p->ctx, d.log_n, d.rounds, p->fft_tw, &p->fold_tw);
     if (s) return s;
-    HIP_TRY(hipStreamCreateWithFlags(&p->own, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-    p->ev_a.resize(d.rounds);
-    p->ev_b.resize(d.rounds);
-    for (uint32_t r = 0; r < d.rounds; ++r) {
-        HIP_TRY(hipEventCreateWithFlags(&p->ev_a[r], hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&p->ev_b[r], hipEventDisableTiming));
-    }
 
-    // ---- arena: layers, full trees, transcript scratch
+    // ---- arena: layers, full trees, transcript state
     std::vector<size_t> layer_at(d.rounds + 1), tree_at(d.rounds);
     size_t at = 0;
     for (uint32_t r = 0; r <= d.rounds; ++r) {
@@ -137,8 +159,6 @@ azk_status prover_build(azk_circle_prover *p)
     }
     const size_t ch_at = at;
     at = align_up(at + 16 * (size_t)d.rounds, 256);
-    const size_t q_at = at;
-    at = align_up(at + 4 * (size_t)d.query_count, 256);
     const size_t st_at = at;
     at = align_up(at + 200, 256);
     const size_t st0_at = at;
@@ -151,7 +171,6 @@ azk_status prover_build(azk_circle_prover *p)
     for (uint32_t r = 0; r <= d.rounds; ++r) p->layers[r] = (uint4 *)(base + layer_at[r]);
     for (uint32_t r = 0; r < d.rounds; ++r) p->trees[r] = (uint32_t *)(base + tree_at[r]);
     p->challenges = (uint32_t *)(base + ch_at);
-    p->qidx = (uint32_t *)(base + q_at);
     p->tstate = (uint64_t *)(base + st_at);
     p->tstate_init = (uint64_t *)(base + st0_at);
 
@@ -253,120 +272,187 @@ azk_status prover_build(azk_circle_prover *p)
     }
     p->query_words = qw;
     p->proof_words = p->queries_off + (uint64_t)d.query_count * qw;
-    std::vector<uint32_t> tmpl(p->proof_words, 0);
-    memcpy(tmpl.data(), head.data(), head.size());
-    tmpl[nfin_at] = (uint32_t)nfin;
-    tmpl[nfin_at + 1] = (uint32_t)(nfin >> 32);
-    tmpl[q_count_at] = d.query_count;
-    s = p->proof.ensure(4 * p->proof_words);
-    if (!s) s = p->host_proof.ensure(4 * p->proof_words);
-    if (!s) s = p->host_tail.ensure(200 + 16 * nfin);
-    if (!s) s = p->host_qidx.ensure(4 * (size_t)d.query_count);
-    if (s) return s;
-    HIP_TRY(hipMemcpy(p->proof.p, tmpl.data(), 4 * p->proof_words, hipMemcpyHostToDevice));
+    p->proof_template.assign(p->proof_words, 0);
+    memcpy(p->proof_template.data(), head.data(), head.size());
+    p->proof_template[nfin_at] = (uint32_t)nfin;
+    p->proof_template[nfin_at + 1] = (uint32_t)(nfin >> 32);
+    p->proof_template[q_count_at] = d.query_count;
     return AZK_OK;
 }
 
-EmitParams emit_params(azk_circle_prover *p, const uint4 *layer0)
+// Builds the launch program of a job list.
+azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &jobs)
 {
-    EmitParams e{};
-    e.proof = (uint32_t *)p->proof.p;
-    e.roots_off = p->roots_off;
-    e.final_off = p->final_off;
-    e.queries_off = p->queries_off;
-    e.query_words = p->query_words;
-    e.rounds = p->d.rounds;
-    e.log_n = p->d.log_n;
-    e.final_count = p->nfin;
-    for (uint32_t r = 0; r < p->d.rounds; ++r) {
-        e.layer_off[r] = p->layer_off[r];
-        e.roots[r] = p->trees[r] + 8 * (2 * (p->n >> r) - 2);
-        e.trees[r] = p->trees[r];
-        e.layers[r] = r == 0 ? layer0 : p->layers[r];
+    R.ctx = ctx;
+    R.jobs = jobs;
+    for (const Job &j : jobs) R.rounds = std::max(R.rounds, j.p->d.rounds);
+    R.tr_a.resize(R.rounds);
+    R.tr_b.resize(R.rounds);
+    R.fold_big.resize(R.rounds);
+    R.fold_leaf.resize(R.rounds);
+    R.tree_r.resize(R.rounds);
+    size_t tail_bytes = 0, qidx_bytes = 0, proof_bytes = 0;
+    for (const Job &j : jobs) {
+        azk_circle_prover *p = j.p;
+        const uint32_t R_j = p->d.rounds;
+        const TranscriptOp *ops = (const TranscriptOp *)p->ops.p;
+        plan_fft(R.fft_pass, R.fft_final, j.src, p->layers[0], (const uint32_t *)p->fft_tw.p, p->d.log_n, p->log_t,
+                 j.mode == 0, p->trees[0]);
+        plan_tree(R.tree0, p->trees[0], p->n, std::min(p->log_t, p->d.log_n));
+        uint64_t off = 0, m = p->n;
+        for (uint32_t r = 0; r < R_j; ++r) {
+            R.tr_a[r].add(TranscriptDesc{ops + p->prog_a[r].first, p->tstate, p->prog_a[r].second, 0}, 1);
+            if (r + 1 < R_j)
+                R.tr_b[r + 1].add(TranscriptDesc{ops + p->prog_b[r + 1].first, p->tstate, p->prog_b[r + 1].second, 0}, 1);
+            const uint64_t n_out = m / 2;
+            const bool commit_next = r + 1 < R_j;
+            const uint32_t lt = plan_fold(R.fold_big[r], R.fold_leaf[r], r == 0 ? (const void *)j.layer0() : (const void *)p->layers[r],
+                                          p->layers[r + 1], (const uint32_t *)p->fold_tw.p + off, p->challenges + 4 * r, n_out,
+                                          p->log_t, commit_next ? p->trees[r + 1] : nullptr);
+            if (commit_next) plan_tree(R.tree_r[r], p->trees[r + 1], n_out, lt);
+            off += n_out;
+            m = n_out;
+        }
+        R.tail_off.push_back(tail_bytes);
+        tail_bytes = align_up(tail_bytes + 200 + 16 * p->nfin, 256);
+        R.qidx_off.push_back(qidx_bytes);
+        qidx_bytes = align_up(qidx_bytes + 4 * (size_t)p->d.query_count, 256);
+        R.proof_off.push_back(proof_bytes);
+        proof_bytes = align_up(proof_bytes + 4 * p->proof_words, 256);
     }
-    e.final_layer = (const uint32_t *)p->layers[p->d.rounds];
-    e.query_pairs = p->qidx;
-    return e;
+    azk_status s = R.tail_dev.ensure(tail_bytes);
+    if (!s) s = R.qidx_dev.ensure(qidx_bytes);
+    if (!s) s = R.proof_dev.ensure(proof_bytes);
+    if (!s) s = R.tail_host.ensure(tail_bytes);
+    if (!s) s = R.qidx_host.ensure(qidx_bytes);
+    if (!s) s = R.proof_host.ensure(proof_bytes);
+    if (s) return s;
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        azk_circle_prover *p = jobs[k].p;
+        uint32_t *tail = (uint32_t *)((uint8_t *)R.tail_dev.p + R.tail_off[k]);
+        R.tail.add(CopyDesc{(const uint32_t *)p->tstate, tail, 50, 0, 0}, 1);
+        const uint64_t fw = 4 * p->nfin;
+        R.tail.add(CopyDesc{(const uint32_t *)p->layers[p->d.rounds], tail + 50, fw, 0, 0}, (uint32_t)((fw + 1023) / 1024));
+        // proof words start from the constant template; the gather fills the rest
+        uint32_t *proof = (uint32_t *)((uint8_t *)R.proof_dev.p + R.proof_off[k]);
+        HIP_TRY(hipMemcpy(proof, p->proof_template.data(), 4 * p->proof_words, hipMemcpyHostToDevice));
+        EmitParams e{};
+        e.proof = proof;
+        e.roots_off = p->roots_off;
+        e.final_off = p->final_off;
+        e.queries_off = p->queries_off;
+        e.query_words = p->query_words;
+        e.rounds = p->d.rounds;
+        e.log_n = p->d.log_n;
+        e.final_count = p->nfin;
+        for (uint32_t r = 0; r < p->d.rounds; ++r) {
+            e.layer_off[r] = p->layer_off[r];
+            e.roots[r] = p->trees[r] + 8 * (2 * (p->n >> r) - 2);
+            e.trees[r] = p->trees[r];
+            e.layers[r] = r == 0 ? jobs[k].layer0() : p->layers[r];
+        }
+        e.final_layer = (const uint32_t *)p->layers[p->d.rounds];
+        e.query_pairs = (const uint32_t *)((uint8_t *)R.qidx_dev.p + R.qidx_off[k]);
+        e.queries = p->d.query_count;
+        R.emit.add(e, 1 + p->d.query_count * p->d.rounds);
+    }
+    // pack every table into one device buffer
+    for (auto &l : R.fft_pass) l.off = R.tables.put(l.v);
+    R.fft_final.off = R.tables.put(R.fft_final.v);
+    for (auto &l : R.tree0) l.off = R.tables.put(l.v);
+    for (uint32_t r = 0; r < R.rounds; ++r) {
+        R.tr_a[r].off = R.tables.put(R.tr_a[r].v);
+        R.tr_b[r].off = R.tables.put(R.tr_b[r].v);
+        R.fold_big[r].off = R.tables.put(R.fold_big[r].v);
+        R.fold_leaf[r].off = R.tables.put(R.fold_leaf[r].v);
+        for (auto &l : R.tree_r[r]) l.off = R.tables.put(l.v);
+    }
+    R.tail.off = R.tables.put(R.tail.v);
+    R.emit.off = R.tables.put(R.emit.v);
+    s = R.tables.upload();
+    if (s) return s;
+    HIP_TRY(hipStreamCreateWithFlags(&R.side, hipStreamNonBlocking));
+    R.ev_a.resize(R.rounds);
+    R.ev_b.resize(R.rounds);
+    for (uint32_t r = 0; r < R.rounds; ++r) {
+        HIP_TRY(hipEventCreateWithFlags(&R.ev_a[r], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&R.ev_b[r], hipEventDisableTiming));
+    }
+    return AZK_OK;
 }
 
-const uint4 *layer0_of(azk_circle_prover *p, const void *src, int mode)
-{
-    return mode == 0 ? p->layers[0] : (const uint4 *)src;
-}
+template <class D>
+const D *tab(const Runner &R, const Launch<D> &l) { return R.tables.at<D>(l.off); }
 
-// Graph A (see the file header).  mode 0: src = line-basis coefficients; mode 1: src = codeword.
-azk_status enqueue_a(azk_circle_prover *p, hipStream_t st, hipStream_t side, const void *src, int mode, PhaseClock &clk)
+azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
 {
-    const azk_circle_prover_desc &d = p->d;
-    const uint32_t lt0 = std::min(p->log_t, d.log_n);
-    const uint4 *layer0 = layer0_of(p, src, mode);
-    const TranscriptOp *ops = (const TranscriptOp *)p->ops.p;
     clk.mark(PH_START);
-    if (mode == 0) {
-        const void *cur = src;
-        for (const FftPass &fp : fft_passes(d.log_n, p->log_t)) {
-            launch_fft_pass(st, cur, p->layers[0], (const uint32_t *)p->fft_tw.p, d.log_n, fp.lo, fp.k, fp.m);
-            cur = p->layers[0];
-        }
-        clk.mark(PH_FFT);
-        launch_fft_final(st, cur, p->layers[0], (const uint32_t *)p->fft_tw.p, d.log_n, lt0, true, p->trees[0]);
-    } else {
-        clk.mark(PH_FFT);
-        launch_fft_final(st, src, nullptr, nullptr, d.log_n, lt0, false, p->trees[0]);
-    }
+    for (auto &l : R.fft_pass) launch_fft_pass(st, tab(R, l), l.n(), l.blocks, l.max_log);
+    clk.mark(PH_FFT);
+    launch_fft_final(st, tab(R, R.fft_final), R.fft_final.n(), R.fft_final.blocks, R.fft_final.max_log);
     clk.mark(PH_COMMIT0);
-    launch_tree_upper(st, p->trees[0], p->n, lt0, kUpperLogChunk);
+    for (auto &l : R.tree0) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log);
     clk.mark(PH_TREE);
-    uint64_t off = 0, m = p->n;
-    for (uint32_t r = 0; r < d.rounds; ++r) {
-        if (r > 0) HIP_TRY(hipStreamWaitEvent(st, p->ev_b[r], 0));
-        launch_transcript(st, ops + p->prog_a[r].first, p->prog_a[r].second, p->tstate);
+    for (uint32_t r = 0; r < R.rounds; ++r) {
+        if (r > 0 && !R.tr_b[r].empty()) HIP_TRY(hipStreamWaitEvent(st, R.ev_b[r], 0));
+        launch_transcript(st, tab(R, R.tr_a[r]), R.tr_a[r].n());
         clk.mark(PH_TRANSCRIPT);
-        const bool commit_next = r + 1 < d.rounds;
-        if (commit_next) {
-            HIP_TRY(hipEventRecord(p->ev_a[r], st));
-            HIP_TRY(hipStreamWaitEvent(side, p->ev_a[r], 0));
-            launch_transcript(side, ops + p->prog_b[r + 1].first, p->prog_b[r + 1].second, p->tstate);
-            HIP_TRY(hipEventRecord(p->ev_b[r + 1], side));
+        if (r + 1 < R.rounds && !R.tr_b[r + 1].empty()) {
+            HIP_TRY(hipEventRecord(R.ev_a[r], st));
+            HIP_TRY(hipStreamWaitEvent(R.side, R.ev_a[r], 0));
+            launch_transcript(R.side, tab(R, R.tr_b[r + 1]), R.tr_b[r + 1].n());
+            HIP_TRY(hipEventRecord(R.ev_b[r + 1], R.side));
         }
-        const uint64_t n_out = m / 2;
-        const uint32_t lt = enqueue_fold(st, r == 0 ? (const void *)layer0 : (const void *)p->layers[r], p->layers[r + 1],
-                                         (const uint32_t *)p->fold_tw.p + off, p->challenges + 4 * r, n_out, p->log_t,
-                                         commit_next ? p->trees[r + 1] : nullptr);
+        auto &fb = R.fold_big[r], &fl = R.fold_leaf[r];
+        launch_fold(st, tab(R, fb), fb.n(), fb.blocks, fb.max_log);
+        launch_fold_leafwise(st, tab(R, fl), fl.n(), fl.blocks);
         clk.mark(PH_FOLD);
-        if (commit_next) {
-            launch_tree_upper(st, p->trees[r + 1], n_out, lt, kUpperLogChunk);
-            clk.mark(PH_TREE);
-        }
-        off += n_out;
-        m = n_out;
+        for (auto &l : R.tree_r[r]) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log);
+        clk.mark(PH_TREE);
     }
-    uint8_t *tail = (uint8_t *)p->host_tail.p;
-    HIP_TRY(hipMemcpyAsync(tail, p->tstate, 200, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(tail + 200, p->layers[d.rounds], 16 * p->nfin, hipMemcpyDeviceToHost, st));
+    launch_copy(st, tab(R, R.tail), R.tail.n(), R.tail.blocks);
+    HIP_TRY(hipMemcpyAsync(R.tail_host.p, R.tail_dev.p, R.tail_dev.n, hipMemcpyDeviceToHost, st));
     clk.mark(PH_TAIL);
     return AZK_OK;
 }
 
-// Host step: final-layer frame + final layer + query frame, then Q indices mod N/2
-// (CircleFRITranscriptV1.derive, CircleProofFormat.swift:687-722; grinding is 0).
-void host_final(azk_circle_prover *p)
+// Host step of one job: final-layer frame + final layer + query frame, then Q indices
+// mod N/2 (CircleFRITranscriptV1.derive, CircleProofFormat.swift:687-722; grinding is 0).
+void host_final(const Runner &R, size_t k)
 {
+    const azk_circle_prover *p = R.jobs[k].p;
     host::Transcript t;
-    const uint8_t *tail = (const uint8_t *)p->host_tail.p;
+    const uint8_t *tail = (const uint8_t *)R.tail_host.p + R.tail_off[k];
     memcpy(t.s, tail, 200);
     t.absorb(p->final_frame.data(), p->final_frame.size());
     t.absorb(tail + 200, 16 * p->nfin);
     t.absorb(p->query_frame.data(), p->query_frame.size());
-    t.squeeze_u32(p->d.query_count, (uint32_t)(p->n / 2), (uint32_t *)p->host_qidx.p);
+    t.squeeze_u32(p->d.query_count, (uint32_t)(p->n / 2), (uint32_t *)((uint8_t *)R.qidx_host.p + R.qidx_off[k]));
 }
 
-// Graph B: indices up, proof gather, proof down.
-azk_status enqueue_b(azk_circle_prover *p, hipStream_t st, const void *src, int mode)
+void host_finals(const Runner &R)
 {
-    HIP_TRY(hipMemcpyAsync(p->qidx, p->host_qidx.p, 4 * (size_t)p->d.query_count, hipMemcpyHostToDevice, st));
-    launch_emit_proof(st, emit_params(p, layer0_of(p, src, mode)), p->d.query_count);
-    HIP_TRY(hipMemcpyAsync(p->host_proof.p, p->proof.p, 4 * p->proof_words, hipMemcpyDeviceToHost, st));
+    const size_t jobs = R.jobs.size();
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nt = std::min(jobs, hw);
+    if (nt <= 1) {
+        for (size_t k = 0; k < jobs; ++k) host_final(R, k);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; ++t)
+        pool.emplace_back([&R, t, nt, jobs]() {
+            for (size_t k = t; k < jobs; k += nt) host_final(R, k);
+        });
+    for (size_t k = 0; k < jobs; k += nt) host_final(R, k);
+    for (auto &th : pool) th.join();
+}
+
+azk_status enqueue_b(Runner &R, hipStream_t st)
+{
+    HIP_TRY(hipMemcpyAsync(R.qidx_dev.p, R.qidx_host.p, R.qidx_dev.n, hipMemcpyHostToDevice, st));
+    launch_emit_proof(st, tab(R, R.emit), R.emit.n(), R.emit.blocks);
+    HIP_TRY(hipMemcpyAsync(R.proof_host.p, R.proof_dev.p, R.proof_dev.n, hipMemcpyDeviceToHost, st));
     return AZK_OK;
 }
 
@@ -388,50 +474,40 @@ azk_status capture(hipStream_t st, hipGraphExec_t *out, F &&enqueue)
     return AZK_OK;
 }
 
-azk_status ensure_graphs(azk_circle_prover *p, hipStream_t st, const void *src, int mode)
+azk_status ensure_graphs(Runner &R, hipStream_t st)
 {
-    if (!p->graph_a || p->graph_src != src || p->graph_mode != mode) {
-        if (p->graph_a) (void)hipGraphExecDestroy(p->graph_a);
-        if (p->graph_b) (void)hipGraphExecDestroy(p->graph_b);
-        p->graph_a = p->graph_b = nullptr;
-        PhaseClock none(st, false);
-        azk_status s = capture(st, &p->graph_a, [&]() { return enqueue_a(p, st, p->side, src, mode, none); });
-        if (!s) s = capture(st, &p->graph_b, [&]() { return enqueue_b(p, st, src, mode); });
-        if (s) return s;
-        p->graph_src = src;
-        p->graph_mode = mode;
-    }
-    return AZK_OK;
+    if (R.graph_a) return AZK_OK;
+    PhaseClock none(st, false);
+    azk_status s = capture(st, &R.graph_a, [&]() { return enqueue_a(R, st, none); });
+    if (!s) s = capture(st, &R.graph_b, [&]() { return enqueue_b(R, st); });
+    return s;
 }
 
-azk_status prove_locked(azk_circle_prover *p, const void *src, int mode, uint8_t *proof, size_t cap, size_t *len,
-                        azk_stats *stats)
+// Runs a program (graph replay or eager launches with optional phase timing).
+azk_status run(Runner &R, bool graph, azk_phase_times *phases, azk_stats *stats)
 {
-    const size_t bytes = 4 * p->proof_words;
-    if (cap < bytes || !proof) return fail(AZK_ERR_INVALID_LAYOUT, "proof capacity %zu < %zu", cap, bytes);
-    hipStream_t st = p->ctx->stream;
-    const bool graph = p->use_graph && !p->timing;
+    hipStream_t st = R.ctx->stream;
     if (graph) {
-        azk_status s = ensure_graphs(p, st, src, mode);
+        azk_status s = ensure_graphs(R, st);
         if (s) return s;
     }
     Timer t;
     azk_status s = t.init();
     if (s) return s;
-    PhaseClock clk(st, p->timing);
+    PhaseClock clk(st, phases != nullptr);
     const uint64_t w0 = now_ns();
     HIP_TRY(hipEventRecord(t.a, st));
-    if (graph) HIP_TRY(hipGraphLaunch(p->graph_a, st));
-    else if ((s = enqueue_a(p, st, p->side, src, mode, clk))) return s;
+    if (graph) HIP_TRY(hipGraphLaunch(R.graph_a, st));
+    else if ((s = enqueue_a(R, st, clk))) return s;
     HIP_TRY(hipGetLastError());
     const uint64_t w1 = now_ns();
     HIP_TRY(hipStreamSynchronize(st));
     const uint64_t h0 = now_ns();
-    host_final(p);
+    host_finals(R);
     const uint64_t h1 = now_ns();
     clk.mark(PH_HOST);
-    if (graph) HIP_TRY(hipGraphLaunch(p->graph_b, st));
-    else if ((s = enqueue_b(p, st, src, mode))) return s;
+    if (graph) HIP_TRY(hipGraphLaunch(R.graph_b, st));
+    else if ((s = enqueue_b(R, st))) return s;
     HIP_TRY(hipGetLastError());
     clk.mark(PH_TAIL);
     HIP_TRY(hipEventRecord(t.b, st));
@@ -441,7 +517,7 @@ azk_status prove_locked(azk_circle_prover *p, const void *src, int mode, uint8_t
         stats->wall_ns = now_ns() - w0;
         stats->gpu_ns = t.elapsed_ns();
     }
-    if (p->timing && clk.ev.size() > 1) {
+    if (phases && clk.ev.size() > 1) {
         azk_phase_times ph{};
         uint64_t *slot[] = {nullptr, &ph.fft_ns, &ph.commit0_ns, &ph.fold_ns, &ph.tree_ns, &ph.transcript_ns, &ph.emit_ns,
                             nullptr};
@@ -454,14 +530,35 @@ azk_status prove_locked(azk_circle_prover *p, const void *src, int mode, uint8_t
         float ms = 0;
         (void)hipEventElapsedTime(&ms, clk.ev.front(), clk.ev.back());
         ph.total_ns = (uint64_t)(ms * 1e6);
-        p->phases = ph;
+        *phases = ph;
     }
-    memcpy(proof, p->host_proof.p, bytes);
+    return AZK_OK;
+}
+
+const uint8_t *proof_of(const Runner &R, size_t k) { return (const uint8_t *)R.proof_host.p + R.proof_off[k]; }
+
+azk_status prove_single(azk_circle_prover *p, const void *src, int mode, uint8_t *proof, size_t cap, size_t *len,
+                        azk_stats *stats)
+{
+    const size_t bytes = 4 * p->proof_words;
+    if (cap < bytes || !proof) return fail(AZK_ERR_INVALID_LAYOUT, "proof capacity %zu < %zu", cap, bytes);
+    if (!p->runner || p->runner->jobs[0].src != src || p->runner->jobs[0].mode != mode) {
+        p->runner.reset();
+        auto r = std::make_unique<Runner>();
+        azk_status s = runner_build(*r, p->ctx, {Job{p, src, mode}});
+        if (s) return s;
+        p->runner = std::move(r);
+    }
+    azk_status s = run(*p->runner, p->use_graph && !p->timing, p->timing ? &p->phases : nullptr, stats);
+    if (s) return s;
+    memcpy(proof, proof_of(*p->runner, 0), bytes);
     if (len) *len = bytes;
     return AZK_OK;
 }
 
 } // namespace
+
+azk_circle_prover::~azk_circle_prover() = default;
 
 extern "C" azk_status azk_circle_prover_create(azk_context *ctx, const azk_circle_prover_desc *desc,
                                                azk_circle_prover **out)
@@ -537,7 +634,7 @@ extern "C" azk_status azk_circle_prove_fft_coefficients_resident(azk_circle_prov
     if (((uintptr_t)coeffs->ptr + off) % 16) return fail(AZK_ERR_INVALID_LAYOUT, "coefficients must be 16-byte aligned");
     HIP_TRY(hipSetDevice(p->ctx->device));
     std::lock_guard<std::mutex> gc(p->ctx->mu);
-    return prove_locked(p, (uint8_t *)coeffs->ptr + off, 0, proof, cap, len, stats);
+    return prove_single(p, (uint8_t *)coeffs->ptr + off, 0, proof, cap, len, stats);
 }
 
 extern "C" azk_status azk_circle_prove_evaluations_resident(azk_circle_prover *p, const azk_buffer *evals, size_t off,
@@ -550,7 +647,7 @@ extern "C" azk_status azk_circle_prove_evaluations_resident(azk_circle_prover *p
     if (((uintptr_t)evals->ptr + off) % 16) return fail(AZK_ERR_INVALID_LAYOUT, "evaluations must be 16-byte aligned");
     HIP_TRY(hipSetDevice(p->ctx->device));
     std::lock_guard<std::mutex> gc(p->ctx->mu);
-    return prove_locked(p, (uint8_t *)evals->ptr + off, 1, proof, cap, len, stats);
+    return prove_single(p, (uint8_t *)evals->ptr + off, 1, proof, cap, len, stats);
 }
 
 extern "C" azk_status azk_circle_prove_fft_coefficients(azk_circle_prover *p, const void *coeffs, size_t bytes,
@@ -565,7 +662,7 @@ extern "C" azk_status azk_circle_prove_fft_coefficients(azk_circle_prover *p, co
     if (s) return s;
     std::lock_guard<std::mutex> gc(p->ctx->mu);
     HIP_TRY(hipMemcpyAsync(p->coeff_staging.p, coeffs, need, hipMemcpyHostToDevice, p->ctx->stream));
-    return prove_locked(p, p->coeff_staging.p, 0, proof, cap, len, stats);
+    return prove_single(p, p->coeff_staging.p, 0, proof, cap, len, stats);
 }
 
 extern "C" azk_status azk_circle_prover_clear(azk_circle_prover *p)
@@ -573,21 +670,15 @@ extern "C" azk_status azk_circle_prover_clear(azk_circle_prover *p)
     CHECK_ARG(p);
     std::lock_guard<std::mutex> g(p->mu);
     HIP_TRY(hipSetDevice(p->ctx->device));
-    // zero every buffer that held witness-derived data; keep the constant parts
+    std::lock_guard<std::mutex> gc(p->ctx->mu);
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    // zero every buffer that held witness-derived data, then restore the constant prefix state
     std::vector<uint8_t> st0(200);
     HIP_TRY(hipMemcpy(st0.data(), p->tstate_init, 200, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> tmpl(p->proof_words);
-    HIP_TRY(hipMemcpy(tmpl.data(), p->proof.p, 4 * p->proof_words, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemset(p->arena.p, 0, p->arena.n));
     HIP_TRY(hipMemcpy(p->tstate_init, st0.data(), 200, hipMemcpyHostToDevice));
     if (p->coeff_staging.p) HIP_TRY(hipMemset(p->coeff_staging.p, 0, p->coeff_staging.n));
-    memset(tmpl.data() + p->roots_off, 0, 32 * (size_t)p->d.rounds);
-    memset(tmpl.data() + p->final_off, 0, 16 * p->nfin);
-    memset(tmpl.data() + p->queries_off, 0, 4 * (size_t)p->d.query_count * p->query_words);
-    HIP_TRY(hipMemcpy(p->proof.p, tmpl.data(), 4 * p->proof_words, hipMemcpyHostToDevice));
-    memset(p->host_proof.p, 0, p->host_proof.n);
-    memset(p->host_tail.p, 0, p->host_tail.n);
-    memset(p->host_qidx.p, 0, p->host_qidx.n);
+    p->runner.reset();  // its staging held proof bytes
     return AZK_OK;
 }
 
@@ -596,17 +687,7 @@ extern "C" azk_status azk_circle_prover_clear(azk_circle_prover *p)
 struct azk_circle_batch {
     azk_context *ctx = nullptr;
     std::vector<azk_circle_prover *> provers;
-    std::vector<const void *> srcs;
-    hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> done;
-    ~azk_circle_batch()
-    {
-        if (graph_a) (void)hipGraphExecDestroy(graph_a);
-        if (graph_b) (void)hipGraphExecDestroy(graph_b);
-        if (fork) (void)hipEventDestroy(fork);
-        for (auto e : done) (void)hipEventDestroy(e);
-    }
+    Runner runner;
 };
 
 extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
@@ -614,6 +695,7 @@ extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers,
 {
     CHECK_ARG(provers && coeffs && offsets && out && jobs);
     azk_context *ctx = provers[0]->ctx;
+    std::vector<Job> list;
     for (uint32_t j = 0; j < jobs; ++j) {
         CHECK_ARG(provers[j] && coeffs[j]);
         if (provers[j]->ctx != ctx) return fail(AZK_ERR_INVALID_ARGUMENT, "all provers must share one context");
@@ -622,32 +704,15 @@ extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers,
         azk_status s = check_range(coeffs[j], offsets[j], (size_t)16 << provers[j]->d.log_n);
         if (s) return s;
         if (((uintptr_t)coeffs[j]->ptr + offsets[j]) % 16) return fail(AZK_ERR_INVALID_LAYOUT, "job %u misaligned", j);
+        list.push_back(Job{provers[j], (const uint8_t *)coeffs[j]->ptr + offsets[j], 0});
     }
     auto *b = new azk_circle_batch();
     b->ctx = ctx;
     b->provers.assign(provers, provers + jobs);
-    for (uint32_t j = 0; j < jobs; ++j) b->srcs.push_back((const uint8_t *)coeffs[j]->ptr + offsets[j]);
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> gc(ctx->mu);
-    HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-    b->done.resize(jobs);
-    for (auto &e : b->done) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    hipStream_t st = ctx->stream;
-    auto fork_join = [&](bool phase_a) -> azk_status {
-        HIP_TRY(hipEventRecord(b->fork, st));
-        for (uint32_t j = 0; j < jobs; ++j) {
-            azk_circle_prover *p = b->provers[j];
-            HIP_TRY(hipStreamWaitEvent(p->own, b->fork, 0));
-            PhaseClock none(p->own, false);
-            azk_status s = phase_a ? enqueue_a(p, p->own, p->side, b->srcs[j], 0, none) : enqueue_b(p, p->own, b->srcs[j], 0);
-            if (s) return s;
-            HIP_TRY(hipEventRecord(b->done[j], p->own));
-            HIP_TRY(hipStreamWaitEvent(st, b->done[j], 0));
-        }
-        return AZK_OK;
-    };
-    azk_status s = capture(st, &b->graph_a, [&]() { return fork_join(true); });
-    if (!s) s = capture(st, &b->graph_b, [&]() { return fork_join(false); });
+    azk_status s = runner_build(b->runner, ctx, list);
+    if (!s) s = ensure_graphs(b->runner, ctx->stream);
     if (s) {
         delete b;
         return s;
@@ -668,46 +733,20 @@ extern "C" azk_status azk_circle_batch_prove(azk_circle_batch *b, uint8_t *const
                                              size_t *lens, azk_stats *stats)
 {
     CHECK_ARG(b && proofs && caps);
-    const uint32_t jobs = (uint32_t)b->provers.size();
-    for (uint32_t j = 0; j < jobs; ++j) {
+    const size_t jobs = b->provers.size();
+    for (size_t j = 0; j < jobs; ++j) {
         CHECK_ARG(proofs[j]);
-        if (caps[j] < 4 * b->provers[j]->proof_words) return fail(AZK_ERR_INVALID_LAYOUT, "proof capacity of job %u", j);
+        if (caps[j] < 4 * b->provers[j]->proof_words) return fail(AZK_ERR_INVALID_LAYOUT, "proof capacity of job %zu", j);
     }
     std::vector<std::unique_lock<std::mutex>> locks;
     for (auto *p : b->provers) locks.emplace_back(p->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));
     std::lock_guard<std::mutex> gc(b->ctx->mu);
-    hipStream_t st = b->ctx->stream;
-    Timer t;
-    azk_status s = t.init();
+    azk_status s = run(b->runner, true, nullptr, stats);
     if (s) return s;
-    const uint64_t w0 = now_ns();
-    HIP_TRY(hipEventRecord(t.a, st));
-    HIP_TRY(hipGraphLaunch(b->graph_a, st));
-    const uint64_t w1 = now_ns();
-    HIP_TRY(hipStreamSynchronize(st));
-    // host finals in parallel (each is a serial sponge over one job's final layer)
-    const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const uint32_t nt = std::min(jobs, hw);
-    std::vector<std::thread> pool;
-    for (uint32_t k = 1; k < nt; ++k)
-        pool.emplace_back([&, k]() {
-            for (uint32_t j = k; j < jobs; j += nt) host_final(b->provers[j]);
-        });
-    for (uint32_t j = 0; j < jobs; j += nt) host_final(b->provers[j]);
-    for (auto &th : pool) th.join();
-    HIP_TRY(hipGraphLaunch(b->graph_b, st));
-    HIP_TRY(hipEventRecord(t.b, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    for (uint32_t j = 0; j < jobs; ++j) {
-        azk_circle_prover *p = b->provers[j];
-        memcpy(proofs[j], p->host_proof.p, 4 * p->proof_words);
-        if (lens) lens[j] = 4 * p->proof_words;
-    }
-    if (stats) {
-        stats->submit_ns = w1 - w0;
-        stats->wall_ns = now_ns() - w0;
-        stats->gpu_ns = t.elapsed_ns();
+    for (size_t j = 0; j < jobs; ++j) {
+        memcpy(proofs[j], proof_of(b->runner, j), 4 * b->provers[j]->proof_words);
+        if (lens) lens[j] = 4 * b->provers[j]->proof_words;
     }
     return AZK_OK;
 }

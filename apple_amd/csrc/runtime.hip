@@ -58,17 +58,6 @@ std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t)
     return v;
 }
 
-void enqueue_fft(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
-                 uint32_t *tree)
-{
-    const void *cur = src;
-    for (const FftPass &p : fft_passes(log_n, log_t)) {
-        launch_fft_pass(st, cur, dst, tw, log_n, p.lo, p.k, p.m);
-        cur = dst;
-    }
-    launch_fft_final(st, cur, dst, tw, log_n, std::min(log_t, log_n), true, tree);
-}
-
 azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw)
 {
     const uint64_t n = 1ull << log_n;
@@ -86,20 +75,80 @@ azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t round
     return AZK_OK;
 }
 
-
-
-uint32_t enqueue_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
-                      uint32_t log_t, uint32_t *tree)
+std::vector<TreeStep> tree_schedule(uint64_t m, uint32_t level, uint32_t max_log_c)
 {
+    std::vector<TreeStep> v;
+    const uint32_t log_m = ilog2(m);
+    while (level < log_m) {
+        uint32_t log_c = std::min(max_log_c, log_m - level);
+        v.push_back({level, log_c, (m >> level) >> log_c});
+        level += log_c;
+    }
+    return v;
+}
+
+void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &final_, const void *src, void *dst,
+              const uint32_t *tw, uint32_t log_n, uint32_t log_t, bool run_fft, uint32_t *tree)
+{
+    const uint32_t lt = std::min(log_t, log_n);
+    const void *cur = src;
+    if (run_fft) {
+        auto fp = fft_passes(log_n, log_t);
+        if (passes.size() < fp.size()) passes.resize(fp.size());
+        for (size_t i = 0; i < fp.size(); ++i) {
+            FftPassDesc d{(const uint4 *)cur, (uint4 *)dst, tw, log_n, fp[i].lo, fp[i].k, fp[i].m, 0, 0};
+            passes[i].add(d, (uint32_t)(1ull << (log_n - log_t)), log_t);
+            cur = dst;
+        }
+    }
+    FftFinalDesc f{(const uint4 *)cur, (uint4 *)dst, tw, tree, log_n, lt, run_fft ? 1u : 0u, 0};
+    final_.add(f, (uint32_t)(1ull << (log_n - lt)), lt);
+}
+
+uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in, void *out, const uint32_t *tw,
+                   const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree)
+{
+    const uint32_t log_out = ilog2(n_out);
     if (tree && n_out < kLeafwiseBelow) {
-        uint32_t lt = std::min<uint32_t>(kLeafwiseLogTile, ilog2(n_out));
-        launch_fold_leafwise(st, in, out, tw, ch, n_out, lt, tree);
+        const uint32_t lt = std::min(kLeafwiseLogTile, log_out);
+        leaf.add(FoldDesc{(const uint4 *)in, (uint4 *)out, tw, ch, tree, n_out, lt, 0}, (uint32_t)(n_out >> lt), lt);
         return lt;
     }
-    uint32_t lt = std::min<uint32_t>(log_t, ilog2(n_out));
-    launch_fold(st, in, out, tw, ch, n_out, lt, tree);
+    const uint32_t lt = std::min(log_t, log_out);
+    big.add(FoldDesc{(const uint4 *)in, (uint4 *)out, tw, ch, tree, n_out, lt, 0}, (uint32_t)(n_out >> lt), lt);
     return lt;
 }
+
+void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level)
+{
+    auto sch = tree_schedule(m, level, kUpperLogChunk);
+    if (steps.size() < sch.size()) steps.resize(sch.size());
+    for (size_t k = 0; k < sch.size(); ++k)
+        steps[k].add(TreeDesc{tree, m, sch[k].level, sch[k].log_c, 0, 0}, (uint32_t)sch[k].chunks, sch[k].log_c);
+}
+
+// A one-off program for the single-job plans: tables are built per call, uploaded,
+// and launched on the context stream.
+struct OneShot {
+    TableArena arena;
+    std::vector<Launch<FftPassDesc>> passes;
+    Launch<FftFinalDesc> final_;
+    std::vector<Launch<TreeDesc>> trees;
+    std::vector<Launch<FoldDesc>> folds_big, folds_leaf;
+    Launch<MerkleRawDesc> raw;
+    Launch<CopyDesc> copy;
+    void reset()  // keeps the device table buffer
+    {
+        arena.host.clear();
+        passes.clear();
+        final_ = {};
+        trees.clear();
+        folds_big.clear();
+        folds_leaf.clear();
+        raw = {};
+        copy = {};
+    }
+};
 
 } // namespace azk::rt
 
@@ -406,6 +455,7 @@ struct azk_merkle_plan {
     uint32_t log_m, log_t;
     DeviceMem tree, staging, path;
     bool committed = false;
+    OneShot shot;
     std::mutex mu;
 };
 
@@ -418,7 +468,7 @@ extern "C" azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_
         return fail(AZK_ERR_UNSUPPORTED_ONE_BLOCK_LENGTH, "leaves are hashed as one block (<= 136 bytes), got %u",
                     desc->leaf_length);
     if (desc->leaf_stride < desc->leaf_length) return fail(AZK_ERR_INVALID_LAYOUT, "leaf stride shorter than leaf length");
-    auto *p = new azk_merkle_plan{ctx, *desc, ilog2(desc->leaf_count), 0, {}, {}, {}, false, {}};
+    auto *p = new azk_merkle_plan{ctx, *desc, ilog2(desc->leaf_count), 0, {}, {}, {}, false, {}, {}};
     uint32_t lt = desc->subtree_log ? desc->subtree_log : kDefaultLogTile;
     if (lt > kDefaultLogTile) lt = kDefaultLogTile;  // 24 x 2^11 bytes of LDS per block
     p->log_t = std::min(lt, p->log_m);
@@ -434,9 +484,20 @@ extern "C" void azk_merkle_plan_destroy(azk_merkle_plan *p) { delete p; }
 
 static azk_status merkle_enqueue(azk_merkle_plan *p, const void *leaves)
 {
-    launch_merkle_raw(p->ctx->stream, leaves, p->d.leaf_stride, p->d.leaf_length, p->d.leaf_count, p->log_t,
-                      (uint32_t *)p->tree.p);
-    launch_tree_upper(p->ctx->stream, (uint32_t *)p->tree.p, p->d.leaf_count, p->log_t, kUpperLogChunk);
+    OneShot &o = p->shot;
+    o.reset();
+    const uint64_t m = p->d.leaf_count;
+    o.raw.add(MerkleRawDesc{(const uint8_t *)leaves, (uint32_t *)p->tree.p, m, p->d.leaf_stride, p->d.leaf_length, p->log_t, 0},
+              (uint32_t)(m >> p->log_t), p->log_t);
+    plan_tree(o.trees, (uint32_t *)p->tree.p, m, p->log_t);
+    o.raw.off = o.arena.put(o.raw.v);
+    for (auto &t : o.trees) t.off = o.arena.put(t.v);
+    azk_status s = o.arena.upload();
+    if (s) return s;
+    const bool aligned = ((p->d.leaf_stride & 3u) == 0) && (((uintptr_t)leaves & 3u) == 0);
+    hipStream_t st = p->ctx->stream;
+    launch_merkle_raw(st, o.arena.at<MerkleRawDesc>(o.raw.off), o.raw.n(), o.raw.blocks, aligned, o.raw.max_log);
+    for (auto &t : o.trees) launch_tree(st, o.arena.at<TreeDesc>(t.off), t.n(), t.blocks, t.max_log);
     return AZK_OK;
 }
 
@@ -531,6 +592,7 @@ struct azk_prehashed_plan {
     azk_context *ctx;
     uint32_t count, log_m;
     DeviceMem tree;
+    OneShot shot;
     std::mutex mu;
 };
 
@@ -539,7 +601,7 @@ extern "C" azk_status azk_prehashed_plan_create(azk_context *ctx, uint32_t leaf_
     CHECK_ARG(ctx && out);
     if (!is_pow2(leaf_count))
         return fail(AZK_ERR_INVALID_LEAF_COUNT, "Leaf count must be a non-zero power of two, got %u", leaf_count);
-    auto *p = new azk_prehashed_plan{ctx, leaf_count, ilog2(leaf_count), {}, {}};
+    auto *p = new azk_prehashed_plan{ctx, leaf_count, ilog2(leaf_count), {}, {}, {}};
     HIP_TRY(hipSetDevice(ctx->device));
     azk_status s = p->tree.ensure(((size_t)2 * leaf_count - 1) * 32);
     if (s) { delete p; return s; }
@@ -558,10 +620,20 @@ extern "C" azk_status azk_prehashed_commit_resident(azk_prehashed_plan *p, const
     if (((uintptr_t)hashes->ptr + off) % 4) return fail(AZK_ERR_INVALID_LAYOUT, "leaf hashes must be 4-byte aligned");
     HIP_TRY(hipSetDevice(p->ctx->device));
     std::lock_guard<std::mutex> gc(p->ctx->mu);
+    OneShot &o = p->shot;
+    o.reset();
+    const uint64_t words = (uint64_t)p->count * 8;
+    o.copy.add(CopyDesc{(const uint32_t *)((uint8_t *)hashes->ptr + off), (uint32_t *)p->tree.p, words, 0, 0},
+               (uint32_t)((words + 1023) / 1024));
+    plan_tree(o.trees, (uint32_t *)p->tree.p, p->count, 0);
+    o.copy.off = o.arena.put(o.copy.v);
+    for (auto &t : o.trees) t.off = o.arena.put(t.v);
+    s = o.arena.upload();
+    if (s) return s;
     s = run_timed(p->ctx, stats, [&]() {
-        launch_copy_words(p->ctx->stream, (const uint32_t *)((uint8_t *)hashes->ptr + off), (uint32_t *)p->tree.p,
-                          (uint64_t)p->count * 8);
-        launch_tree_upper(p->ctx->stream, (uint32_t *)p->tree.p, p->count, 0, kUpperLogChunk);
+        hipStream_t st = p->ctx->stream;
+        launch_copy(st, o.arena.at<CopyDesc>(o.copy.off), o.copy.n(), o.copy.blocks);
+        for (auto &t : o.trees) launch_tree(st, o.arena.at<TreeDesc>(t.off), t.n(), t.blocks, t.max_log);
         return AZK_OK;
     });
     if (s) return s;
@@ -575,14 +647,31 @@ struct azk_codeword_plan {
     azk_context *ctx;
     uint32_t log_n;
     DeviceMem tw, in, out;
+    OneShot shot;
     std::mutex mu;
 };
+
+// Circle FFT of one job through the table-driven pass/final kernels.
+static azk_status codeword_enqueue(azk_codeword_plan *p, const void *src, void *dst)
+{
+    OneShot &o = p->shot;
+    o.reset();
+    plan_fft(o.passes, o.final_, src, dst, (const uint32_t *)p->tw.p, p->log_n, kDefaultLogTile, true, nullptr);
+    for (auto &l : o.passes) l.off = o.arena.put(l.v);
+    o.final_.off = o.arena.put(o.final_.v);
+    azk_status s = o.arena.upload();
+    if (s) return s;
+    hipStream_t st = p->ctx->stream;
+    for (auto &l : o.passes) launch_fft_pass(st, o.arena.at<FftPassDesc>(l.off), l.n(), l.blocks, l.max_log);
+    launch_fft_final(st, o.arena.at<FftFinalDesc>(o.final_.off), o.final_.n(), o.final_.blocks, o.final_.max_log);
+    return AZK_OK;
+}
 
 extern "C" azk_status azk_codeword_plan_create(azk_context *ctx, uint32_t log_n, azk_codeword_plan **out)
 {
     CHECK_ARG(ctx && out);
     if (log_n < 1 || log_n > 28) return fail(AZK_ERR_INVALID_LAYOUT, "domain log size %u outside 1..28", log_n);
-    auto *p = new azk_codeword_plan{ctx, log_n, {}, {}, {}, {}};
+    auto *p = new azk_codeword_plan{ctx, log_n, {}, {}, {}, {}, {}};
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> gc(ctx->mu);
     azk_status s = materialize_twiddles(ctx, log_n, 1, p->tw, nullptr);
@@ -607,9 +696,7 @@ extern "C" azk_status azk_codeword_execute_resident(azk_codeword_plan *p, const 
     HIP_TRY(hipSetDevice(p->ctx->device));
     std::lock_guard<std::mutex> gc(p->ctx->mu);
     return run_timed(p->ctx, stats, [&]() {
-        enqueue_fft(p->ctx->stream, (uint8_t *)coeffs->ptr + coff, (uint8_t *)out->ptr + ooff, (const uint32_t *)p->tw.p,
-                    p->log_n, kDefaultLogTile, nullptr);
-        return AZK_OK;
+        return codeword_enqueue(p, (uint8_t *)coeffs->ptr + coff, (uint8_t *)out->ptr + ooff);
     });
 }
 
@@ -627,7 +714,8 @@ extern "C" azk_status azk_codeword_execute(azk_codeword_plan *p, const void *coe
     std::lock_guard<std::mutex> gc(p->ctx->mu);
     return run_timed(p->ctx, stats, [&]() -> azk_status {
         HIP_TRY(hipMemcpyAsync(p->in.p, coeffs, bytes, hipMemcpyHostToDevice, p->ctx->stream));
-        enqueue_fft(p->ctx->stream, p->in.p, p->out.p, (const uint32_t *)p->tw.p, p->log_n, kDefaultLogTile, nullptr);
+        azk_status e = codeword_enqueue(p, p->in.p, p->out.p);
+        if (e) return e;
         HIP_TRY(hipMemcpyAsync(evals, p->out.p, bytes, hipMemcpyDeviceToHost, p->ctx->stream));
         return AZK_OK;
     });
@@ -649,6 +737,7 @@ struct azk_fold_plan {
     azk_context *ctx;
     uint32_t log_n, rounds;
     DeviceMem fft_tw, fold_tw, a, b, ch;
+    OneShot shot;
     std::mutex mu;
 };
 
@@ -657,7 +746,7 @@ extern "C" azk_status azk_fold_plan_create(azk_context *ctx, uint32_t log_n, uin
     CHECK_ARG(ctx && out);
     if (log_n < 1 || log_n > 28 || rounds < 1 || rounds > log_n || rounds > (uint32_t)kMaxRounds)
         return fail(AZK_ERR_INVALID_LAYOUT, "fold chain log_n %u rounds %u", log_n, rounds);
-    auto *p = new azk_fold_plan{ctx, log_n, rounds, {}, {}, {}, {}, {}, {}};
+    auto *p = new azk_fold_plan{ctx, log_n, rounds, {}, {}, {}, {}, {}, {}, {}};
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> gc(ctx->mu);
     azk_status s = materialize_twiddles(ctx, log_n, rounds, p->fft_tw, &p->fold_tw);
@@ -672,18 +761,27 @@ extern "C" void azk_fold_plan_destroy(azk_fold_plan *p) { delete p; }
 
 static azk_status fold_enqueue(azk_fold_plan *p, const void *in, void *out)
 {
+    OneShot &o = p->shot;
+    o.reset();
+    o.folds_big.resize(p->rounds);
+    o.folds_leaf.resize(p->rounds);
     const uint64_t n = 1ull << p->log_n;
     const void *cur = in;
     uint64_t off = 0, m = n;
     for (uint32_t r = 0; r < p->rounds; ++r) {
         void *dst = (r + 1 == p->rounds) ? out : (r % 2 ? p->b.p : p->a.p);
-        uint64_t n_out = m / 2;
-        enqueue_fold(p->ctx->stream, cur, dst, (const uint32_t *)p->fold_tw.p + off, (const uint32_t *)p->ch.p + 4 * r, n_out,
-                     kDefaultLogTile, nullptr);
+        const uint64_t n_out = m / 2;
+        plan_fold(o.folds_big[r], o.folds_leaf[r], cur, dst, (const uint32_t *)p->fold_tw.p + off,
+                  (const uint32_t *)p->ch.p + 4 * r, n_out, kDefaultLogTile, nullptr);
         cur = dst;
         off += n_out;
         m = n_out;
     }
+    for (auto &l : o.folds_big) l.off = o.arena.put(l.v);
+    azk_status s = o.arena.upload();
+    if (s) return s;
+    for (auto &l : o.folds_big)
+        launch_fold(p->ctx->stream, o.arena.at<FoldDesc>(l.off), l.n(), l.blocks, l.max_log);
     return AZK_OK;
 }
 

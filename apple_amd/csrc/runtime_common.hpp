@@ -160,17 +160,65 @@ struct FftPass { uint32_t lo, k, m; };
 // log_t - 3 stages, each on tiles of 2^log_t elements with >= 8 contiguous columns.
 std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t);
 
-// Enqueues the Circle FFT of `src` (interleaved line-basis coefficients) into `dst`;
-// with `tree`, the final pass also hashes the codeword and builds its tile subtrees.
-void enqueue_fft(hipStream_t st, const void *src, void *dst, const uint32_t *tw, uint32_t log_n, uint32_t log_t,
-                 uint32_t *tree);
-
 // FFT twiddles (and, with `fold_tw`, the first `rounds` stages inverted = FRI fold twiddles).
 azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw);
 
-// One FRI fold round (layer of 2 n_out -> n_out), choosing the tile form by size;
-// returns the log tile size whose roots remain for the upper tree (0 if no tree).
-uint32_t enqueue_fold(hipStream_t st, const void *in, void *out, const uint32_t *tw, const uint32_t *ch, uint64_t n_out,
-                      uint32_t log_t, uint32_t *tree);
+// Descriptor tables of many launches packed into one device buffer.
+struct TableArena {
+    std::vector<uint8_t> host;
+    DeviceMem dev;
+    template <class D>
+    size_t put(const std::vector<D> &v)
+    {
+        size_t off = align_up(host.size(), 64);
+        host.resize(off + v.size() * sizeof(D));
+        if (!v.empty()) memcpy(host.data() + off, v.data(), v.size() * sizeof(D));
+        return off;
+    }
+    azk_status upload()
+    {
+        azk_status s = dev.ensure(std::max<size_t>(host.size(), 64));
+        if (s) return s;
+        if (!host.empty()) HIP_TRY(hipMemcpy(dev.p, host.data(), host.size(), hipMemcpyHostToDevice));
+        return AZK_OK;
+    }
+    template <class D>
+    const D *at(size_t off) const { return (const D *)((const uint8_t *)dev.p + off); }
+};
+
+// One table-driven launch: descriptors (first_block assigned in order) and blocks.
+template <class D>
+struct Launch {
+    std::vector<D> v;
+    uint32_t blocks = 0;
+    uint32_t max_log = 0;  // largest tile / chunk log of the entries (sets LDS)
+    size_t off = 0;
+    void add(D d, uint32_t nblocks, uint32_t log_size = 0)
+    {
+        if (!nblocks) return;
+        d.first_block = blocks;
+        v.push_back(d);
+        blocks += nblocks;
+        max_log = std::max(max_log, log_size);
+    }
+    bool empty() const { return v.empty(); }
+    uint32_t n() const { return (uint32_t)v.size(); }
+};
+
+// Upper-tree schedule of one tree: launches of <= 2^max_log_c-node chunks from `level`
+// to the root; step k of several trees can share one launch.
+struct TreeStep { uint32_t level, log_c; uint64_t chunks; };
+std::vector<TreeStep> tree_schedule(uint64_t m, uint32_t level, uint32_t max_log_c);
+
+// Appends the FFT of one job (src -> dst, optional leaf-hash tree) to pass/final launches.
+void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &final_, const void *src, void *dst,
+              const uint32_t *tw, uint32_t log_n, uint32_t log_t, bool run_fft, uint32_t *tree);
+
+// Appends one fold round of one job to the big-tile or latency-form launch; returns the
+// log tile size (the level whose nodes remain for the upper tree).
+uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in, void *out, const uint32_t *tw,
+                   const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree);
+
+void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level);
 
 } // namespace azk::rt
