@@ -47,6 +47,19 @@ def commit_perms(log_n: int, rounds: int) -> int:
     return sum(2 * (n >> r) - 1 for r in range(rounds))
 
 
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE passes
+    (profiles/pmc_bytes_per_dispatch_2^21.txt, counters scaled by a known-size copy)."""
+    path = os.path.join(ROOT, "profiles", "pmc_bytes_per_dispatch_2^21.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == kernel:
+            return int(parts[1]) + int(parts[2])
+    return None
+
+
 def pct(v, q):
     return float(np.percentile(np.asarray(v, dtype=np.float64), q))
 
@@ -66,6 +79,8 @@ def cpu_baseline(coeffs, sample_commits: int) -> dict:
 
 
 def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
+    """64 independent commits, log sizes uniform in 14..21 (seeded), one batch program vs
+    the same jobs proved one after another; every batched proof equals its plain proof."""
     rng = np.random.default_rng(seed)
     logs = [int(x) for x in rng.integers(14, 22, size=jobs)]
     provers, bufs = [], []
@@ -80,19 +95,33 @@ def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
         for pr, b in zip(provers, bufs):
             pr.prove_fft_coefficients_resident(b)
         t_plain.append(time.perf_counter() - t0)
-    proofs, _ = azk.prove_batch(provers, bufs)
-    assert proofs == plain, "batched proofs differ from plain single-job proofs"
+    plan = azk.CircleBatchPlan(provers, bufs)
+    if plan.prove() != plain:
+        raise SystemExit("FATAL: batched proofs differ from plain single-job proofs")
     t_batch = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        azk.prove_batch(provers, bufs)
+        plan.prove()
         t_batch.append(time.perf_counter() - t0)
+    plan.close()
     for pr in provers:
         pr.close()
-    plain_rate = jobs / min(t_plain)
-    batch_rate = jobs / min(t_batch)
-    return {"jobs": jobs, "log_sizes": logs, "plain_commits_per_s": plain_rate, "batched_commits_per_s": batch_rate,
-            "speedup": batch_rate / plain_rate, "bit_exact_vs_plain": True}
+    plain_rate = jobs / float(np.median(t_plain))
+    batch_rate = jobs / float(np.median(t_batch))
+    nbytes = sum(commit_bytes(ln, ROUNDS) for ln in logs)
+    return {"workload": "64 circle_pcs_commit jobs, log_n ~ U{14..21}", "jobs": jobs,
+            "plain_commits_per_s": plain_rate, "batched_commits_per_s": batch_rate,
+            "speedup": batch_rate / plain_rate, "batched_gbps": nbytes / float(np.median(t_batch)) / 1e9,
+            "batch_ms": 1e3 * float(np.median(t_batch)), "bit_exact_vs_plain": True}
+
+
+def calibrate_copy(ctx, log_count: int = 23) -> None:
+    """A plain copy of known size for scaling the PMC byte counters: k_copy moves
+    2^23 x 32 bytes (256 MiB) in and out (the prehashed commit's first kernel)."""
+    count = 1 << log_count
+    hashes = ctx.buffer(32 * count)
+    plan = azk.SHA3PrehashedLeavesMerkleCommitPlan(ctx, count)
+    plan.commit_resident(hashes)
 
 
 def main() -> int:
@@ -103,6 +132,7 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=1, help="compare the first proof with the oracle")
     ap.add_argument("--cpu-sample", type=int, default=2, help="commits timed on the CPU oracle (0 = skip)")
     ap.add_argument("--batched", type=int, default=1)
+    ap.add_argument("--calibrate", type=int, default=0, help="run the known-size copy kernel (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard whole per GPU; "
@@ -111,6 +141,8 @@ def main() -> int:
 
     ctx = azk.Context(0)
     caps = ctx.capabilities()
+    if args.calibrate:
+        calibrate_copy(ctx)
     coeffs = inputs.circle_fft_coefficients(seed=1, log_n=LOG_N)
     buf = ctx.upload(coeffs)
     prover = azk.CircleCodewordPCSFRIProverV1(ctx, LOG_N, ROUNDS, SECURITY, DIGEST)
@@ -162,7 +194,9 @@ def main() -> int:
     t0_s = phases["commit0_ms"] / 1e3
     achieved = perms0 * KECCAK_OPS / t0_s / 1e12
     roofline = {"bound": "valu", "kernel": "k_fft_final (fused leaf hash + treelet)", "achieved": achieved,
-                "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS,
+                "traffic": pmc_traffic("k_fft_final"),
+                "traffic_source": "profiles/pmc_bytes_per_dispatch_2^21.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
                 "algorithmic_ops_per_launch": perms0 * KECCAK_OPS, "avg_launch_ms": t0_s * 1e3}
 
     line = {
