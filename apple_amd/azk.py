@@ -64,6 +64,11 @@ class ProverDesc(C.Structure):
                 ("public_input_digest", C.c_uint8 * 32)]
 
 
+class SumcheckResult(C.Structure):
+    _fields_ = [("claimed_sum", C.c_uint32), ("final_evaluation", C.c_uint32), ("initial_digest", C.c_uint8 * 32),
+                ("round_polys", C.POINTER(C.c_uint32)), ("challenges", C.POINTER(C.c_uint32))]
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [("fft_ns", C.c_uint64), ("commit0_ns", C.c_uint64), ("fold_ns", C.c_uint64),
                 ("tree_ns", C.c_uint64), ("transcript_ns", C.c_uint64), ("emit_ns", C.c_uint64),
@@ -145,6 +150,12 @@ def load() -> C.CDLL:
         "azk_circle_batch_prove": (st, [vp, C.POINTER(vp), C.POINTER(sz), C.POINTER(sz), sp]),
         "azk_circle_prove_batch": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp),
                                         C.POINTER(sz), C.POINTER(sz), sp]),
+        "azk_sumcheck_plan_create": (st, [vp, u32, C.POINTER(vp)]),
+        "azk_sumcheck_plan_destroy": (None, [vp]),
+        "azk_sumcheck_proof_size": (sz, [u32]),
+        "azk_sumcheck_prove": (st, [vp, vp, sz, vp, vp, sz, C.POINTER(sz), sp]),
+        "azk_sumcheck_prove_resident": (st, [vp, vp, sz, vp, vp, C.POINTER(SumcheckResult), sp]),
+        "azk_sumcheck_verify": (st, [vp, sz, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -562,3 +573,57 @@ class CircleBatchPlan:
             self.close()
         except Exception:
             pass
+
+
+class M31MultilinearSumcheckPlan(_Plan):
+    """GPU form of M31MultilinearSumcheckProofBuilderV1 (SumcheckOracle.swift:817-885)."""
+    _destroy = "azk_sumcheck_plan_destroy"
+
+    def __init__(self, ctx: Context, log_n: int):
+        self.ctx, self.log_n = ctx, log_n
+        h = vp()
+        _ck(ctx._lib.azk_sumcheck_plan_create(ctx.h, log_n, C.byref(h)))
+        self.h = h
+        self.proof_size = int(ctx._lib.azk_sumcheck_proof_size(log_n))
+        self._out = np.zeros(self.proof_size, np.uint8)
+        self.last_stats: dict = {}
+
+    def prove(self, evaluations, claimed_sum: int | None = None) -> bytes:
+        """M31MultilinearSumcheckProofCodecV1 bytes of the proof."""
+        a = np.ascontiguousarray(evaluations, dtype=np.uint32)
+        cs = C.c_uint32(claimed_sum) if claimed_sum is not None else None
+        n, s = C.c_size_t(0), Stats()
+        _ck(self.ctx._lib.azk_sumcheck_prove(self.h, _ptr(a), a.size, C.byref(cs) if cs is not None else None,
+                                             _ptr(self._out), self.proof_size, C.byref(n), C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return self._out[: n.value].tobytes()
+
+    def prove_resident(self, evals: Buffer, offset: int = 0, initial_digest: bytes | None = None,
+                       claimed_sum: int | None = None) -> dict:
+        """Round polynomials, challenges, final evaluation and statement fields."""
+        r = SumcheckResult()
+        polys = np.zeros(2 * self.log_n, np.uint32)
+        ch = np.zeros(self.log_n, np.uint32)
+        r.round_polys = polys.ctypes.data_as(C.POINTER(u32))
+        r.challenges = ch.ctypes.data_as(C.POINTER(u32))
+        dg = None
+        if initial_digest is not None:
+            if len(initial_digest) != 32:
+                raise AZKError(6, "initial digest must be 32 bytes")
+            dg = (C.c_uint8 * 32)(*initial_digest)
+        cs = C.c_uint32(claimed_sum) if claimed_sum is not None else None
+        s = Stats()
+        _ck(self.ctx._lib.azk_sumcheck_prove_resident(self.h, evals.h, offset, dg, C.byref(cs) if cs is not None else None,
+                                                      C.byref(r), C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return {"claimed_sum": r.claimed_sum, "final_evaluation": r.final_evaluation,
+                "initial_digest": bytes(r.initial_digest), "round_polys": polys, "challenges": ch}
+
+
+def sumcheck_verify(proof: bytes) -> bool:
+    """M31MultilinearSumcheckProofCodecV1.decode + M31MultilinearSumcheckVerifierV1.verify."""
+    lib = load()
+    b = np.frombuffer(proof, np.uint8)
+    ok = C.c_int(0)
+    _ck(lib.azk_sumcheck_verify(b.ctypes.data, b.size, C.byref(ok)))
+    return bool(ok.value)

@@ -455,15 +455,6 @@ __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restric
 
 // ============================================================== transcript
 
-// Wave-uniform 64-bit load: the value is moved to SGPRs so the sponge runs on the
-// scalar unit (measured 8.8 us per permutation on one wave vs 12.3 us on VALU).
-__device__ __forceinline__ uint64_t ld_uniform(const uint64_t *p)
-{
-    uint64_t v = *p;
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return (uint64_t)lo | ((uint64_t)hi << 32);
-}
-
 // Absorbs `len` bytes at `src` (8-byte aligned, readable up to the next multiple
 // of 8) as one SHA3-padded message.  Reference: TranscriptState.absorb
 // (SHA3Oracle.swift:19-39), Metal transcript_absorb_keccak :1324-1366.

@@ -241,6 +241,39 @@ azk_status azk_circle_prove_batch(azk_circle_prover *const *provers, const azk_b
                                   const size_t *offsets, uint32_t jobs, uint8_t *const *proofs,
                                   const size_t *capacities, size_t *proof_lens, azk_stats *stats);
 
+/* ---- M31 multilinear sum-check (the serving job)
+ * M31MultilinearSumcheckProofBuilderV1.prove (SumcheckOracle.swift:817-885); the
+ * reference runs it on the CPU only.  Rounds, transcript and folds run on the device;
+ * the initial digest M31SumcheckEncodingV1.digestWords (:1288-1299), a serial SHA3 over
+ * 4n bytes, runs on a host thread. */
+typedef struct azk_sumcheck_plan azk_sumcheck_plan;
+typedef struct azk_sumcheck_result {
+    uint32_t claimed_sum;        /* statement.claimedHypercubeSum */
+    uint32_t final_evaluation;
+    uint8_t initial_digest[32];  /* statement.initialEvaluationDigest */
+    uint32_t *round_polys;       /* caller memory, 2 * rounds words (e0, e1) or NULL */
+    uint32_t *challenges;        /* caller memory, rounds words or NULL */
+} azk_sumcheck_result;
+/* Plan for 2^log_n lanes, 1 <= log_n <= 30 (M31MultilinearSumcheckStatementV1 :711-739). */
+azk_status azk_sumcheck_plan_create(azk_context *ctx, uint32_t log_n, azk_sumcheck_plan **out);
+void azk_sumcheck_plan_destroy(azk_sumcheck_plan *plan);
+/* Bytes of M31MultilinearSumcheckProofCodecV1.encode for 2^log_n lanes (:1176-1194). */
+size_t azk_sumcheck_proof_size(uint32_t log_n);
+/* prove(evaluations:claimedHypercubeSum:) on a host vector; writes the codec bytes.
+ * claimed_sum may be NULL; a wrong one gives AZK_ERR_CORRECTNESS, non-canonical words
+ * AZK_ERR_INVALID_LAYOUT. */
+azk_status azk_sumcheck_prove(azk_sumcheck_plan *plan, const uint32_t *evals, size_t count, const uint32_t *claimed_sum,
+                              uint8_t *proof, size_t capacity, size_t *proof_len, azk_stats *stats);
+/* Resident form: words already in device memory.  `initial_digest` is digestWords of
+ * the same words when the caller has it, else NULL (the words are read back and hashed). */
+azk_status azk_sumcheck_prove_resident(azk_sumcheck_plan *plan, const azk_buffer *evals, size_t offset,
+                                       const uint8_t *initial_digest, const uint32_t *claimed_sum,
+                                       azk_sumcheck_result *result, azk_stats *stats);
+/* M31MultilinearSumcheckProofCodecV1.decode + M31MultilinearSumcheckVerifierV1.verify
+ * (:941-1064,1196-1233) against the statement the proof carries; *accepted = 1 or 0.
+ * Malformed bytes give AZK_ERR_INVALID_LAYOUT (the reference's decode throws). */
+azk_status azk_sumcheck_verify(const uint8_t *proof, size_t len, int *accepted);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1228,6 +1228,40 @@ int ora_sumcheck_multilinear(const uint32_t *ev, size_t n, uint32_t *round_polys
     return 0;
 }
 
+/* M31MultilinearSumcheckProofCodecV1.encode, SumcheckOracle.swift:1176-1194, of the
+ * proof built above: magic "AZKMSV1\0", u32 proof version, u32 statement version,
+ * u64 lanes, u32 rounds, u32 claimed sum, 32-byte digest, u64-counted initial words,
+ * u64-counted (e0, e1) pairs, u64-counted challenges, u32 final evaluation. */
+size_t ora_sumcheck_multilinear_proof(const uint32_t *ev, size_t n, uint8_t *out, size_t cap)
+{
+    if (n < 2 || !is_pow2(n)) return 0;
+    uint32_t rounds = ilog2(n);
+    size_t need = 8 + 4 + 4 + 8 + 4 + 4 + 32 + (8 + 4 * n) + (8 + 8 * (size_t)rounds) + (8 + 4 * (size_t)rounds) + 4;
+    if (cap < need) return 0;
+    uint32_t *rp = (uint32_t *)malloc(8 * rounds), *ch = (uint32_t *)malloc(4 * rounds), fin, cs;
+    uint8_t dg[32];
+    if (ora_sumcheck_multilinear(ev, n, rp, ch, &fin, &cs, dg)) { free(rp); free(ch); return 0; }
+    static const uint8_t magic[8] = {0x41, 0x5a, 0x4b, 0x4d, 0x53, 0x56, 0x31, 0x00};
+    uint8_t *p = out;
+    memcpy(p, magic, 8); p += 8;
+    put_u32(p, 1); p += 4;
+    put_u32(p, 1); p += 4;
+    put_u64(p, n); p += 8;
+    put_u32(p, rounds); p += 4;
+    put_u32(p, cs); p += 4;
+    memcpy(p, dg, 32); p += 32;
+    put_u64(p, n); p += 8;
+    for (size_t k = 0; k < n; ++k, p += 4) put_u32(p, ev[k]);
+    put_u64(p, rounds); p += 8;
+    for (uint32_t r = 0; r < 2 * rounds; ++r, p += 4) put_u32(p, rp[r]);
+    put_u64(p, rounds); p += 8;
+    for (uint32_t r = 0; r < rounds; ++r, p += 4) put_u32(p, ch[r]);
+    put_u32(p, fin); p += 4;
+    free(rp);
+    free(ch);
+    return (size_t)(p - out);
+}
+
 /* ------------------------------------------------------------------------- */
 /* SplitMix64 inputs                         Planner/MetalProofPlanner.swift:758-789 */
 /* ------------------------------------------------------------------------- */

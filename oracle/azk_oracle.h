@@ -102,6 +102,8 @@ int ora_sumcheck_chunk(const uint32_t *evals, size_t n, uint32_t rounds, uint32_
 int ora_sumcheck_multilinear(const uint32_t *evals, size_t n, uint32_t *round_polys, uint32_t *challenges,
                              uint32_t *final_eval, uint32_t *claimed_sum, uint8_t digest_out[32]);
 void ora_sumcheck_words_digest(const uint32_t *words, size_t n, uint8_t out[32]);
+/* codec bytes of the multilinear proof; returns the length, 0 on a bad layout */
+size_t ora_sumcheck_multilinear_proof(const uint32_t *evals, size_t n, uint8_t *out, size_t cap);
 
 /* ---- seeded inputs ---- */
 void ora_splitmix_bytes(uint64_t seed, size_t n, uint8_t *out);

@@ -91,6 +91,7 @@ def _declare(l: C.CDLL) -> None:
         "ora_sumcheck_chunk": (C.c_int, [u32p, sz, C.c_uint32, u32p, u32p, u32p]),
         "ora_sumcheck_multilinear": (C.c_int, [u32p, sz, u32p, u32p, u32p, u32p, u8p]),
         "ora_sumcheck_words_digest": (None, [u32p, sz, u8p]),
+        "ora_sumcheck_multilinear_proof": (sz, [u32p, sz, u8p, sz]),
         "ora_splitmix_bytes": (None, [C.c_uint64, sz, u8p]),
         "ora_splitmix_m31": (None, [C.c_uint64, sz, u32p]),
     }
@@ -384,6 +385,18 @@ def sumcheck_multilinear(evals):
     _check(lib().ora_sumcheck_multilinear(_p32(e), n, _p32(rp), _p32(ch), _p32(fin), _p32(cs), _p8(dg)), "multilinear")
     return {"round_polys": rp.reshape(-1, 2).tolist(), "challenges": ch.tolist(), "final": int(fin[0]),
             "claimed_sum": int(cs[0]), "digest": dg.tobytes()}
+
+
+def sumcheck_multilinear_proof(evals) -> bytes:
+    e = np.ascontiguousarray(evals, dtype=np.uint32)
+    n = e.size
+    rounds = n.bit_length() - 1
+    cap = 80 + 4 * n + 12 * rounds + 64
+    out = np.zeros(cap, np.uint8)
+    k = lib().ora_sumcheck_multilinear_proof(_p32(e), n, _p8(out), cap)
+    if k == 0:
+        raise ValueError("oracle rejected the sum-check layout")
+    return out[:k].tobytes()
 
 
 # ---- seeded inputs -------------------------------------------------------
