@@ -69,6 +69,16 @@ class SumcheckResult(C.Structure):
                 ("round_polys", C.POINTER(C.c_uint32)), ("challenges", C.POINTER(C.c_uint32))]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("device", C.c_int), ("max_batch", C.c_uint32), ("sumcheck_lanes", C.c_uint32),
+                ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("kind", C.c_uint32), ("batch_jobs", C.c_uint32), ("result_bytes", C.c_size_t),
+                ("submit_ns", C.c_uint64), ("admit_ns", C.c_uint64), ("start_ns", C.c_uint64), ("done_ns", C.c_uint64)]
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [("fft_ns", C.c_uint64), ("commit0_ns", C.c_uint64), ("fold_ns", C.c_uint64),
                 ("tree_ns", C.c_uint64), ("transcript_ns", C.c_uint64), ("emit_ns", C.c_uint64),
@@ -156,6 +166,15 @@ def load() -> C.CDLL:
         "azk_sumcheck_prove": (st, [vp, vp, sz, vp, vp, sz, C.POINTER(sz), sp]),
         "azk_sumcheck_prove_resident": (st, [vp, vp, sz, vp, vp, C.POINTER(SumcheckResult), sp]),
         "azk_sumcheck_verify": (st, [vp, sz, C.POINTER(C.c_int)]),
+        "azk_server_create": (st, [C.POINTER(ServerDesc), C.POINTER(vp)]),
+        "azk_server_destroy": (None, [vp]),
+        "azk_server_submit_commit": (st, [vp, C.POINTER(ProverDesc), vp, sz, C.POINTER(C.c_uint64)]),
+        "azk_server_submit_sumcheck": (st, [vp, vp, sz, C.POINTER(C.c_uint64)]),
+        "azk_server_wait": (st, [vp, C.c_uint64, C.POINTER(JobInfo)]),
+        "azk_server_take": (st, [vp, C.c_uint64, vp, sz, C.POINTER(sz)]),
+        "azk_now_ns": (C.c_uint64, []),
+        "azk_host_alloc": (st, [sz, C.POINTER(vp)]),
+        "azk_host_free": (None, [vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -627,3 +646,104 @@ def sumcheck_verify(proof: bytes) -> bool:
     ok = C.c_int(0)
     _ck(lib.azk_sumcheck_verify(b.ctypes.data, b.size, C.byref(ok)))
     return bool(ok.value)
+
+
+def now_ns() -> int:
+    return int(load().azk_now_ns())
+
+
+class PinnedArray:
+    """Pinned host memory (azk_host_alloc) viewed as a numpy array; inputs placed here
+    upload asynchronously."""
+
+    def __init__(self, shape, dtype=np.uint32):
+        self.lib = load()
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        h = vp()
+        _ck(self.lib.azk_host_alloc(n, C.byref(h)))
+        self.h = h
+        buf = (C.c_uint8 * max(n, 1)).from_address(h.value)
+        self.array = np.frombuffer(buf, np.uint8, count=n).view(dt).reshape(shape)
+
+    def close(self) -> None:
+        if self.h:
+            self.array = None
+            self.lib.azk_host_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prover_desc(log_n: int, rounds: int, security=(2, 32, 1, 0), public_input_digest: bytes = b"") -> ProverDesc:
+    d = ProverDesc()
+    d.log_n, d.rounds = log_n, rounds
+    d.log_blowup, d.query_count, d.folding_step, d.grinding_bits = security
+    dg = public_input_digest or bytes(32)
+    if len(dg) != 32:
+        raise AZKError(6, "public input digest must be 32 bytes")
+    d.public_input_digest[:] = list(dg)
+    return d
+
+
+class Server:
+    """Serving form (azk_server_*): commits and sum-checks submitted at any time."""
+
+    def __init__(self, device: int = 0, max_batch: int = 64, sumcheck_lanes: int = 4, hbm_budget: int = 0):
+        self.lib = load()
+        d = ServerDesc(device, max_batch, sumcheck_lanes, 0, hbm_budget)
+        h = vp()
+        _ck(self.lib.azk_server_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self._keep: dict = {}
+
+    def submit_commit(self, desc: ProverDesc, coeffs: np.ndarray) -> int:
+        a = np.ascontiguousarray(coeffs, dtype=np.uint32)
+        t = C.c_uint64(0)
+        _ck(self.lib.azk_server_submit_commit(self.h, C.byref(desc), a.ctypes.data, a.nbytes, C.byref(t)))
+        self._keep[t.value] = a  # read in place until the job is done
+        return t.value
+
+    def submit_sumcheck(self, evals: np.ndarray) -> int:
+        a = np.ascontiguousarray(evals, dtype=np.uint32)
+        t = C.c_uint64(0)
+        _ck(self.lib.azk_server_submit_sumcheck(self.h, a.ctypes.data, a.size, C.byref(t)))
+        self._keep[t.value] = a
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        i = JobInfo()
+        rc = self.lib.azk_server_wait(self.h, ticket, C.byref(i))
+        info = {k: getattr(i, k) for k, _ in JobInfo._fields_}
+        info["error"] = self.lib.azk_last_error().decode() if rc else ""
+        return info
+
+    def take(self, ticket: int) -> bytes:
+        i = JobInfo()
+        rc = self.lib.azk_server_wait(self.h, ticket, C.byref(i))
+        if rc:
+            msg = self.lib.azk_last_error().decode()
+            self.lib.azk_server_take(self.h, ticket, None, 0, None)
+            self._keep.pop(ticket, None)
+            raise AZKError(rc, msg)
+        out = np.zeros(max(i.result_bytes, 1), np.uint8)
+        n = C.c_size_t(0)
+        _ck(self.lib.azk_server_take(self.h, ticket, out.ctypes.data, out.size, C.byref(n)))
+        self._keep.pop(ticket, None)
+        return out[: n.value].tobytes()
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.azk_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

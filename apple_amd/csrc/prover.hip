@@ -24,10 +24,11 @@
 #include <thread>
 
 #include "host_sha3.hpp"
-#include "runtime_common.hpp"
+#include "prover_internal.hpp"
 
 using namespace azk;
 using namespace azk::rt;
+using namespace azk::prv;
 
 namespace {
 
@@ -54,90 +55,11 @@ std::vector<uint8_t> descriptor(uint32_t log_n)
     return d;
 }
 
-struct PhaseClock {
-    std::vector<hipEvent_t> ev;
-    std::vector<int> tag;
-    hipStream_t st;
-    bool on;
-    PhaseClock(hipStream_t s, bool enabled) : st(s), on(enabled) {}
-    ~PhaseClock() { for (auto e : ev) (void)hipEventDestroy(e); }
-    void mark(int t)
-    {
-        if (!on) return;
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
-        (void)hipEventRecord(e, st);
-        ev.push_back(e);
-        tag.push_back(t);
-    }
-};
-enum { PH_START = 0, PH_FFT, PH_COMMIT0, PH_FOLD, PH_TREE, PH_TRANSCRIPT, PH_TAIL, PH_HOST };
-
-struct Runner;
-
 } // namespace
-
-struct azk_circle_prover {
-    azk_context *ctx = nullptr;
-    azk_circle_prover_desc d{};
-    uint32_t log_t = kDefaultLogTile;
-    uint64_t n = 0, nfin = 0;
-    DeviceMem fft_tw, fold_tw, arena, frames, ops, coeff_staging;
-    std::vector<uint4 *> layers;    // rounds + 1 layers (layer 0 = codeword, layer R = final)
-    std::vector<uint32_t *> trees;  // one full tree per committed layer
-    uint32_t *challenges = nullptr;
-    uint64_t *tstate = nullptr, *tstate_init = nullptr;
-    std::vector<std::pair<uint32_t, uint32_t>> prog_a, prog_b;  // (first op, count) per round
-    std::vector<uint8_t> final_frame, query_frame;
-    std::vector<uint32_t> proof_template;
-    uint64_t roots_off = 0, final_off = 0, queries_off = 0, query_words = 0, proof_words = 0;
-    std::vector<uint32_t> layer_off;
-    bool use_graph = true, timing = false;
-    std::unique_ptr<Runner> runner;  // cached single-job program
-    azk_phase_times phases{};
-    std::mutex mu;
-    ~azk_circle_prover();
-};
 
 struct azk_circle_batch;
 
-namespace {
-
-struct Job {
-    azk_circle_prover *p;
-    const void *src;
-    int mode;  // 0: src = line-basis coefficients, 1: src = codeword
-    const uint4 *layer0() const { return mode == 0 ? p->layers[0] : (const uint4 *)src; }
-};
-
-struct Runner {
-    azk_context *ctx = nullptr;
-    std::vector<Job> jobs;
-    TableArena tables;
-    std::vector<Launch<FftPassDesc>> fft_pass;
-    Launch<FftFinalDesc> fft_final;
-    std::vector<Launch<TreeDesc>> tree0;
-    uint32_t rounds = 0;
-    std::vector<Launch<TranscriptDesc>> tr_a, tr_b;
-    std::vector<Launch<FoldDesc>> fold_big, fold_leaf;
-    std::vector<std::vector<Launch<TreeDesc>>> tree_r;
-    Launch<CopyDesc> tail;
-    Launch<EmitParams> emit;
-    DeviceMem tail_dev, qidx_dev, proof_dev;
-    PinnedMem tail_host, qidx_host, proof_host;
-    std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job
-    hipStream_t side = nullptr;
-    std::vector<hipEvent_t> ev_a, ev_b;
-    hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
-    ~Runner()
-    {
-        if (graph_a) (void)hipGraphExecDestroy(graph_a);
-        if (graph_b) (void)hipGraphExecDestroy(graph_b);
-        for (auto e : ev_a) (void)hipEventDestroy(e);
-        for (auto e : ev_b) (void)hipEventDestroy(e);
-        if (side) (void)hipStreamDestroy(side);
-    }
-};
+namespace azk::prv {
 
 azk_status prover_build(azk_circle_prover *p)
 {
@@ -371,12 +293,13 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
     R.emit.off = R.tables.put(R.emit.v);
     s = R.tables.upload();
     if (s) return s;
-    HIP_TRY(hipStreamCreateWithFlags(&R.side, hipStreamNonBlocking));
-    R.ev_a.resize(R.rounds);
-    R.ev_b.resize(R.rounds);
-    for (uint32_t r = 0; r < R.rounds; ++r) {
-        HIP_TRY(hipEventCreateWithFlags(&R.ev_a[r], hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&R.ev_b[r], hipEventDisableTiming));
+    if (!R.side) HIP_TRY(hipStreamCreateWithFlags(&R.side, hipStreamNonBlocking));
+    while (R.ev_a.size() < R.rounds) {
+        hipEvent_t a, b;
+        HIP_TRY(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+        R.ev_a.push_back(a);
+        HIP_TRY(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+        R.ev_b.push_back(b);
     }
     return AZK_OK;
 }
@@ -556,15 +479,39 @@ azk_status prove_single(azk_circle_prover *p, const void *src, int mode, uint8_t
     return AZK_OK;
 }
 
-} // namespace
-
-azk_circle_prover::~azk_circle_prover() = default;
-
-extern "C" azk_status azk_circle_prover_create(azk_context *ctx, const azk_circle_prover_desc *desc,
-                                               azk_circle_prover **out)
+void runner_reset(Runner &R)
 {
-    CHECK_ARG(ctx && desc && out);
-    const azk_circle_prover_desc &d = *desc;
+    R.jobs.clear();
+    R.tables.host.clear();
+    R.fft_pass.clear();
+    R.fft_final = {};
+    R.tree0.clear();
+    R.rounds = 0;
+    R.tr_a.clear();
+    R.tr_b.clear();
+    R.fold_big.clear();
+    R.fold_leaf.clear();
+    R.tree_r.clear();
+    R.tail = {};
+    R.emit = {};
+    R.tail_off.clear();
+    R.qidx_off.clear();
+    R.proof_off.clear();
+}
+
+size_t prover_footprint(const azk_circle_prover_desc &d)
+{
+    const size_t n = (size_t)1 << d.log_n;
+    size_t layers = 0, trees = 0;
+    for (uint32_t r = 0; r <= d.rounds; ++r) layers += align_up((size_t)16 * (n >> r), 256);
+    for (uint32_t r = 0; r < d.rounds; ++r) trees += align_up((2 * (n >> r) - 1) * 32, 256);
+    const size_t twiddles = 4 * n + 4 * n;  // FFT stages + inverted fold stages
+    const size_t staging = 16 * n;          // device copy of host coefficients
+    return layers + trees + twiddles + staging + 64 * 1024;
+}
+
+azk_status validate_desc(const azk_circle_prover_desc &d)
+{
     // CirclePCSFRIProofV1.init / CircleFRISecurityParametersV1.init (CircleProofFormat.swift:129-158,355-393)
     if (d.log_n < 2 || d.log_n > 28 || d.rounds < 1 || d.rounds > d.log_n || d.rounds > (uint32_t)kMaxRounds)
         return fail(AZK_ERR_INVALID_LAYOUT, "log_n %u / rounds %u outside the supported shape", d.log_n, d.rounds);
@@ -574,6 +521,13 @@ extern "C" azk_status azk_circle_prover_create(azk_context *ctx, const azk_circl
     if (d.grinding_bits != 0)
         return fail(AZK_ERR_INVALID_KERNEL_CONFIG, "grinding is not implemented on the device path (grinding_bits = %u)",
                     d.grinding_bits);
+    return AZK_OK;
+}
+
+azk_status prover_create(azk_context *ctx, const azk_circle_prover_desc &d, azk_circle_prover **out)
+{
+    azk_status s = validate_desc(d);
+    if (s) return s;
     auto *p = new azk_circle_prover();
     p->ctx = ctx;
     p->d = d;
@@ -581,13 +535,24 @@ extern "C" azk_status azk_circle_prover_create(azk_context *ctx, const azk_circl
     p->nfin = p->n >> d.rounds;
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> gc(ctx->mu);
-    azk_status s = prover_build(p);
+    s = prover_build(p);
     if (s) {
         delete p;
         return s;
     }
     *out = p;
     return AZK_OK;
+}
+
+} // namespace azk::prv
+
+azk_circle_prover::~azk_circle_prover() = default;
+
+extern "C" azk_status azk_circle_prover_create(azk_context *ctx, const azk_circle_prover_desc *desc,
+                                               azk_circle_prover **out)
+{
+    CHECK_ARG(ctx && desc && out);
+    return prover_create(ctx, *desc, out);
 }
 
 extern "C" void azk_circle_prover_destroy(azk_circle_prover *p)

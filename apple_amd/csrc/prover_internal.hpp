@@ -1,0 +1,112 @@
+// Internal interface of the Circle PCS prover, shared by the plain and batched entry
+// points (prover.hip) and the serving loop (server.hip).  Not part of the C ABI.
+#pragma once
+#include <memory>
+
+#include "runtime_common.hpp"
+
+namespace azk::prv {
+
+using namespace azk::rt;
+
+struct PhaseClock {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> tag;
+    hipStream_t st;
+    bool on;
+    PhaseClock(hipStream_t s, bool enabled) : st(s), on(enabled) {}
+    ~PhaseClock() { for (auto e : ev) (void)hipEventDestroy(e); }
+    void mark(int t)
+    {
+        if (!on) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, st);
+        ev.push_back(e);
+        tag.push_back(t);
+    }
+};
+enum { PH_START = 0, PH_FFT, PH_COMMIT0, PH_FOLD, PH_TREE, PH_TRANSCRIPT, PH_TAIL, PH_HOST };
+
+struct Runner;
+
+} // namespace azk::prv
+
+struct azk_circle_prover {
+    azk_context *ctx = nullptr;
+    azk_circle_prover_desc d{};
+    uint32_t log_t = azk::rt::kDefaultLogTile;
+    uint64_t n = 0, nfin = 0;
+    azk::rt::DeviceMem fft_tw, fold_tw, arena, frames, ops, coeff_staging;
+    std::vector<uint4 *> layers;    // rounds + 1 layers (layer 0 = codeword, layer R = final)
+    std::vector<uint32_t *> trees;  // one full tree per committed layer
+    uint32_t *challenges = nullptr;
+    uint64_t *tstate = nullptr, *tstate_init = nullptr;
+    std::vector<std::pair<uint32_t, uint32_t>> prog_a, prog_b;  // (first op, count) per round
+    std::vector<uint8_t> final_frame, query_frame;
+    std::vector<uint32_t> proof_template;
+    uint64_t roots_off = 0, final_off = 0, queries_off = 0, query_words = 0, proof_words = 0;
+    std::vector<uint32_t> layer_off;
+    bool use_graph = true, timing = false;
+    std::unique_ptr<azk::prv::Runner> runner;  // cached single-job program
+    azk_phase_times phases{};
+    std::mutex mu;
+    ~azk_circle_prover();
+};
+
+namespace azk::prv {
+
+struct Job {
+    azk_circle_prover *p;
+    const void *src;
+    int mode;  // 0: src = line-basis coefficients, 1: src = codeword
+    const uint4 *layer0() const { return mode == 0 ? p->layers[0] : (const uint4 *)src; }
+};
+
+struct Runner {
+    azk_context *ctx = nullptr;
+    std::vector<Job> jobs;
+    TableArena tables;
+    std::vector<Launch<FftPassDesc>> fft_pass;
+    Launch<FftFinalDesc> fft_final;
+    std::vector<Launch<TreeDesc>> tree0;
+    uint32_t rounds = 0;
+    std::vector<Launch<TranscriptDesc>> tr_a, tr_b;
+    std::vector<Launch<FoldDesc>> fold_big, fold_leaf;
+    std::vector<std::vector<Launch<TreeDesc>>> tree_r;
+    Launch<CopyDesc> tail;
+    Launch<EmitParams> emit;
+    DeviceMem tail_dev, qidx_dev, proof_dev;
+    PinnedMem tail_host, qidx_host, proof_host;
+    std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job
+    hipStream_t side = nullptr;
+    std::vector<hipEvent_t> ev_a, ev_b;
+    hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
+    ~Runner()
+    {
+        if (graph_a) (void)hipGraphExecDestroy(graph_a);
+        if (graph_b) (void)hipGraphExecDestroy(graph_b);
+        for (auto e : ev_a) (void)hipEventDestroy(e);
+        for (auto e : ev_b) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
+    }
+};
+
+// Builds the launch program of a job list into R (which must be fresh or reset).
+azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &jobs);
+// Forgets the job list and launch tables of R; keeps its device/pinned staging, side
+// stream and events for the next build.  R must not hold captured graphs.
+void runner_reset(Runner &R);
+azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk);
+void host_finals(const Runner &R);
+azk_status enqueue_b(Runner &R, hipStream_t st);
+const uint8_t *proof_of(const Runner &R, size_t k);
+// Descriptor rules of CircleFRISecurityParametersV1 / CirclePCSFRIProofV1 plus the
+// device path's limits (folding step 1, no grinding).
+azk_status validate_desc(const azk_circle_prover_desc &d);
+// Validates the descriptor and builds a prover; the caller holds no locks.
+azk_status prover_create(azk_context *ctx, const azk_circle_prover_desc &d, azk_circle_prover **out);
+// Device bytes a prover of this shape holds (twiddles, layers, trees, staging).
+size_t prover_footprint(const azk_circle_prover_desc &d);
+
+} // namespace azk::prv

@@ -1,0 +1,509 @@
+// Serving loop: Circle PCS commits and M31 multilinear sum-checks arriving at any time,
+// admitted against an HBM budget, commits run by continuous batching.
+//
+// No reference counterpart: the reference runs one synchronous command buffer per call
+// (MetalContext.swift, CircleCodeword.swift:1007-1018).  Every job's result is the same
+// bytes its plain call returns (the batched program is byte-identical per job).
+//
+//  * commit lane (one thread, one stream): each iteration takes every admitted commit
+//    (up to max_batch) and runs them as one table-driven program (graph A eager ->
+//    host final-layer step -> graph B eager).  While the device runs A, the lane admits
+//    the next arrivals and starts their coefficient uploads on a separate copy stream,
+//    so the next iteration starts on resident inputs.  Provers (their layers, trees,
+//    twiddles and input staging) stay cached per shape and are reused.
+//  * sum-check lanes (threads, one stream each): a job's words are hashed on the host
+//    (the serial initial digest) while its device rounds run.  An idle lane releases
+//    its plan.
+//  * admission: a job is admitted when the bytes of live device state (cached provers,
+//    sum-check plans) plus its own need fit the budget; idle cached state is evicted
+//    first when it does not.  A job larger than the whole budget fails with
+//    AZK_ERR_ALLOCATION.
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <string>
+#include <thread>
+
+#include "prover_internal.hpp"
+
+using namespace azk;
+using namespace azk::rt;
+using namespace azk::prv;
+
+namespace {
+
+enum { KIND_COMMIT = 0, KIND_SUMCHECK = 1 };
+
+struct SJob {
+    uint64_t ticket = 0;
+    uint32_t kind = 0;
+    azk_circle_prover_desc desc{};
+    const void *input = nullptr;
+    size_t count = 0;  // sum-check words
+    size_t need = 0;   // device bytes of the state it needs
+    azk_status status = AZK_OK;
+    std::string err;
+    std::vector<uint8_t> result;
+    uint64_t t_submit = 0, t_admit = 0, t_start = 0, t_done = 0;
+    uint32_t batch_jobs = 0;
+    bool done = false;
+    azk_circle_prover *prover = nullptr;
+    hipEvent_t uploaded = nullptr;
+};
+
+struct ShapeKey {
+    azk_circle_prover_desc d;
+    bool operator<(const ShapeKey &o) const { return memcmp(&d, &o.d, sizeof d) < 0; }
+};
+
+size_t sumcheck_need(uint32_t log_n) { return ((size_t)4 << log_n) * 7 / 4 + 64 * 1024; }
+
+} // namespace
+
+struct azk_server {
+    azk_server_desc d{};
+    azk_context *ctx = nullptr;               // commit lane
+    std::vector<azk_context *> sc_ctx;        // one per sum-check lane
+    hipStream_t upload = nullptr;
+    std::mutex mu;
+    std::condition_variable cv_commit, cv_sc, cv_done;
+    std::deque<SJob *> pend_commit, pend_sc;
+    std::map<uint64_t, std::unique_ptr<SJob>> jobs;
+    uint64_t next_ticket = 1;
+    size_t budget = 0, held = 0;  // held: bytes of live provers and plans
+    size_t sc_waiting = 0;        // bytes a blocked sum-check job needs (eviction request)
+    bool stop = false;
+    // commit lane state (commit thread only, except `held` under mu)
+    std::multimap<ShapeKey, azk_circle_prover *> idle;
+    std::deque<SJob *> ready;  // admitted, upload enqueued
+    Runner runner;
+    std::thread commit_thr;
+    std::vector<std::thread> sc_thr;
+};
+
+namespace {
+
+void finish(azk_server *s, SJob *j, azk_status st, const char *msg)
+{
+    std::lock_guard<std::mutex> g(s->mu);
+    j->status = st;
+    if (st && msg) j->err = msg;
+    j->t_done = now_ns();
+    j->done = true;
+    s->cv_done.notify_all();
+}
+
+// Evicts idle provers until `want` more bytes fit (commit thread).
+void evict_idle(azk_server *s, size_t want)
+{
+    while (!s->idle.empty()) {
+        size_t held;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            held = s->held;
+        }
+        if (held + want <= s->budget) return;
+        auto it = s->idle.begin();
+        azk_circle_prover *p = it->second;
+        const size_t fp = prover_footprint(p->d);
+        s->idle.erase(it);
+        azk_circle_prover_destroy(p);
+        std::lock_guard<std::mutex> g(s->mu);
+        s->held -= fp;
+    }
+}
+
+// Admits pending commits (first fit in arrival order) and enqueues their uploads.
+void admit_commits(azk_server *s)
+{
+    for (;;) {
+        SJob *j = nullptr;
+        azk_circle_prover *p = nullptr;
+        bool create = false;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            for (auto it = s->pend_commit.begin(); it != s->pend_commit.end(); ++it) {
+                SJob *c = *it;
+                auto idle = s->idle.find(ShapeKey{c->desc});
+                if (idle != s->idle.end()) {
+                    p = idle->second;
+                    s->idle.erase(idle);
+                } else if (s->held + c->need <= s->budget) {
+                    s->held += c->need;  // reserved; the prover is built below
+                    create = true;
+                } else {
+                    continue;
+                }
+                j = c;
+                s->pend_commit.erase(it);
+                break;
+            }
+        }
+        if (!j) return;
+        if (create) {
+            azk_status st = prover_create(s->ctx, j->desc, &p);
+            if (st) {
+                std::string m = azk_last_error();
+                {
+                    std::lock_guard<std::mutex> g(s->mu);
+                    s->held -= j->need;
+                }
+                finish(s, j, st, m.c_str());
+                continue;
+            }
+        }
+        j->prover = p;
+        const size_t bytes = (size_t)16 << j->desc.log_n;
+        azk_status st = p->coeff_staging.ensure(bytes);
+        hipError_t e = hipSuccess;
+        if (!st) e = hipMemcpyAsync(p->coeff_staging.p, j->input, bytes, hipMemcpyHostToDevice, s->upload);
+        if (!st && e == hipSuccess && !j->uploaded) e = hipEventCreateWithFlags(&j->uploaded, hipEventDisableTiming);
+        if (!st && e == hipSuccess) e = hipEventRecord(j->uploaded, s->upload);
+        if (st || e != hipSuccess) {
+            s->idle.emplace(ShapeKey{j->desc}, p);
+            j->prover = nullptr;
+            finish(s, j, st ? st : AZK_ERR_COMMAND_FAILED, st ? azk_last_error() : hipGetErrorString(e));
+            continue;
+        }
+        j->t_admit = now_ns();
+        s->ready.push_back(j);
+    }
+}
+
+azk_status run_batch(azk_server *s, std::vector<SJob *> &batch)
+{
+    hipStream_t st = s->ctx->stream;
+    std::vector<Job> list;
+    for (SJob *j : batch) list.push_back(Job{j->prover, j->prover->coeff_staging.p, 0});
+    runner_reset(s->runner);
+    azk_status r = runner_build(s->runner, s->ctx, list);
+    if (r) return r;
+    for (SJob *j : batch) HIP_TRY(hipStreamWaitEvent(st, j->uploaded, 0));
+    const uint64_t t0 = now_ns();
+    for (SJob *j : batch) j->t_start = t0;
+    PhaseClock none(st, false);
+    r = enqueue_a(s->runner, st, none);
+    if (r) return r;
+    admit_commits(s);  // the next batch's uploads overlap this one
+    HIP_TRY(hipStreamSynchronize(st));
+    host_finals(s->runner);
+    r = enqueue_b(s->runner, st);
+    if (r) return r;
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t k = 0; k < batch.size(); ++k) {
+        SJob *j = batch[k];
+        const uint8_t *pr = proof_of(s->runner, k);
+        j->result.assign(pr, pr + 4 * j->prover->proof_words);
+    }
+    return AZK_OK;
+}
+
+void commit_lane(azk_server *s)
+{
+    (void)hipSetDevice(s->d.device);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            s->cv_commit.wait(g, [&] { return s->stop || !s->pend_commit.empty() || !s->ready.empty() || s->sc_waiting; });
+            if (s->stop && s->pend_commit.empty() && s->ready.empty()) break;
+        }
+        admit_commits(s);
+        size_t sc_want;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            sc_want = s->sc_waiting;
+        }
+        if (sc_want) {
+            evict_idle(s, sc_want);
+            std::lock_guard<std::mutex> g(s->mu);
+            s->sc_waiting = 0;  // served as far as idle provers allow; the lane asks again if still blocked
+        }
+        if (s->ready.empty()) {
+            // Nothing admissible: free idle provers for the head job, or fail it if it can
+            // never fit.
+            SJob *head = nullptr;
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                if (!s->pend_commit.empty()) head = s->pend_commit.front();
+            }
+            if (!head) continue;
+            evict_idle(s, head->need);
+            admit_commits(s);
+            if (s->ready.empty()) {
+                bool never;
+                {
+                    std::lock_guard<std::mutex> g(s->mu);
+                    never = head->need > s->budget;
+                    if (never) s->pend_commit.pop_front();
+                }
+                if (never) {
+                    finish(s, head, AZK_ERR_ALLOCATION, "job exceeds the server's HBM budget");
+                } else {
+                    std::unique_lock<std::mutex> g(s->mu);  // sum-check lanes hold the memory
+                    s->cv_commit.wait_for(g, std::chrono::milliseconds(1));
+                }
+                continue;
+            }
+        }
+        std::vector<SJob *> batch;
+        while (!s->ready.empty() && batch.size() < s->d.max_batch) {
+            batch.push_back(s->ready.front());
+            s->ready.pop_front();
+        }
+        azk_status r = run_batch(s, batch);
+        std::string m = r ? azk_last_error() : "";
+        for (SJob *j : batch) {
+            j->batch_jobs = (uint32_t)batch.size();
+            s->idle.emplace(ShapeKey{j->desc}, j->prover);
+            j->prover = nullptr;
+            finish(s, j, r, m.c_str());
+        }
+    }
+}
+
+void sumcheck_lane(azk_server *s, size_t lane)
+{
+    (void)hipSetDevice(s->d.device);
+    azk_sumcheck_plan *plan = nullptr;
+    uint32_t plan_log = 0;
+    for (;;) {
+        SJob *j = nullptr;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            for (;;) {
+                if (!s->pend_sc.empty()) {
+                    SJob *c = s->pend_sc.front();
+                    const uint32_t log_n = ilog2(c->count);
+                    const size_t freed = (plan && plan_log != log_n) ? sumcheck_need(plan_log) : 0;
+                    const size_t add = (plan && plan_log == log_n) ? 0 : c->need;
+                    if (s->held - freed + add <= s->budget || c->need > s->budget) {
+                        j = c;
+                        s->pend_sc.pop_front();
+                        if (s->sc_waiting == c->need) s->sc_waiting = 0;
+                        if (c->need <= s->budget) s->held = s->held - freed + add;
+                        break;
+                    }
+                    s->sc_waiting = c->need;  // ask the commit lane to evict idle provers
+                    s->cv_commit.notify_one();
+                    s->cv_sc.wait_for(g, std::chrono::milliseconds(1));
+                    continue;
+                }
+                if (s->stop) break;
+                if (plan) {  // idle: give the plan's memory back to the budget
+                    g.unlock();
+                    azk_sumcheck_plan_destroy(plan);
+                    plan = nullptr;
+                    g.lock();
+                    s->held -= sumcheck_need(plan_log);
+                    continue;
+                }
+                s->cv_sc.wait(g);
+            }
+        }
+        if (!j) break;
+        if (j->need > s->budget) {
+            finish(s, j, AZK_ERR_ALLOCATION, "job exceeds the server's HBM budget");
+            continue;
+        }
+        const uint32_t log_n = ilog2(j->count);
+        j->t_admit = now_ns();
+        if (plan && plan_log != log_n) {
+            azk_sumcheck_plan_destroy(plan);
+            plan = nullptr;
+        }
+        azk_status st = AZK_OK;
+        if (!plan) {
+            st = azk_sumcheck_plan_create(s->sc_ctx[lane], log_n, &plan);
+            if (st) {
+                plan = nullptr;
+                std::lock_guard<std::mutex> g(s->mu);
+                s->held -= j->need;
+            } else {
+                plan_log = log_n;
+            }
+        }
+        if (!st) {
+            j->t_start = now_ns();
+            j->result.resize(azk_sumcheck_proof_size(log_n));
+            size_t len = 0;
+            st = azk_sumcheck_prove(plan, (const uint32_t *)j->input, j->count, nullptr, j->result.data(),
+                                    j->result.size(), &len, nullptr);
+            j->result.resize(st ? 0 : len);
+        }
+        j->batch_jobs = 1;
+        std::string m = st ? azk_last_error() : "";
+        finish(s, j, st, m.c_str());
+    }
+    if (plan) {
+        azk_sumcheck_plan_destroy(plan);
+        std::lock_guard<std::mutex> g(s->mu);
+        s->held -= sumcheck_need(plan_log);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+uint64_t azk_now_ns(void) { return now_ns(); }
+
+azk_status azk_host_alloc(size_t bytes, void **out)
+{
+    CHECK_ARG(out);
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+        return fail(AZK_ERR_ALLOCATION, "hipHostMalloc(%zu) failed", bytes);
+    return AZK_OK;
+}
+
+void azk_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+azk_status azk_server_create(const azk_server_desc *desc, azk_server **out)
+{
+    CHECK_ARG(desc && out);
+    auto *s = new azk_server();
+    s->d = *desc;
+    if (!s->d.max_batch) s->d.max_batch = 64;
+    if (!s->d.sumcheck_lanes) s->d.sumcheck_lanes = 4;
+    if (s->d.sumcheck_lanes > 16) s->d.sumcheck_lanes = 16;
+    azk_status st = azk_context_create(s->d.device, &s->ctx);
+    if (st) { delete s; return st; }
+    s->budget = s->d.hbm_budget;
+    if (!s->budget) {
+        size_t free_b = 0, total = 0;
+        if (hipMemGetInfo(&free_b, &total) != hipSuccess) free_b = (size_t)8 << 30;
+        s->budget = free_b / 2;
+    }
+    for (uint32_t k = 0; k < s->d.sumcheck_lanes && !st; ++k) {
+        azk_context *c = nullptr;
+        st = azk_context_create(s->d.device, &c);
+        if (!st) s->sc_ctx.push_back(c);
+    }
+    if (!st && hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) != hipSuccess)
+        st = fail(AZK_ERR_ALLOCATION, "hipStreamCreate failed");
+    if (st) {
+        for (auto *c : s->sc_ctx) azk_context_destroy(c);
+        azk_context_destroy(s->ctx);
+        delete s;
+        return st;
+    }
+    s->commit_thr = std::thread(commit_lane, s);
+    for (size_t k = 0; k < s->sc_ctx.size(); ++k) s->sc_thr.emplace_back(sumcheck_lane, s, k);
+    *out = s;
+    return AZK_OK;
+}
+
+void azk_server_destroy(azk_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_commit.notify_all();
+    s->cv_sc.notify_all();
+    s->commit_thr.join();
+    for (auto &t : s->sc_thr) t.join();
+    (void)hipSetDevice(s->d.device);
+    (void)hipStreamSynchronize(s->upload);
+    for (auto &kv : s->idle) azk_circle_prover_destroy(kv.second);
+    for (auto &kv : s->jobs)
+        if (kv.second->uploaded) (void)hipEventDestroy(kv.second->uploaded);
+    (void)hipStreamDestroy(s->upload);
+    for (auto *c : s->sc_ctx) azk_context_destroy(c);
+    azk_context *ctx = s->ctx;
+    delete s;  // the Runner's staging and streams go first
+    azk_context_destroy(ctx);
+}
+
+azk_status azk_server_submit_commit(azk_server *s, const azk_circle_prover_desc *desc, const void *coeffs, size_t bytes,
+                                    uint64_t *ticket)
+{
+    CHECK_ARG(s && desc && coeffs && ticket);
+    azk_status st = validate_desc(*desc);
+    if (st) return st;
+    if (bytes < ((size_t)16 << desc->log_n))
+        return fail(AZK_ERR_INVALID_LAYOUT, "coefficient bytes %zu < %zu", bytes, (size_t)16 << desc->log_n);
+    auto j = std::make_unique<SJob>();
+    j->kind = KIND_COMMIT;
+    j->desc = *desc;
+    j->input = coeffs;
+    j->need = prover_footprint(*desc);
+    j->t_submit = now_ns();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
+        j->ticket = s->next_ticket++;
+        *ticket = j->ticket;
+        s->pend_commit.push_back(j.get());
+        s->jobs.emplace(j->ticket, std::move(j));
+    }
+    s->cv_commit.notify_one();
+    return AZK_OK;
+}
+
+azk_status azk_server_submit_sumcheck(azk_server *s, const uint32_t *evals, size_t count, uint64_t *ticket)
+{
+    CHECK_ARG(s && evals && ticket);
+    if (count < 2 || (count & (count - 1)) || count > ((size_t)1 << 30))
+        return fail(AZK_ERR_INVALID_LAYOUT, "sum-check lane count %zu is not a power of two in [2, 2^30]", count);
+    auto j = std::make_unique<SJob>();
+    j->kind = KIND_SUMCHECK;
+    j->input = evals;
+    j->count = count;
+    j->need = sumcheck_need(ilog2(count));
+    j->t_submit = now_ns();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
+        j->ticket = s->next_ticket++;
+        *ticket = j->ticket;
+        s->pend_sc.push_back(j.get());
+        s->jobs.emplace(j->ticket, std::move(j));
+    }
+    s->cv_sc.notify_one();
+    return AZK_OK;
+}
+
+azk_status azk_server_wait(azk_server *s, uint64_t ticket, azk_job_info *info)
+{
+    CHECK_ARG(s && info);
+    std::unique_lock<std::mutex> g(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return fail(AZK_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+    SJob *j = it->second.get();
+    s->cv_done.wait(g, [&] { return j->done; });
+    info->status = j->status;
+    info->kind = j->kind;
+    info->batch_jobs = j->batch_jobs;
+    info->result_bytes = j->result.size();
+    info->submit_ns = j->t_submit;
+    info->admit_ns = j->t_admit;
+    info->start_ns = j->t_start;
+    info->done_ns = j->t_done;
+    if (j->status) return fail(j->status, "job %llu: %s", (unsigned long long)ticket, j->err.c_str());
+    return AZK_OK;
+}
+
+azk_status azk_server_take(azk_server *s, uint64_t ticket, uint8_t *out, size_t cap, size_t *len)
+{
+    CHECK_ARG(s);
+    std::unique_lock<std::mutex> g(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return fail(AZK_ERR_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+    SJob *j = it->second.get();
+    s->cv_done.wait(g, [&] { return j->done; });
+    if (out && j->status == AZK_OK) {
+        if (cap < j->result.size()) return fail(AZK_ERR_INVALID_LAYOUT, "result capacity %zu < %zu", cap, j->result.size());
+        memcpy(out, j->result.data(), j->result.size());
+    }
+    if (len) *len = j->result.size();
+    if (j->uploaded) (void)hipEventDestroy(j->uploaded);
+    s->jobs.erase(it);
+    return AZK_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,109 @@
+"""Serving form: admission by HBM footprint, continuous batching, and determinism (each
+job's result alone equals its result in a mix of concurrent jobs)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from apple_amd import azk, inputs, serving
+
+
+def test_trace_is_seeded_and_poisson():
+    a = serving.make_trace(5, 400, rate=100.0)
+    assert a == serving.make_trace(5, 400, rate=100.0)
+    gaps = np.diff([0.0] + [t for t, _, _ in a])
+    assert 0.007 < gaps.mean() < 0.013          # mean inter-arrival 1 / rate
+    kinds = [k for _, k, _ in a]
+    assert 150 < kinds.count("commit") < 250     # half of each kind
+    assert all(12 <= lg <= 22 for _, k, lg in a if k == "commit")
+    assert all(16 <= lg <= 22 for _, k, lg in a if k == "sumcheck")
+    assert all(t == 0.0 for t, _, _ in serving.make_trace(5, 10, rate=0))
+
+
+def test_sequential_model():
+    trace = [(0.0, "commit", 12), (0.5, "commit", 12), (0.6, "sumcheck", 16)]
+    r = serving.sequential_latencies(trace, [1.0, 1.0, 0.5])
+    # starts 0, 1, 2; finishes 1, 2, 2.5
+    assert r["p50_ms"] == pytest.approx(1500.0)
+    assert r["elapsed_s"] == pytest.approx(2.5)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = azk.Context(0)
+    yield c
+    c.close()
+
+
+def small_trace():
+    return [(0.0, "commit", 10), (0.0, "sumcheck", 12), (0.0, "commit", 13), (0.0, "commit", 10),
+            (0.0, "sumcheck", 9), (0.0, "commit", 11), (0.0, "sumcheck", 14), (0.0, "commit", 12),
+            (0.0, "commit", 14), (0.0, "sumcheck", 12), (0.0, "commit", 9), (0.0, "commit", 13)]
+
+
+def alone(ctx, kind, lg, data):
+    if kind == "commit":
+        p = azk.CircleCodewordPCSFRIProverV1(ctx, lg, serving.commit_rounds(lg), serving.SECURITY, serving.DIGEST)
+        return p.prove_fft_coefficients(data.get(kind, lg))
+    return azk.M31MultilinearSumcheckPlan(ctx, lg).prove(data.get(kind, lg))
+
+
+@pytest.mark.gpu
+def test_server_mix_matches_plain_calls(ctx):
+    trace = small_trace()
+    data = serving.Inputs(trace, seed=3)
+    srv = azk.Server(0, max_batch=64, sumcheck_lanes=2)
+    print("server mix of", len(trace), "jobs", flush=True)
+    r = serving.run_server(srv, trace, data, collect=True)
+    srv.close()
+    assert r["failed"] == 0
+    want = {(k, lg): alone(ctx, k, lg, data) for _, k, lg in trace}
+    for kind, lg, info, proof in r["results"]:
+        assert proof == want[(kind, lg)], (kind, lg)
+        assert info["kind"] == (0 if kind == "commit" else 1)
+    # one commit and one sum-check also against the oracle
+    c = data.get("commit", 10)
+    cw = O.circle_fft(c, 10)
+    ref, _ = O.fri_prove(cw.reshape(-1, 4), 10, serving.SECURITY, serving.DIGEST, serving.commit_rounds(10))
+    assert want[("commit", 10)] == ref
+    assert want[("sumcheck", 12)] == O.sumcheck_multilinear_proof(data.get("sumcheck", 12))
+    assert r["mean_commit_batch"] > 1  # jobs submitted together share device programs
+
+
+@pytest.mark.gpu
+def test_server_admission_limits_concurrency(ctx):
+    trace = [(0.0, "commit", 14)] * 4 + [(0.0, "sumcheck", 14)] * 2
+    data = serving.Inputs(trace, seed=4)
+    # room for one 2^14 commit's state (2.9 MiB) and the sum-check lanes, not two commits
+    budget = 4 << 20
+    srv = azk.Server(0, max_batch=64, sumcheck_lanes=2, hbm_budget=budget)
+    print("server tight budget", flush=True)
+    r = serving.run_server(srv, trace, data, collect=True)
+    srv.close()
+    assert r["failed"] == 0
+    want_c = alone(ctx, "commit", 14, data)
+    want_s = alone(ctx, "sumcheck", 14, data)
+    for kind, _, info, proof in r["results"]:
+        assert proof == (want_c if kind == "commit" else want_s)
+        if kind == "commit":
+            assert info["batch_jobs"] == 1
+
+
+@pytest.mark.gpu
+def test_server_rejects_and_recovers(ctx):
+    srv = azk.Server(0, hbm_budget=4 << 20)
+    big = inputs.circle_fft_coefficients(1, 16)
+    t = srv.submit_commit(azk.prover_desc(16, 10, serving.SECURITY, serving.DIGEST), big)
+    info = srv.wait(t)
+    assert info["status"] == 2  # AZK_ERR_ALLOCATION: larger than the whole budget
+    with pytest.raises(azk.AZKError):
+        srv.take(t)
+    with pytest.raises(azk.AZKError):
+        srv.submit_sumcheck(np.zeros(12, np.uint32))  # not a power of two
+    with pytest.raises(azk.AZKError):
+        srv.submit_commit(azk.prover_desc(10, 11, serving.SECURITY, serving.DIGEST), big)  # rounds > log_n
+    small = inputs.circle_fft_coefficients(2, 8)
+    t2 = srv.submit_commit(azk.prover_desc(8, 3, serving.SECURITY, serving.DIGEST), small)
+    got = srv.take(t2)
+    srv.close()
+    p = azk.CircleCodewordPCSFRIProverV1(ctx, 8, 3, serving.SECURITY, serving.DIGEST)
+    assert got == p.prove_fft_coefficients(small)
