@@ -12,8 +12,8 @@
 //    so the next iteration starts on resident inputs.  Provers (their layers, trees,
 //    twiddles and input staging) stay cached per shape and are reused.
 //  * sum-check lanes (threads, one stream each): a job's words are hashed on the host
-//    (the serial initial digest) while its device rounds run.  An idle lane releases
-//    its plan.
+//    (the serial initial digest) while its device rounds run.  An idle lane keeps its
+//    plan for the next job of the same size unless another job waits for memory.
 //  * admission: a job is admitted when the bytes of live device state (cached provers,
 //    sum-check plans) plus its own need fit the budget; idle cached state is evicted
 //    first when it does not.  A job larger than the whole budget fails with
@@ -72,6 +72,7 @@ struct azk_server {
     uint64_t next_ticket = 1;
     size_t budget = 0, held = 0;  // held: bytes of live provers and plans
     size_t sc_waiting = 0;        // bytes a blocked sum-check job needs (eviction request)
+    bool commit_blocked = false;  // the head commit waits for memory (idle plans are released)
     bool stop = false;
     // commit lane state (commit thread only, except `held` under mu)
     std::multimap<ShapeKey, azk_circle_prover *> idle;
@@ -240,10 +241,16 @@ void commit_lane(azk_server *s)
                     finish(s, head, AZK_ERR_ALLOCATION, "job exceeds the server's HBM budget");
                 } else {
                     std::unique_lock<std::mutex> g(s->mu);  // sum-check lanes hold the memory
+                    s->commit_blocked = true;
+                    s->cv_sc.notify_all();
                     s->cv_commit.wait_for(g, std::chrono::milliseconds(1));
                 }
                 continue;
             }
+        }
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->commit_blocked = false;
         }
         std::vector<SJob *> batch;
         while (!s->ready.empty() && batch.size() < s->d.max_batch) {
@@ -289,7 +296,8 @@ void sumcheck_lane(azk_server *s, size_t lane)
                     continue;
                 }
                 if (s->stop) break;
-                if (plan) {  // idle: give the plan's memory back to the budget
+                if (plan && (s->commit_blocked || s->sc_waiting)) {
+                    // idle while another job waits for memory: give the plan's bytes back
                     g.unlock();
                     azk_sumcheck_plan_destroy(plan);
                     plan = nullptr;
@@ -297,7 +305,8 @@ void sumcheck_lane(azk_server *s, size_t lane)
                     s->held -= sumcheck_need(plan_log);
                     continue;
                 }
-                s->cv_sc.wait(g);
+                if (plan) s->cv_sc.wait_for(g, std::chrono::milliseconds(2));
+                else s->cv_sc.wait(g);
             }
         }
         if (!j) break;

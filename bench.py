@@ -115,6 +115,42 @@ def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
             "batch_ms": 1e3 * float(np.median(t_batch)), "bit_exact_vs_plain": True}
 
 
+def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 96) -> dict:
+    """Poisson stream of commits (log N 12..22) and multilinear sum-checks (2^16..2^22),
+    half each, through the C ABI server; rate = 0.8 x the server's saturated throughput.
+    Compared with the same stream run as plain sequential calls (one job at a time)."""
+    from apple_amd import serving
+    shapes = [(0.0, "commit", lg) for lg in range(12, 23)] + [(0.0, "sumcheck", lg) for lg in range(16, 23)]
+    data = serving.Inputs(shapes, seed=seed)
+    srv = azk.Server(0, max_batch=64, sumcheck_lanes=8)
+    serving.run_server(srv, shapes + shapes, data)  # warm: provers and plans of every shape
+    sat = serving.run_server(srv, serving.make_trace(seed, sat_jobs, rate=0), data)
+    rate = 0.8 * sat["jobs_per_s"]
+    trace = serving.make_trace(seed + 1, jobs, rate=rate)
+    run = serving.run_server(srv, trace, data, collect=True)
+    # determinism: every result equals the same shape's result from the saturated mix
+    ref = {}
+    for kind, lg, info, proof in serving.run_server(srv, shapes, data, collect=True)["results"]:
+        ref[(kind, lg)] = hashlib.sha3_256(proof).digest()
+    same = all(info["status"] == 0 and hashlib.sha3_256(proof).digest() == ref[(kind, lg)]
+               for kind, lg, info, proof in run["results"])
+    srv.close()
+    service = serving.service_times(ctx, trace, data)
+    seq = serving.sequential_latencies(trace, service)
+    seq_sat = len(trace) / sum(service)
+    return {
+        "workload": "poisson mix: commits log N U{12..22} (10 rounds, 32 queries) + multilinear sum-checks 2^U{16..22}, "
+                    "50/50, seed %d" % seed,
+        "jobs": jobs, "rate_jobs_per_s": rate, "saturated_jobs_per_s": sat["jobs_per_s"],
+        "jobs_per_s": run["jobs_per_s"], "p50_ms": run["p50_ms"], "p99_ms": run["p99_ms"],
+        "mean_commit_batch": run["mean_commit_batch"], "failed": run["failed"],
+        "results_match_alone": bool(same),
+        "plain_sequential": {"saturated_jobs_per_s": seq_sat, "jobs_per_s": seq["jobs_per_s"],
+                             "p50_ms": seq["p50_ms"], "p99_ms": seq["p99_ms"]},
+        "speedup_saturated": sat["jobs_per_s"] / seq_sat,
+    }
+
+
 def calibrate_copy(ctx, log_count: int = 23) -> None:
     """A plain copy of known size for scaling the PMC byte counters: k_copy moves
     2^23 x 32 bytes (256 MiB) in and out (the prehashed commit's first kernel)."""
@@ -132,6 +168,7 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=1, help="compare the first proof with the oracle")
     ap.add_argument("--cpu-sample", type=int, default=2, help="commits timed on the CPU oracle (0 = skip)")
     ap.add_argument("--batched", type=int, default=1)
+    ap.add_argument("--serving", type=int, default=1)
     ap.add_argument("--calibrate", type=int, default=0, help="run the known-size copy kernel (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -214,6 +251,8 @@ def main() -> int:
     }
     if args.batched:
         line["batched"] = batched_form(ctx)
+    if args.serving:
+        line["serving"] = serving_form(ctx)
     if args.cpu_sample:
         line["cpu_baseline"] = cpu_baseline(coeffs, args.cpu_sample)
     print(json.dumps(line), flush=True)
