@@ -9,9 +9,11 @@ There is no CPU fallback: if libazk.so or a GPU is missing, constructors raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -67,6 +69,31 @@ class ProverDesc(C.Structure):
 class SumcheckResult(C.Structure):
     _fields_ = [("claimed_sum", C.c_uint32), ("final_evaluation", C.c_uint32), ("initial_digest", C.c_uint8 * 32),
                 ("round_polys", C.POINTER(C.c_uint32)), ("challenges", C.POINTER(C.c_uint32))]
+
+
+class ExecPlan(C.Structure):
+    _fields_ = [("log_tile_commit", C.c_uint32), ("log_tile_fold", C.c_uint32), ("log_leafwise_below", C.c_uint32),
+                ("log_tile_leafwise", C.c_uint32), ("log_tree_chunk", C.c_uint32)]
+
+
+class TuningConfig(C.Structure):
+    _fields_ = [("warmup_runs", C.c_uint32), ("measured_runs", C.c_uint32), ("max_candidates", C.c_uint32),
+                ("validation_runs", C.c_uint32), ("database_path", C.c_char_p), ("force", C.c_int)]
+
+
+class TuningResult(C.Structure):
+    _fields_ = [("winner", ExecPlan), ("median_gpu_ns", C.c_double), ("p95_gpu_ns", C.c_double),
+                ("median_submit_ns", C.c_double), ("default_median_gpu_ns", C.c_double),
+                ("candidates_measured", C.c_uint32), ("candidates_rejected", C.c_uint32), ("from_database", C.c_int)]
+
+
+class DriftStatus(C.Structure):
+    _fields_ = [("samples", C.c_uint32), ("stale", C.c_int), ("ema_gpu_ns", C.c_double),
+                ("baseline_gpu_ns", C.c_double), ("relative_drift", C.c_double)]
+
+
+def _plan_dict(x: ExecPlan) -> dict:
+    return {k: getattr(x, k) for k, _ in ExecPlan._fields_}
 
 
 class ServerDesc(C.Structure):
@@ -166,6 +193,10 @@ def load() -> C.CDLL:
         "azk_sumcheck_prove": (st, [vp, vp, sz, vp, vp, sz, C.POINTER(sz), sp]),
         "azk_sumcheck_prove_resident": (st, [vp, vp, sz, vp, vp, C.POINTER(SumcheckResult), sp]),
         "azk_sumcheck_verify": (st, [vp, sz, C.POINTER(C.c_int)]),
+        "azk_circle_prover_set_exec_plan": (st, [vp, C.POINTER(ExecPlan)]),
+        "azk_circle_prover_get_exec_plan": (st, [vp, C.POINTER(ExecPlan)]),
+        "azk_circle_prover_tune": (st, [vp, vp, sz, C.POINTER(TuningConfig), C.POINTER(TuningResult)]),
+        "azk_circle_prover_drift": (st, [vp, C.POINTER(DriftStatus)]),
         "azk_server_create": (st, [C.POINTER(ServerDesc), C.POINTER(vp)]),
         "azk_server_destroy": (None, [vp]),
         "azk_server_submit_commit": (st, [vp, C.POINTER(ProverDesc), vp, sz, C.POINTER(C.c_uint64)]),
@@ -182,6 +213,28 @@ def load() -> C.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+_live: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def _track(obj):
+    _live.add(obj)
+    return obj
+
+
+@atexit.register
+def _close_all() -> None:
+    """Frees what is still alive at interpreter exit while the HIP runtime is up:
+    servers and batches, then plans, buffers and pinned memory, then contexts (a
+    context is freed by the library after the last object built on it)."""
+    for cls in (Server, CircleBatchPlan, _Plan, Buffer, PinnedArray, Context):
+        for obj in list(_live):
+            if isinstance(obj, cls):
+                try:
+                    (obj.free if isinstance(obj, Buffer) else obj.close)()
+                except Exception:
+                    pass
 
 
 def _ck(rc: int) -> None:
@@ -212,6 +265,7 @@ class Context:
         h = vp()
         _ck(lib.azk_context_create(device, C.byref(h)))
         self.h = h
+        _track(self)
         self._lib = lib
 
     def capabilities(self) -> dict:
@@ -256,6 +310,7 @@ class Buffer:
         else:
             _ck(ctx._lib.azk_buffer_wrap(ctx.h, device_ptr, nbytes, C.byref(h)))
         self.h = h
+        _track(self)
         self.nbytes = nbytes
 
     @property
@@ -308,6 +363,7 @@ class KeccakF1600PermutationBatchPlan(_Plan):
         h = vp()
         _ck(ctx._lib.azk_keccakf_plan_create(ctx.h, C.byref(KeccakfDesc(count, input_stride, output_stride)), C.byref(h)))
         self.h = h
+        _track(self)
         self.last_stats: dict = {}
 
     def permute(self, states, verified: bool = False) -> np.ndarray:
@@ -339,6 +395,7 @@ class _OneBlockPlan(_Plan):
         d = OneBlockDesc(count, input_stride, input_length, output_stride)
         _ck(getattr(ctx._lib, self._create)(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
+        _track(self)
         self.last_stats: dict = {}
 
     def hash(self, messages, verified: bool = False) -> np.ndarray:
@@ -380,6 +437,7 @@ class SHA3RawLeavesMerkleCommitPlan(_Plan):
         _ck(ctx._lib.azk_merkle_plan_create(ctx.h, C.byref(MerkleDesc(leaf_count, leaf_stride, leaf_length, subtree_log)),
                                             C.byref(h)))
         self.h = h
+        _track(self)
         self.depth = leaf_count.bit_length() - 1
         self.last_stats: dict = {}
 
@@ -418,6 +476,7 @@ class SHA3PrehashedLeavesMerkleCommitPlan(_Plan):
         h = vp()
         _ck(ctx._lib.azk_prehashed_plan_create(ctx.h, leaf_count, C.byref(h)))
         self.h = h
+        _track(self)
 
     def commit_resident(self, leaf_hashes: Buffer, offset: int = 0) -> bytes:
         root = np.zeros(32, np.uint8)
@@ -435,6 +494,7 @@ class CircleCodewordPlan(_Plan):
         h = vp()
         _ck(ctx._lib.azk_codeword_plan_create(ctx.h, log_n, C.byref(h)))
         self.h = h
+        _track(self)
         self.last_stats: dict = {}
 
     def execute(self, coeffs: np.ndarray) -> np.ndarray:
@@ -466,6 +526,7 @@ class CircleFRIFoldChainPlan(_Plan):
         h = vp()
         _ck(ctx._lib.azk_fold_plan_create(ctx.h, log_n, rounds, C.byref(h)))
         self.h = h
+        _track(self)
 
     def execute(self, evals: np.ndarray, challenges) -> np.ndarray:
         a = _host(np.ascontiguousarray(evals, dtype=np.uint32))
@@ -498,6 +559,7 @@ class CircleCodewordPCSFRIProverV1(_Plan):
         h = vp()
         _ck(ctx._lib.azk_circle_prover_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
+        _track(self)
         self.proof_size = int(ctx._lib.azk_circle_prover_proof_size(h))
         self._out = np.zeros(self.proof_size, np.uint8)
         self.last_stats: dict = {}
@@ -540,6 +602,35 @@ class CircleCodewordPCSFRIProverV1(_Plan):
     def clear(self) -> None:
         _ck(self.ctx._lib.azk_circle_prover_clear(self.h))
 
+    def set_exec_plan(self, **plan) -> None:
+        """Tile shapes (azk_circle_exec_plan); unspecified knobs keep their current value."""
+        x = ExecPlan()
+        _ck(self.ctx._lib.azk_circle_prover_get_exec_plan(self.h, C.byref(x)))
+        for k, v in plan.items():
+            setattr(x, k, v)
+        _ck(self.ctx._lib.azk_circle_prover_set_exec_plan(self.h, C.byref(x)))
+
+    def exec_plan(self) -> dict:
+        x = ExecPlan()
+        _ck(self.ctx._lib.azk_circle_prover_get_exec_plan(self.h, C.byref(x)))
+        return _plan_dict(x)
+
+    def tune(self, coeffs: Buffer, offset: int = 0, database_path: str | None = None, force: bool = False,
+             warmup_runs: int = 0, measured_runs: int = 0, max_candidates: int = 0, validation_runs: int = 0) -> dict:
+        """Races tile shapes on these coefficients (azk_circle_prover_tune) and keeps the winner."""
+        c = TuningConfig(warmup_runs, measured_runs, max_candidates, validation_runs,
+                         database_path.encode() if database_path else None, int(force))
+        r = TuningResult()
+        _ck(self.ctx._lib.azk_circle_prover_tune(self.h, coeffs.h, offset, C.byref(c), C.byref(r)))
+        out = {k: getattr(r, k) for k, _ in TuningResult._fields_ if k != "winner"}
+        out["winner"] = _plan_dict(r.winner)
+        return out
+
+    def drift(self) -> dict:
+        d = DriftStatus()
+        _ck(self.ctx._lib.azk_circle_prover_drift(self.h, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in DriftStatus._fields_}
+
 
 def prove_batch(provers: list, coeff_buffers: list, offsets=None) -> tuple[list[bytes], dict]:
     """Many independent commits in one captured graph (azk_circle_prove_batch)."""
@@ -570,6 +661,7 @@ class CircleBatchPlan:
         h = vp()
         _ck(self.lib.azk_circle_batch_create(ph, bh, offs, n, C.byref(h)))
         self.h = h
+        _track(self)
         self._outs = [np.zeros(p.proof_size, np.uint8) for p in self.provers]
         self._op = (vp * n)(*[o.ctypes.data for o in self._outs])
         self._caps = (sz * n)(*[p.proof_size for p in self.provers])
@@ -603,6 +695,7 @@ class M31MultilinearSumcheckPlan(_Plan):
         h = vp()
         _ck(ctx._lib.azk_sumcheck_plan_create(ctx.h, log_n, C.byref(h)))
         self.h = h
+        _track(self)
         self.proof_size = int(ctx._lib.azk_sumcheck_proof_size(log_n))
         self._out = np.zeros(self.proof_size, np.uint8)
         self.last_stats: dict = {}
@@ -663,6 +756,7 @@ class PinnedArray:
         h = vp()
         _ck(self.lib.azk_host_alloc(n, C.byref(h)))
         self.h = h
+        _track(self)
         buf = (C.c_uint8 * max(n, 1)).from_address(h.value)
         self.array = np.frombuffer(buf, np.uint8, count=n).view(dt).reshape(shape)
 
@@ -699,6 +793,7 @@ class Server:
         h = vp()
         _ck(self.lib.azk_server_create(C.byref(d), C.byref(h)))
         self.h = h
+        _track(self)
         self._keep: dict = {}
 
     def submit_commit(self, desc: ProverDesc, coeffs: np.ndarray) -> int:

@@ -81,12 +81,16 @@ __device__ __forceinline__ qm31 qm_mul(qm31 x, qm31 y)
 
 constexpr uint32_t M31_INV2 = 1u << 30;
 
+// x * 2^-1 for canonical x: x / 2 when x is even, (x + p) / 2 = (x >> 1) + 2^30 when odd;
+// the same canonical value as m31_mul(x, M31_INV2) in three ops instead of ten.
+__device__ __forceinline__ uint32_t m31_half(uint32_t x) { return (x >> 1) | ((x & 1u) << 30); }
+
 // FRI fold of one adjacent pair with an M31 twiddle t embedded as (t, 0, 0, 0):
 // (l + r + ch * ((l - r) * t)) * 2^-1.   QM31FRIFold.swift:11-44
 __device__ __forceinline__ qm31 fri_fold_pair(qm31 l, qm31 r, uint32_t t, qm31 ch)
 {
     qm31 m = qm_add(qm_add(l, r), qm_mul(ch, qm_scale(qm_sub(l, r), t)));
-    return qm_scale(m, M31_INV2);
+    return {m31_half(m.a), m31_half(m.b), m31_half(m.c), m31_half(m.d)};
 }
 
 // Circle group: (x1, y1) * (x2, y2) = (x1 x2 - y1 y2, x1 y2 + y1 x2).  CircleDomain.swift:31-40

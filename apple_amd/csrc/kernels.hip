@@ -422,6 +422,21 @@ __global__ __launch_bounds__(256) void k_fold(const FoldDesc *__restrict__ tab, 
     tile_tree_qm31(lds, lds + 4 * T, T, base, tree, d.n_out);
 }
 
+// Streaming form when no commitment follows (the fold chain plan): one output per
+// thread over a full grid of 256-output blocks, measured fastest on MI355X (5.9 TB/s
+// at 2^26 -> 2^25 against 4.5-5.1 TB/s for grid-stride loops).
+__global__ __launch_bounds__(256) void k_fold_stream(const FoldDesc *__restrict__ tab, uint32_t ntab)
+{
+    uint32_t tile;
+    const FoldDesc &d = find_desc(tab, ntab, tile);
+    const uint64_t g = ((uint64_t)tile << 8) + threadIdx.x;
+    if (g >= d.n_out) return;
+    const uint32_t *ch = d.ch;
+    const qm31 c = {ch[0], ch[1], ch[2], ch[3]};
+    const uint4 l = d.in[2 * g], r = d.in[2 * g + 1];
+    d.out[g] = st_qm(fri_fold_pair(ld_qm(l), ld_qm(r), d.tw[g], c));
+}
+
 // Latency form for small layers: one output element per thread, T <= 256 outputs
 // per block; each thread folds its pair, hashes the new leaf and the block reduces
 // its leaf hashes in LDS (serial chain: one leaf permutation + log2 T levels).
@@ -680,6 +695,12 @@ void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks,
 {
     if (!blocks) return;
     hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
+}
+
+void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks)
+{
+    if (!blocks) return;
+    hipLaunchKernelGGL(k_fold_stream, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
 void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks)

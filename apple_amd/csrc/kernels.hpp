@@ -114,6 +114,8 @@ void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t 
 void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
 void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
 void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
+// No commitment: one output per thread, 256 outputs per block (log_t ignored).
+void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
 void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c);
 void launch_transcript(hipStream_t st, const TranscriptDesc *t, uint32_t n);
 void launch_copy(hipStream_t st, const CopyDesc *t, uint32_t n, uint32_t blocks);

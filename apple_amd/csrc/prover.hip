@@ -218,9 +218,10 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
         azk_circle_prover *p = j.p;
         const uint32_t R_j = p->d.rounds;
         const TranscriptOp *ops = (const TranscriptOp *)p->ops.p;
-        plan_fft(R.fft_pass, R.fft_final, j.src, p->layers[0], (const uint32_t *)p->fft_tw.p, p->d.log_n, p->log_t,
+        const ExecPlan &xp = p->xp;
+        plan_fft(R.fft_pass, R.fft_final, j.src, p->layers[0], (const uint32_t *)p->fft_tw.p, p->d.log_n, xp.log_tile_commit,
                  j.mode == 0, p->trees[0]);
-        plan_tree(R.tree0, p->trees[0], p->n, std::min(p->log_t, p->d.log_n));
+        plan_tree(R.tree0, p->trees[0], p->n, std::min(xp.log_tile_commit, p->d.log_n), xp.log_tree_chunk);
         uint64_t off = 0, m = p->n;
         for (uint32_t r = 0; r < R_j; ++r) {
             R.tr_a[r].add(TranscriptDesc{ops + p->prog_a[r].first, p->tstate, p->prog_a[r].second, 0}, 1);
@@ -230,8 +231,9 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
             const bool commit_next = r + 1 < R_j;
             const uint32_t lt = plan_fold(R.fold_big[r], R.fold_leaf[r], r == 0 ? (const void *)j.layer0() : (const void *)p->layers[r],
                                           p->layers[r + 1], (const uint32_t *)p->fold_tw.p + off, p->challenges + 4 * r, n_out,
-                                          p->log_t, commit_next ? p->trees[r + 1] : nullptr);
-            if (commit_next) plan_tree(R.tree_r[r], p->trees[r + 1], n_out, lt);
+                                          xp.log_tile_fold, commit_next ? p->trees[r + 1] : nullptr,
+                                          xp.log_leafwise_below, xp.log_tile_leafwise);
+            if (commit_next) plan_tree(R.tree_r[r], p->trees[r + 1], n_out, lt, xp.log_tree_chunk);
             off += n_out;
             m = n_out;
         }
@@ -472,10 +474,17 @@ azk_status prove_single(azk_circle_prover *p, const void *src, int mode, uint8_t
         if (s) return s;
         p->runner = std::move(r);
     }
-    azk_status s = run(*p->runner, p->use_graph && !p->timing, p->timing ? &p->phases : nullptr, stats);
+    azk_stats local{};
+    azk_stats *st = stats ? stats : &local;
+    azk_status s = run(*p->runner, p->use_graph && !p->timing, p->timing ? &p->phases : nullptr, st);
     if (s) return s;
     memcpy(proof, proof_of(*p->runner, 0), bytes);
     if (len) *len = bytes;
+    if (p->drift_on) {  // PlanDatabase.recordLiveObservation: EMA with alpha 0.2
+        const double x = (double)st->gpu_ns;
+        p->drift_ema_ns = p->drift_samples ? 0.2 * x + 0.8 * p->drift_ema_ns : x;
+        ++p->drift_samples;
+    }
     return AZK_OK;
 }
 
@@ -540,6 +549,7 @@ azk_status prover_create(azk_context *ctx, const azk_circle_prover_desc &d, azk_
         delete p;
         return s;
     }
+    ctx_retain(p->ctx);
     *out = p;
     return AZK_OK;
 }
@@ -560,7 +570,7 @@ extern "C" void azk_circle_prover_destroy(azk_circle_prover *p)
     if (!p) return;
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
-    delete p;
+    destroy_on_ctx(p);
 }
 
 extern "C" size_t azk_circle_prover_proof_size(const azk_circle_prover *p) { return p ? 4 * p->proof_words : 0; }
@@ -570,6 +580,26 @@ extern "C" azk_status azk_circle_prover_set_graph(azk_circle_prover *p, int enab
     CHECK_ARG(p);
     std::lock_guard<std::mutex> g(p->mu);
     p->use_graph = enable != 0;
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_circle_prover_set_exec_plan(azk_circle_prover *p, const azk_circle_exec_plan *plan)
+{
+    CHECK_ARG(p && plan);
+    azk_status s = check_exec_plan(*plan);
+    if (s) return s;
+    std::lock_guard<std::mutex> g(p->mu);
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    (void)hipStreamSynchronize(p->ctx->stream);
+    p->xp = to_exec_plan(*plan);
+    p->runner.reset();  // the cached program was built for the previous tiles
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *p, azk_circle_exec_plan *plan)
+{
+    CHECK_ARG(p && plan);
+    *plan = from_exec_plan(p->xp);
     return AZK_OK;
 }
 

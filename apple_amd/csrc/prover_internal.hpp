@@ -35,7 +35,7 @@ struct Runner;
 struct azk_circle_prover {
     azk_context *ctx = nullptr;
     azk_circle_prover_desc d{};
-    uint32_t log_t = azk::rt::kDefaultLogTile;
+    azk::rt::ExecPlan xp;  // tile shapes of the program (default or tuned)
     uint64_t n = 0, nfin = 0;
     azk::rt::DeviceMem fft_tw, fold_tw, arena, frames, ops, coeff_staging;
     std::vector<uint4 *> layers;    // rounds + 1 layers (layer 0 = codeword, layer R = final)
@@ -50,6 +50,10 @@ struct azk_circle_prover {
     bool use_graph = true, timing = false;
     std::unique_ptr<azk::prv::Runner> runner;  // cached single-job program
     azk_phase_times phases{};
+    // drift of a tuned plan: EMA of live prove times against the tuned median
+    bool drift_on = false;
+    uint32_t drift_samples = 0;
+    double drift_baseline_ns = 0, drift_ema_ns = 0;
     std::mutex mu;
     ~azk_circle_prover();
 };
@@ -101,6 +105,11 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk);
 void host_finals(const Runner &R);
 azk_status enqueue_b(Runner &R, hipStream_t st);
 const uint8_t *proof_of(const Runner &R, size_t k);
+// One prove through the prover's cached single-job program (rebuilt when the source,
+// the mode or the execution plan changed).  Caller holds p->mu and the context lock.
+azk_status prove_single(azk_circle_prover *p, const void *src, int mode, uint8_t *proof, size_t cap, size_t *len,
+                        azk_stats *stats);
+
 // Descriptor rules of CircleFRISecurityParametersV1 / CirclePCSFRIProofV1 plus the
 // device path's limits (folding step 1, no grinding).
 azk_status validate_desc(const azk_circle_prover_desc &d);

@@ -16,6 +16,17 @@ thread_local std::string g_err;
 
 namespace azk::rt {
 
+void ctx_retain(azk_context *c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void ctx_release(azk_context *c)
+{
+    if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
 azk_status fail(azk_status s, const char *fmt, ...)
 {
     char buf[512];
@@ -41,21 +52,53 @@ bool overlaps(const azk_buffer *a, size_t ao, size_t an, const azk_buffer *b, si
     uintptr_t a0 = (uintptr_t)a->ptr + ao, b0 = (uintptr_t)b->ptr + bo;
     return a0 < b0 + bn && b0 < a0 + an;
 }
-std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t)
+std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t log_t_pass)
 {
     std::vector<FftPass> v;
-    uint32_t lt = std::min(log_t, log_n);
+    uint32_t lt = std::min(log_t_final, log_n);
     uint32_t S = log_n - lt;
     if (!S) return v;
-    uint32_t kmax = log_t - 3;
+    uint32_t kmax = log_t_pass - 3;
     uint32_t P = (S + kmax - 1) / kmax, hi = log_n;
     for (uint32_t p = 0; p < P; ++p) {
         uint32_t k = S / P + (p < S % P ? 1 : 0);
         uint32_t lo = hi - k;
-        v.push_back({lo, k, log_t - k});
+        v.push_back({lo, k, std::min(log_t_pass - k, lo)});  // columns come from the lo low bits
         hi = lo;
     }
     return v;
+}
+
+azk_status check_exec_plan(const azk_circle_exec_plan &x)
+{
+    if (x.log_tile_commit < 6 || x.log_tile_commit > kDefaultLogTile || x.log_tile_fold < 6 ||
+        x.log_tile_fold > kDefaultLogTile || x.log_tile_leafwise < 5 || x.log_tile_leafwise > kLeafwiseLogTile ||
+        x.log_leafwise_below > 31 || x.log_tree_chunk < 4 || x.log_tree_chunk > kUpperLogChunk)
+        return fail(AZK_ERR_INVALID_KERNEL_CONFIG,
+                    "execution plan outside the supported tiles (commit/fold 6..11, leafwise 5..8, chunk 4..10)");
+    return AZK_OK;
+}
+
+ExecPlan to_exec_plan(const azk_circle_exec_plan &x)
+{
+    ExecPlan e;
+    e.log_tile_commit = x.log_tile_commit;
+    e.log_tile_fold = x.log_tile_fold;
+    e.log_leafwise_below = x.log_leafwise_below;
+    e.log_tile_leafwise = x.log_tile_leafwise;
+    e.log_tree_chunk = x.log_tree_chunk;
+    return e;
+}
+
+azk_circle_exec_plan from_exec_plan(const ExecPlan &e)
+{
+    azk_circle_exec_plan x{};
+    x.log_tile_commit = e.log_tile_commit;
+    x.log_tile_fold = e.log_tile_fold;
+    x.log_leafwise_below = e.log_leafwise_below;
+    x.log_tile_leafwise = e.log_tile_leafwise;
+    x.log_tree_chunk = e.log_tree_chunk;
+    return x;
 }
 
 azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw)
@@ -97,7 +140,8 @@ void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &fi
         if (passes.size() < fp.size()) passes.resize(fp.size());
         for (size_t i = 0; i < fp.size(); ++i) {
             FftPassDesc d{(const uint4 *)cur, (uint4 *)dst, tw, log_n, fp[i].lo, fp[i].k, fp[i].m, 0, 0};
-            passes[i].add(d, (uint32_t)(1ull << (log_n - log_t)), log_t);
+            const uint32_t tlog = fp[i].k + fp[i].m;
+            passes[i].add(d, (uint32_t)(1ull << (log_n - tlog)), tlog);
             cur = dst;
         }
     }
@@ -106,11 +150,12 @@ void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &fi
 }
 
 uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in, void *out, const uint32_t *tw,
-                   const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree)
+                   const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree, uint32_t log_leafwise_below,
+                   uint32_t log_t_leaf)
 {
     const uint32_t log_out = ilog2(n_out);
-    if (tree && n_out < kLeafwiseBelow) {
-        const uint32_t lt = std::min(kLeafwiseLogTile, log_out);
+    if (tree && log_out < log_leafwise_below) {
+        const uint32_t lt = std::min(log_t_leaf, log_out);
         leaf.add(FoldDesc{(const uint4 *)in, (uint4 *)out, tw, ch, tree, n_out, lt, 0}, (uint32_t)(n_out >> lt), lt);
         return lt;
     }
@@ -119,9 +164,9 @@ uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in
     return lt;
 }
 
-void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level)
+void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level, uint32_t log_chunk)
 {
-    auto sch = tree_schedule(m, level, kUpperLogChunk);
+    auto sch = tree_schedule(m, level, log_chunk);
     if (steps.size() < sch.size()) steps.resize(sch.size());
     for (size_t k = 0; k < sch.size(); ++k)
         steps[k].add(TreeDesc{tree, m, sch[k].level, sch[k].log_c, 0, 0}, (uint32_t)sch[k].chunks, sch[k].log_c);
@@ -179,11 +224,7 @@ azk_status azk_context_create(int device, azk_context **out)
 
 void azk_context_destroy(azk_context *ctx)
 {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    if (ctx) ctx_release(ctx);
 }
 
 azk_status azk_context_capabilities(azk_context *ctx, azk_capabilities *caps)
@@ -215,6 +256,7 @@ azk_status azk_buffer_alloc(azk_context *ctx, size_t bytes, azk_buffer **out)
     void *p = nullptr;
     if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return fail(AZK_ERR_ALLOCATION, "hipMalloc(%zu) failed", bytes);
     *out = new azk_buffer{ctx, p, bytes, true};
+    ctx_retain(ctx);
     return AZK_OK;
 }
 
@@ -222,6 +264,7 @@ azk_status azk_buffer_wrap(azk_context *ctx, void *device_ptr, size_t bytes, azk
 {
     CHECK_ARG(ctx && out && device_ptr);
     *out = new azk_buffer{ctx, device_ptr, bytes, false};
+    ctx_retain(ctx);
     return AZK_OK;
 }
 
@@ -232,7 +275,9 @@ void azk_buffer_free(azk_buffer *b)
         (void)hipSetDevice(b->ctx->device);
         (void)hipFree(b->ptr);
     }
+    azk_context *c = b->ctx;
     delete b;
+    ctx_release(c);
 }
 
 size_t azk_buffer_size(const azk_buffer *b) { return b ? b->bytes : 0; }
@@ -279,10 +324,11 @@ extern "C" azk_status azk_keccakf_plan_create(azk_context *ctx, const azk_keccak
     if (desc->input_stride < 200 || desc->output_stride < 200 || desc->input_stride % 8 || desc->output_stride % 8)
         return fail(AZK_ERR_INVALID_LAYOUT, "Keccak-f strides must be multiples of 8 and >= 200");
     *out = new azk_keccakf_plan{ctx, *desc, {}, {}, {}};
+    ctx_retain(ctx);
     return AZK_OK;
 }
 
-extern "C" void azk_keccakf_plan_destroy(azk_keccakf_plan *p) { delete p; }
+extern "C" void azk_keccakf_plan_destroy(azk_keccakf_plan *p) { destroy_on_ctx(p); }
 
 extern "C" azk_status azk_keccakf_permute_resident(azk_keccakf_plan *p, const azk_buffer *in, size_t in_off,
                                                    azk_buffer *out, size_t out_off, azk_stats *stats)
@@ -370,6 +416,7 @@ static azk_status hash_plan_create(azk_context *ctx, const azk_oneblock_desc *de
     if (desc->input_stride < desc->input_length || desc->output_stride < 32)
         return fail(AZK_ERR_INVALID_LAYOUT, "stride shorter than the message or digest");
     *out = new azk_hash_plan{ctx, *desc, suffix, {}, {}, {}};
+    ctx_retain(ctx);
     return AZK_OK;
 }
 
@@ -381,7 +428,7 @@ extern "C" azk_status azk_keccak_256_plan_create(azk_context *ctx, const azk_one
 {
     return hash_plan_create(ctx, d, 0x01, o);
 }
-extern "C" void azk_hash_plan_destroy(azk_hash_plan *p) { delete p; }
+extern "C" void azk_hash_plan_destroy(azk_hash_plan *p) { destroy_on_ctx(p); }
 
 extern "C" azk_status azk_hash_execute_resident(azk_hash_plan *p, const azk_buffer *in, size_t in_off, azk_buffer *out,
                                                 size_t out_off, azk_stats *stats)
@@ -476,11 +523,12 @@ extern "C" azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_
     azk_status s = p->tree.ensure(((size_t)2 * desc->leaf_count - 1) * 32);
     if (!s) s = p->path.ensure(32 * 32 + 32);
     if (s) { delete p; return s; }
+    ctx_retain(p->ctx);
     *out = p;
     return AZK_OK;
 }
 
-extern "C" void azk_merkle_plan_destroy(azk_merkle_plan *p) { delete p; }
+extern "C" void azk_merkle_plan_destroy(azk_merkle_plan *p) { destroy_on_ctx(p); }
 
 static azk_status merkle_enqueue(azk_merkle_plan *p, const void *leaves)
 {
@@ -605,10 +653,11 @@ extern "C" azk_status azk_prehashed_plan_create(azk_context *ctx, uint32_t leaf_
     HIP_TRY(hipSetDevice(ctx->device));
     azk_status s = p->tree.ensure(((size_t)2 * leaf_count - 1) * 32);
     if (s) { delete p; return s; }
+    ctx_retain(p->ctx);
     *out = p;
     return AZK_OK;
 }
-extern "C" void azk_prehashed_plan_destroy(azk_prehashed_plan *p) { delete p; }
+extern "C" void azk_prehashed_plan_destroy(azk_prehashed_plan *p) { destroy_on_ctx(p); }
 
 extern "C" azk_status azk_prehashed_commit_resident(azk_prehashed_plan *p, const azk_buffer *hashes, size_t off,
                                                     uint8_t root[32], azk_stats *stats)
@@ -676,10 +725,11 @@ extern "C" azk_status azk_codeword_plan_create(azk_context *ctx, uint32_t log_n,
     std::lock_guard<std::mutex> gc(ctx->mu);
     azk_status s = materialize_twiddles(ctx, log_n, 1, p->tw, nullptr);
     if (s) { delete p; return s; }
+    ctx_retain(p->ctx);
     *out = p;
     return AZK_OK;
 }
-extern "C" void azk_codeword_plan_destroy(azk_codeword_plan *p) { delete p; }
+extern "C" void azk_codeword_plan_destroy(azk_codeword_plan *p) { destroy_on_ctx(p); }
 
 extern "C" azk_status azk_codeword_execute_resident(azk_codeword_plan *p, const azk_buffer *coeffs, size_t coff,
                                                     azk_buffer *out, size_t ooff, azk_stats *stats)
@@ -754,10 +804,11 @@ extern "C" azk_status azk_fold_plan_create(azk_context *ctx, uint32_t log_n, uin
     if (!s) s = p->b.ensure((size_t)16 << (log_n - 1));
     if (!s) s = p->ch.ensure(16 * rounds);
     if (s) { delete p; return s; }
+    ctx_retain(p->ctx);
     *out = p;
     return AZK_OK;
 }
-extern "C" void azk_fold_plan_destroy(azk_fold_plan *p) { delete p; }
+extern "C" void azk_fold_plan_destroy(azk_fold_plan *p) { destroy_on_ctx(p); }
 
 static azk_status fold_enqueue(azk_fold_plan *p, const void *in, void *out)
 {
@@ -771,8 +822,10 @@ static azk_status fold_enqueue(azk_fold_plan *p, const void *in, void *out)
     for (uint32_t r = 0; r < p->rounds; ++r) {
         void *dst = (r + 1 == p->rounds) ? out : (r % 2 ? p->b.p : p->a.p);
         const uint64_t n_out = m / 2;
-        plan_fold(o.folds_big[r], o.folds_leaf[r], cur, dst, (const uint32_t *)p->fold_tw.p + off,
-                  (const uint32_t *)p->ch.p + 4 * r, n_out, kDefaultLogTile, nullptr);
+        // streaming form (no commitment): 256 outputs per block, the last block partial
+        o.folds_big[r].add(FoldDesc{(const uint4 *)cur, (uint4 *)dst, (const uint32_t *)p->fold_tw.p + off,
+                                    (const uint32_t *)p->ch.p + 4 * r, nullptr, n_out, 8, 0},
+                           (uint32_t)((n_out + 255) >> 8));
         cur = dst;
         off += n_out;
         m = n_out;
@@ -781,7 +834,7 @@ static azk_status fold_enqueue(azk_fold_plan *p, const void *in, void *out)
     azk_status s = o.arena.upload();
     if (s) return s;
     for (auto &l : o.folds_big)
-        launch_fold(p->ctx->stream, o.arena.at<FoldDesc>(l.off), l.n(), l.blocks, l.max_log);
+        launch_fold_stream(p->ctx->stream, o.arena.at<FoldDesc>(l.off), l.n(), l.blocks);
     return AZK_OK;
 }
 

@@ -15,11 +15,16 @@
 #include "../../include/azk.h"
 #include "kernels.hpp"
 
+#include <atomic>
+
 struct azk_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
     std::mutex mu;  // serializes use of `stream` across plans
+    // One reference for the owner (dropped by azk_context_destroy) and one per object
+    // built on the context (buffers, plans, provers): the context outlives them all.
+    std::atomic<int> refs{1};
 };
 
 struct azk_buffer {
@@ -32,6 +37,18 @@ struct azk_buffer {
 namespace azk::rt {
 
 azk_status fail(azk_status s, const char *fmt, ...);
+
+void ctx_retain(azk_context *c);
+void ctx_release(azk_context *c);
+// Deletes an object built on a context, then drops its reference to the context.
+template <class T>
+void destroy_on_ctx(T *p)
+{
+    if (!p) return;
+    azk_context *c = p->ctx;
+    delete p;
+    ctx_release(c);
+}
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
@@ -63,7 +80,6 @@ constexpr uint32_t kP = 0x7fffffffu;
 constexpr uint32_t kDefaultLogTile = 11;    // 2048-element tiles: 48 KB LDS per block
 constexpr uint32_t kUpperLogChunk = 10;     // upper-tree reduction chunk
 constexpr uint32_t kLeafwiseLogTile = 8;    // latency form: one leaf per thread, 256 per block
-constexpr uint64_t kLeafwiseBelow = 1ull << 19;  // layers smaller than this use the latency form
 
 struct DeviceMem {
     void *p = nullptr;
@@ -156,9 +172,10 @@ azk_status run_timed(azk_context *ctx, azk_stats *stats, F &&enqueue)
 
 struct FftPass { uint32_t lo, k, m; };
 
-// Stages log_n-1 .. lt (lt = min(log_t, log_n)) split into passes of at most
-// log_t - 3 stages, each on tiles of 2^log_t elements with >= 8 contiguous columns.
-std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t);
+// Stages log_n-1 .. lt (lt = min(log_t_final, log_n)) split into passes of at most
+// log_t_pass - 3 stages, each on tiles of up to 2^log_t_pass elements with >= 8
+// contiguous columns (fewer when the pass starts below stage log_t_pass - k).
+std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t log_t_pass = kDefaultLogTile);
 
 // FFT twiddles (and, with `fold_tw`, the first `rounds` stages inverted = FRI fold twiddles).
 azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw);
@@ -210,6 +227,19 @@ struct Launch {
 struct TreeStep { uint32_t level, log_c; uint64_t chunks; };
 std::vector<TreeStep> tree_schedule(uint64_t m, uint32_t level, uint32_t max_log_c);
 
+// Tile shapes of the commit program (azk_circle_exec_plan); the defaults are the
+// shapes used when no tuned plan is set.
+struct ExecPlan {
+    uint32_t log_tile_commit = kDefaultLogTile;    // final FFT pass + first-layer treelets
+    uint32_t log_tile_fold = kDefaultLogTile;      // committed folds, big-tile form
+    uint32_t log_leafwise_below = 19;              // layers < 2^this: one leaf per thread
+    uint32_t log_tile_leafwise = kLeafwiseLogTile; // latency-form tile
+    uint32_t log_tree_chunk = kUpperLogChunk;      // upper-tree reduction chunk
+};
+azk_status check_exec_plan(const azk_circle_exec_plan &x);
+ExecPlan to_exec_plan(const azk_circle_exec_plan &x);
+azk_circle_exec_plan from_exec_plan(const ExecPlan &x);
+
 // Appends the FFT of one job (src -> dst, optional leaf-hash tree) to pass/final launches.
 void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &final_, const void *src, void *dst,
               const uint32_t *tw, uint32_t log_n, uint32_t log_t, bool run_fft, uint32_t *tree);
@@ -217,8 +247,10 @@ void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &fi
 // Appends one fold round of one job to the big-tile or latency-form launch; returns the
 // log tile size (the level whose nodes remain for the upper tree).
 uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in, void *out, const uint32_t *tw,
-                   const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree);
+                   const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree,
+                   uint32_t log_leafwise_below = 19, uint32_t log_t_leaf = kLeafwiseLogTile);
 
-void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level);
+void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level,
+               uint32_t log_chunk = kUpperLogChunk);
 
 } // namespace azk::rt

@@ -554,11 +554,12 @@ azk_status azk_sumcheck_plan_create(azk_context *ctx, uint32_t log_n, azk_sumche
     if (!s && hipMemcpy(p->tmpl.p, p->tmpl_host.data(), p->tmpl_host.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
         s = fail(AZK_ERR_COMMAND_FAILED, "template upload failed");
     if (s) { delete p; return s; }
+    ctx_retain(p->ctx);
     *out = p;
     return AZK_OK;
 }
 
-void azk_sumcheck_plan_destroy(azk_sumcheck_plan *p) { delete p; }
+void azk_sumcheck_plan_destroy(azk_sumcheck_plan *p) { destroy_on_ctx(p); }
 
 azk_status azk_sumcheck_prove_resident(azk_sumcheck_plan *p, const azk_buffer *evals, size_t offset,
                                        const uint8_t *initial_digest, const uint32_t *claimed_sum,

@@ -226,6 +226,52 @@ typedef struct azk_phase_times {
 azk_status azk_circle_prover_set_timing(azk_circle_prover *prover, int enable);
 azk_status azk_circle_prover_phase_times(const azk_circle_prover *prover, azk_phase_times *out);
 
+/* ---- Execution plan and autotuner.  The tile shapes of a prover's program: the
+ * reference's plan configurations (MerkleCommitter treelet depth, threads per
+ * threadgroup) chosen by MetalProofPlanner.tuneMerkleCommitExecutionPlan
+ * (Planner/MetalProofPlanner.swift:20-117) and persisted by PlanDatabase
+ * (Planner/PlanDatabase.swift:43-320).  Every plan produces the same proof bytes. */
+typedef struct azk_circle_exec_plan {
+    uint32_t log_tile_commit;     /* final FFT pass + first-layer treelets: 2^6..2^11 leaves per block (11) */
+    uint32_t log_tile_fold;       /* committed folds, big-tile form: 2^6..2^11 outputs per block (11) */
+    uint32_t log_leafwise_below;  /* committed layers below 2^this fold one leaf per thread (19) */
+    uint32_t log_tile_leafwise;   /* 2^5..2^8 leaves per block in that form (8) */
+    uint32_t log_tree_chunk;      /* upper Merkle levels reduced per block, 4..10 (10) */
+} azk_circle_exec_plan;
+azk_status azk_circle_prover_set_exec_plan(azk_circle_prover *prover, const azk_circle_exec_plan *plan);
+azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *prover, azk_circle_exec_plan *plan);
+/* TuningConfiguration (MetalProofPlanner.swift:617-636); zero fields take the defaults. */
+typedef struct azk_tuning_config {
+    uint32_t warmup_runs;       /* 2 */
+    uint32_t measured_runs;     /* 5 */
+    uint32_t max_candidates;    /* 16 plans raced (coordinate descent from the default) */
+    uint32_t validation_runs;   /* 3 random inputs on which the winner must equal the default plan */
+    const char *database_path;  /* JSON-lines plan records; NULL = no persistence */
+    int force;                  /* 1 = race even when the database holds a winner */
+} azk_tuning_config;
+typedef struct azk_tuning_result {
+    azk_circle_exec_plan winner;   /* also set on the prover */
+    double median_gpu_ns, p95_gpu_ns, median_submit_ns;
+    double default_median_gpu_ns;  /* the default plan in the same race */
+    uint32_t candidates_measured, candidates_rejected;
+    int from_database;             /* 1 = winner read from the database, no race run */
+} azk_tuning_result;
+/* Races plans on the caller's coefficients: each candidate's first proof must equal the
+ * default plan's proof (else it is rejected), the objective is median GPU time + median
+ * submit time with p95 as tie break, and the winner must match the default plan on
+ * `validation_runs` random inputs before it is persisted and set. */
+azk_status azk_circle_prover_tune(azk_circle_prover *prover, const azk_buffer *coeffs, size_t offset,
+                                  const azk_tuning_config *config, azk_tuning_result *result);
+/* PlanDriftPolicy / PlanDriftStatus (PlanDatabase.swift:15-36): EMA (alpha 0.2) of live
+ * prove times against the tuned median; stale after >= 8 samples and > 25% drift.
+ * A stale plan is reported, not changed. */
+typedef struct azk_drift_status {
+    uint32_t samples;
+    int stale;
+    double ema_gpu_ns, baseline_gpu_ns, relative_drift;
+} azk_drift_status;
+azk_status azk_circle_prover_drift(const azk_circle_prover *prover, azk_drift_status *out);
+
 /* ---- Batched form: many independent commits of uneven sizes in one captured graph.
  * No reference counterpart (the reference runs one command buffer per job); each
  * job's proof is byte-identical to its plain single-job prove.  A batch plan captures
