@@ -114,6 +114,96 @@ __device__ __forceinline__ void kf_v(u2 (&a)[25])
     kround_v(a, 0x80008008u, 0x80000000u);
 }
 
+// ---------------------------------------------------------------- wavefront-cooperative form
+// One state spread over 25 lanes, two states per wave (lanes 0..24 and 32..56); lane
+// l = x + 5y holds A[x, y] as a (lo, hi) VGPR pair, which is also the state's word l
+// in sponge order.  Cross-lane steps pull with ds_bpermute; rho is a per-lane swap +
+// alignbit.  About 45 instructions per round instead of the one-lane form's 190, so one
+// permutation finishes sooner when only a few states are in flight (a transcript step,
+// the top levels of a Merkle tree).  Lanes 25..31 and
+// 57..63 carry a copy of nothing useful and are never read.  Each round takes two
+// exchanges: theta pulls columns x-1 and x+1 over all rows (20 pulls), and pi + chi
+// pull a lane's B[x], B[x+1], B[x+2] straight from their pi sources (6 pulls).
+struct CoopLane {
+    int cm[5], cp[5];    // byte addresses (lane * 4) of A[x-1, k] and A[x+1, k], k = 0..4
+    int pi0, pi1, pi2;   // pi sources of B[x, y], B[x+1, y], B[x+2, y]: A[(X + 3y) % 5, X]
+    uint32_t sh;         // rho: rotl by r = optional half swap, then alignbit by sh
+    bool sw, lead;       // lead: lane 0 of its state (iota)
+    uint32_t word;       // l = x + 5y (25.. for idle lanes)
+};
+
+__device__ __forceinline__ CoopLane coop_lane()
+{
+    // rho offsets r[x + 5y] as (swap, shift) pairs: r >= 32 swaps halves, r' = r mod 32
+    // is alignbit by 32 - r' (r' = 0 flips the swap and shifts by 0).
+    constexpr uint32_t kSwap = 0x00c1d165u;  // bit l: swap for lane l
+    constexpr uint8_t kShift[25] = {0, 31, 2, 4, 5, 28, 20, 26, 9, 12, 29, 22, 21,
+                                    7, 25, 23, 19, 17, 11, 24, 14, 30, 3, 8, 18};
+    const uint32_t lane = __lane_id(), base = lane & 32u, l = lane & 31u;
+    CoopLane c;
+    c.word = l;
+    if (l >= 25) {
+        const int self = (int)(lane * 4);
+        for (int k = 0; k < 5; ++k) c.cm[k] = c.cp[k] = self;
+        c.pi0 = c.pi1 = c.pi2 = self;
+        c.sh = 0;
+        c.sw = false;
+        c.lead = false;
+        return c;
+    }
+    const uint32_t x = l % 5, y = l / 5;
+    auto at = [&](uint32_t xx, uint32_t yy) { return (int)((base + xx % 5 + 5 * (yy % 5)) * 4); };
+    for (uint32_t k = 0; k < 5; ++k) {
+        c.cm[k] = at(x + 4, k);
+        c.cp[k] = at(x + 1, k);
+    }
+    c.pi0 = at(x + 3 * y, x);
+    c.pi1 = at(x + 1 + 3 * y, x + 1);
+    c.pi2 = at(x + 2 + 3 * y, x + 2);
+    c.sh = kShift[l];
+    c.sw = (kSwap >> l) & 1u;
+    c.lead = l == 0;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t bperm(int addr, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);
+}
+
+__device__ __forceinline__ void kf_coop(uint32_t &lo, uint32_t &hi, const CoopLane &c)
+{
+#pragma unroll 2
+    for (int r = 0; r < 24; ++r) {
+        // theta: columns x-1 and x+1 over all five rows in one exchange
+        uint32_t ml[5], mh[5], pl[5], ph[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            ml[k] = bperm(c.cm[k], lo);
+            mh[k] = bperm(c.cm[k], hi);
+            pl[k] = bperm(c.cp[k], lo);
+            ph[k] = bperm(c.cp[k], hi);
+        }
+        const uint32_t cml = xor3(xor3(ml[0], ml[1], ml[2]), ml[3], ml[4]);
+        const uint32_t cmh = xor3(xor3(mh[0], mh[1], mh[2]), mh[3], mh[4]);
+        const uint32_t cpl = xor3(xor3(pl[0], pl[1], pl[2]), pl[3], pl[4]);
+        const uint32_t cph = xor3(xor3(ph[0], ph[1], ph[2]), ph[3], ph[4]);
+        lo = xor3(lo, cml, __builtin_amdgcn_alignbit(cpl, cph, 31));
+        hi = xor3(hi, cmh, __builtin_amdgcn_alignbit(cph, cpl, 31));
+        // rho at the source lane; pi and chi's two row neighbours in one exchange
+        const uint32_t h2 = c.sw ? lo : hi, l2 = c.sw ? hi : lo;
+        const uint32_t th = __builtin_amdgcn_alignbit(h2, l2, c.sh), tl = __builtin_amdgcn_alignbit(l2, h2, c.sh);
+        const uint32_t b0l = bperm(c.pi0, tl), b1l = bperm(c.pi1, tl), b2l = bperm(c.pi2, tl);
+        const uint32_t b0h = bperm(c.pi0, th), b1h = bperm(c.pi1, th), b2h = bperm(c.pi2, th);
+        lo = b0l ^ (~b1l & b2l);
+        hi = b0h ^ (~b1h & b2h);
+        if (c.lead) {  // iota
+            lo ^= kRcLo[r];
+            hi ^= kRcHi[r];
+        }
+    }
+}
+
 // ---------------------------------------------------------------- SALU form
 __device__ __forceinline__ uint64_t rotl64(uint64_t x, int n) { return n == 0 ? x : ((x << n) | (x >> (64 - n))); }
 
