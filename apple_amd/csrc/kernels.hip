@@ -745,6 +745,8 @@ void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks,
 {
     if (!blocks) return;
     size_t lds = (size_t)(8u << max_log_c) * 4 + (size_t)(4u << max_log_c) * 4;
+    // 4 waves (one per SIMD): measured faster than 16, whose cooperative levels contend
+    // for the CU's LDS crossbar
     hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(256), lds, st, t, n);
 }
 

@@ -9,10 +9,10 @@
 // each launch covers that phase for every job (so small jobs' latency-bound tails
 // overlap), captured as two graphs around one host step:
 //   graph A: Circle FFT passes -> final pass fused with layer-0 leaf hashes + tile
-//            subtrees -> upper trees -> per round r: transcript A(r) {root r,
-//            challenge frame r, squeeze 4} || transcript B(r+1) {commitment frame r+1}
-//            on a side stream; fold r fused with layer r+1 hashing/tiles; upper trees
-//            -> gather of every job's transcript state + final layer -> one D2H copy
+//            subtrees -> upper trees -> per round r: transcript {root r, challenge
+//            frame r, squeeze 4, commitment frame r+1}; fold r fused with layer r+1
+//            hashing/tiles; upper trees -> gather of every job's transcript state +
+//            final layer -> one D2H copy
 //   host:    per job, final-layer frame + final layer + query frame absorbed and the
 //            Q indices squeezed (the reference also derives these on the CPU after
 //            readback, CircleFRIFoldPlan.swift:875-897; one wave runs a serial sponge
@@ -144,20 +144,19 @@ azk_status prover_build(azk_circle_prover *p)
     HIP_TRY(hipMemcpy(p->frames.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
     const uint8_t *fb = (const uint8_t *)p->frames.p;
 
+    // Round r: root r, challenge frame r, squeeze 4 (:819-828), then the next round's
+    // commitment frame (constant bytes; the squeeze read a copy of the state, so absorbing
+    // after it leaves the challenge unchanged).
     std::vector<TranscriptOp> ops;
-    for (uint32_t r = 0; r < d.rounds; ++r) {  // A(r): root, challenge frame, squeeze 4 (:819-828)
+    for (uint32_t r = 0; r < d.rounds; ++r) {
         const uint32_t first = (uint32_t)ops.size();
         if (r == 0) ops.push_back({TOP_LOAD_STATE, 0, 0, 0, 200, p->tstate_init, nullptr});
         const uint64_t m = n >> r;
         ops.push_back({TOP_ABSORB, 0x06, 0, 0, 32, p->trees[r] + 8 * (2 * m - 2), nullptr});
         ops.push_back({TOP_ABSORB, 0x06, 0, 0, chf[r].second, fb + chf[r].first, nullptr});
         ops.push_back({TOP_SQUEEZE_U32, 0, kP, 0, 4, nullptr, p->challenges + 4 * r});
+        if (r + 1 < d.rounds) ops.push_back({TOP_ABSORB, 0x06, 0, 0, cfr[r + 1].second, fb + cfr[r + 1].first, nullptr});
         p->prog_a.push_back({first, (uint32_t)ops.size() - first});
-    }
-    for (uint32_t r = 0; r < d.rounds; ++r) {  // B(r): commitment frame r (r >= 1)
-        const uint32_t first = (uint32_t)ops.size();
-        if (r > 0) ops.push_back({TOP_ABSORB, 0x06, 0, 0, cfr[r].second, fb + cfr[r].first, nullptr});
-        p->prog_b.push_back({first, (uint32_t)ops.size() - first});
     }
     s = p->ops.ensure(ops.size() * sizeof(TranscriptOp));
     if (s) return s;
@@ -209,7 +208,6 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
     R.jobs = jobs;
     for (const Job &j : jobs) R.rounds = std::max(R.rounds, j.p->d.rounds);
     R.tr_a.resize(R.rounds);
-    R.tr_b.resize(R.rounds);
     R.fold_big.resize(R.rounds);
     R.fold_leaf.resize(R.rounds);
     R.tree_r.resize(R.rounds);
@@ -225,8 +223,6 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
         uint64_t off = 0, m = p->n;
         for (uint32_t r = 0; r < R_j; ++r) {
             R.tr_a[r].add(TranscriptDesc{ops + p->prog_a[r].first, p->tstate, p->prog_a[r].second, 0}, 1);
-            if (r + 1 < R_j)
-                R.tr_b[r + 1].add(TranscriptDesc{ops + p->prog_b[r + 1].first, p->tstate, p->prog_b[r + 1].second, 0}, 1);
             const uint64_t n_out = m / 2;
             const bool commit_next = r + 1 < R_j;
             const uint32_t lt = plan_fold(R.fold_big[r], R.fold_leaf[r], r == 0 ? (const void *)j.layer0() : (const void *)p->layers[r],
@@ -286,24 +282,13 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
     for (auto &l : R.tree0) l.off = R.tables.put(l.v);
     for (uint32_t r = 0; r < R.rounds; ++r) {
         R.tr_a[r].off = R.tables.put(R.tr_a[r].v);
-        R.tr_b[r].off = R.tables.put(R.tr_b[r].v);
         R.fold_big[r].off = R.tables.put(R.fold_big[r].v);
         R.fold_leaf[r].off = R.tables.put(R.fold_leaf[r].v);
         for (auto &l : R.tree_r[r]) l.off = R.tables.put(l.v);
     }
     R.tail.off = R.tables.put(R.tail.v);
     R.emit.off = R.tables.put(R.emit.v);
-    s = R.tables.upload();
-    if (s) return s;
-    if (!R.side) HIP_TRY(hipStreamCreateWithFlags(&R.side, hipStreamNonBlocking));
-    while (R.ev_a.size() < R.rounds) {
-        hipEvent_t a, b;
-        HIP_TRY(hipEventCreateWithFlags(&a, hipEventDisableTiming));
-        R.ev_a.push_back(a);
-        HIP_TRY(hipEventCreateWithFlags(&b, hipEventDisableTiming));
-        R.ev_b.push_back(b);
-    }
-    return AZK_OK;
+    return R.tables.upload();
 }
 
 template <class D>
@@ -319,15 +304,8 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
     for (auto &l : R.tree0) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log);
     clk.mark(PH_TREE);
     for (uint32_t r = 0; r < R.rounds; ++r) {
-        if (r > 0 && !R.tr_b[r].empty()) HIP_TRY(hipStreamWaitEvent(st, R.ev_b[r], 0));
         launch_transcript(st, tab(R, R.tr_a[r]), R.tr_a[r].n());
         clk.mark(PH_TRANSCRIPT);
-        if (r + 1 < R.rounds && !R.tr_b[r + 1].empty()) {
-            HIP_TRY(hipEventRecord(R.ev_a[r], st));
-            HIP_TRY(hipStreamWaitEvent(R.side, R.ev_a[r], 0));
-            launch_transcript(R.side, tab(R, R.tr_b[r + 1]), R.tr_b[r + 1].n());
-            HIP_TRY(hipEventRecord(R.ev_b[r + 1], R.side));
-        }
         auto &fb = R.fold_big[r], &fl = R.fold_leaf[r];
         launch_fold(st, tab(R, fb), fb.n(), fb.blocks, fb.max_log);
         launch_fold_leafwise(st, tab(R, fl), fl.n(), fl.blocks);
@@ -497,7 +475,6 @@ void runner_reset(Runner &R)
     R.tree0.clear();
     R.rounds = 0;
     R.tr_a.clear();
-    R.tr_b.clear();
     R.fold_big.clear();
     R.fold_leaf.clear();
     R.tree_r.clear();

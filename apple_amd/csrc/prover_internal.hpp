@@ -42,7 +42,7 @@ struct azk_circle_prover {
     std::vector<uint32_t *> trees;  // one full tree per committed layer
     uint32_t *challenges = nullptr;
     uint64_t *tstate = nullptr, *tstate_init = nullptr;
-    std::vector<std::pair<uint32_t, uint32_t>> prog_a, prog_b;  // (first op, count) per round
+    std::vector<std::pair<uint32_t, uint32_t>> prog_a;  // transcript program of round r: (first op, count)
     std::vector<uint8_t> final_frame, query_frame;
     std::vector<uint32_t> proof_template;
     uint64_t roots_off = 0, final_off = 0, queries_off = 0, query_words = 0, proof_words = 0;
@@ -75,7 +75,7 @@ struct Runner {
     Launch<FftFinalDesc> fft_final;
     std::vector<Launch<TreeDesc>> tree0;
     uint32_t rounds = 0;
-    std::vector<Launch<TranscriptDesc>> tr_a, tr_b;
+    std::vector<Launch<TranscriptDesc>> tr_a;
     std::vector<Launch<FoldDesc>> fold_big, fold_leaf;
     std::vector<std::vector<Launch<TreeDesc>>> tree_r;
     Launch<CopyDesc> tail;
@@ -83,23 +83,18 @@ struct Runner {
     DeviceMem tail_dev, qidx_dev, proof_dev;
     PinnedMem tail_host, qidx_host, proof_host;
     std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job
-    hipStream_t side = nullptr;
-    std::vector<hipEvent_t> ev_a, ev_b;
     hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
     ~Runner()
     {
         if (graph_a) (void)hipGraphExecDestroy(graph_a);
         if (graph_b) (void)hipGraphExecDestroy(graph_b);
-        for (auto e : ev_a) (void)hipEventDestroy(e);
-        for (auto e : ev_b) (void)hipEventDestroy(e);
-        if (side) (void)hipStreamDestroy(side);
     }
 };
 
 // Builds the launch program of a job list into R (which must be fresh or reset).
 azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &jobs);
-// Forgets the job list and launch tables of R; keeps its device/pinned staging, side
-// stream and events for the next build.  R must not hold captured graphs.
+// Forgets the job list and launch tables of R; keeps its device/pinned staging for the
+// next build.  R must not hold captured graphs.
 void runner_reset(Runner &R);
 azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk);
 void host_finals(const Runner &R);
