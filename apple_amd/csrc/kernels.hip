@@ -347,32 +347,82 @@ __device__ __forceinline__ void butterfly(uint4 *s, uint32_t i1, uint32_t i2, ui
     s[i2] = st_qm(qm_sub(l, r));
 }
 
-// Stages lo + k - 1 .. lo of the Circle FFT on a tile of T = 2^(k + m) elements:
-// 2^k strided rows (stride 2^lo) x 2^m contiguous columns, staged through LDS.
-// Reference stage loop: CircleCodewordOracle.evaluateWithCircleFFT
-// (CircleCodeword.swift:72-98), Metal circle_codeword_fft_stage :1881-1908.
-__device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *__restrict__ tw, uint64_t n, uint32_t lo,
-                                                uint32_t k, uint32_t m, uint64_t high)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4 global_u32x4;
+typedef __attribute__((address_space(1))) const uint32_t global_u32;
+
+// The twiddles a tile needs, staged in LDS: stage lo + t uses 2^(k-1-t) consecutive
+// words tw[off(lo + t) + (high << (k-1-t)) + j], stored at tws[2^(k-1-t) + j] (slot 0
+// is unused, so the 2^k slots load with a uniform trip count).  off(s) = n - (n >> s)
+// (CircleCodeword.swift:166-171).
+__device__ __forceinline__ void stage_twiddles(uint32_t *tws, const uint32_t *__restrict__ tw, uint64_t n, uint32_t lo,
+                                               uint32_t k, uint64_t high)
 {
-    const uint32_t T = 1u << (k + m);
-    const uint32_t cmask = (1u << m) - 1u;
+    const global_u32 *g = (const global_u32 *)tw;
+    auto src = [&](uint32_t i) -> uint64_t {
+        if (i == 0) return 0;
+        const uint32_t level = 31 - __builtin_clz(i), sg = lo + k - 1 - level;
+        return (sg ? n - (n >> sg) : 0) + (high << level) + (i - (1u << level));
+    };
+    const uint32_t S = 1u << k;
+    if (S >= 256) {
+        const uint32_t J = S >> 8;
+        uint32_t v[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (j < J) v[j] = g[src(threadIdx.x + 256u * j)];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (j < J) tws[threadIdx.x + 256u * j] = v[j];
+    } else if (threadIdx.x < S) {
+        tws[threadIdx.x] = g[src(threadIdx.x)];
+    }
+}
+
+// Stages lo + k - 1 .. lo of the Circle FFT on a tile of T = 2^(k + m) elements held
+// in LDS as 2^k strided rows (stride 2^lo) x 2^m contiguous columns.  Butterfly b of
+// stage t pairs i1 = b with a zero bit inserted at position t + m and i2 = i1 + 2^(t+m);
+// its twiddle is tws[2^(k-1-t) + (b >> (t + m))].  Reference stage loop:
+// CircleCodewordOracle.evaluateWithCircleFFT (CircleCodeword.swift:72-98), Metal
+// circle_codeword_fft_stage :1881-1908.
+__device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *tws, uint32_t k, uint32_t m)
+{
+    const uint32_t half = 1u << (k + m - 1);
     for (int t = (int)k - 1; t >= 0; --t) {
-        const uint32_t sg = lo + (uint32_t)t;
-        const uint64_t off = sg ? n - (n >> sg) : 0;
-        const uint32_t tmask = (1u << t) - 1u;
-        for (uint32_t b = threadIdx.x; b < T / 2; b += blockDim.x) {
-            uint32_t c = b & cmask, rest = b >> m;
-            uint32_t jlo = rest & tmask, jhi = rest >> t;
-            uint32_t j = (jhi << (t + 1)) | jlo;
-            uint64_t h = (high << (k - (uint32_t)t - 1)) | jhi;
-            uint32_t w = tw[off + h];
-            butterfly(s, (j << m) | c, ((j + (1u << t)) << m) | c, w);
+        const uint32_t sh = (uint32_t)t + m;
+        const uint32_t *tw_t = tws + (1u << (k - 1 - t));
+        for (uint32_t b = threadIdx.x; b < half; b += blockDim.x) {
+            const uint32_t hp = b >> sh;
+            const uint32_t i1 = (hp << (sh + 1)) | (b & ((1u << sh) - 1));
+            butterfly(s, i1, i1 + (1u << sh), tw_t[hp]);
         }
         __syncthreads();
     }
 }
 
-// Non-final pass (lo >= 1): tile index -> (high bits, column chunk).
+// Loads T <= 2048 elements of a tile into LDS (index(L) maps tile slot L to the global
+// element).  For T >= 256 the trip count T / 256 is uniform, so a thread's loads are
+// all in flight (global, not flat) before the first LDS store.
+template <class F>
+__device__ __forceinline__ void load_tile(uint4 *s, const uint4 *__restrict__ src, uint32_t T, F &&index)
+{
+    if (T >= 256) {
+        const global_u32x4 *g = (const global_u32x4 *)src;
+        const uint32_t J = T >> 8;
+        u32x4 v[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (j < J) v[j] = g[index(threadIdx.x + 256u * j)];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (j < J) s[threadIdx.x + 256u * j] = make_uint4(v[j].x, v[j].y, v[j].z, v[j].w);
+    } else if (threadIdx.x < T) {
+        s[threadIdx.x] = src[index(threadIdx.x)];
+    }
+}
+
+// Non-final pass (lo >= 1): tile index -> (high bits, column chunk).  LDS: T elements,
+// then the tile's 2^k twiddle slots.
 __global__ __launch_bounds__(256) void k_fft_pass(const FftPassDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint4 s[];
@@ -386,21 +436,22 @@ __global__ __launch_bounds__(256) void k_fft_pass(const FftPassDesc *__restrict_
     const uint64_t chunk = (uint64_t)tile & ((1ull << chunk_bits) - 1);
     const uint64_t base = (high << (lo + k)) | (chunk << m);
     const uint32_t cmask = (1u << m) - 1u;
-    const uint4 *src = d.src;
-    uint4 *dst = d.dst;
-    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x)
-        s[L] = src[base | ((uint64_t)(L >> m) << lo) | (L & cmask)];
+    uint32_t *tws = reinterpret_cast<uint32_t *>(s + T);
+    stage_twiddles(tws, d.tw, n, lo, k, high);
+    auto at = [&](uint32_t L) { return base | ((uint64_t)(L >> m) << lo) | (L & cmask); };
+    load_tile(s, d.src, T, at);
     __syncthreads();
-    fft_tile_stages(s, d.tw, n, lo, k, m, high);
-    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x)
-        dst[base | ((uint64_t)(L >> m) << lo) | (L & cmask)] = s[L];
+    fft_tile_stages(s, tws, k, m);
+    uint4 *dst = d.dst;
+    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) dst[at(L)] = s[L];
 }
 
 // Final pass: stages log_t - 1 .. 0 on a contiguous tile of T = 2^log_t elements,
 // then the first-layer leaf hashes and the tile's Merkle subtree, as the reference
 // fuses stage 0 with the leaf hash (circle_codeword_fft_stage_leaf_hash :1910-1943)
 // and commits the hashes (SHA3PrehashedLeavesMerkleCommitPlan).  With run_fft == 0
-// the tile is only hashed (a codeword supplied by the caller).
+// the tile is only hashed (a codeword supplied by the caller).  The twiddles sit in
+// the tree's ping-pong region until the subtree is built.
 __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
@@ -411,14 +462,16 @@ __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restric
     uint4 *s = reinterpret_cast<uint4 *>(lds);
     const uint64_t n = 1ull << d.log_n;
     const uint64_t base = (uint64_t)tile * T;
-    const uint4 *src = d.src;
     uint32_t *tree = d.tree;
-    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) s[L] = src[base + L];
+    if (d.run_fft) stage_twiddles(lds + 4 * T, d.tw, n, 0, log_t, tile);
+    load_tile(s, d.src, T, [&](uint32_t L) { return base + L; });
     __syncthreads();
     if (d.run_fft) {
-        fft_tile_stages(s, d.tw, n, 0, log_t, 0, tile);
+        fft_tile_stages(s, lds + 4 * T, log_t, 0);
         uint4 *dst = d.dst;
         for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) dst[base + L] = s[L];
+        // every wave has read its elements before the subtree overwrites leaf pairs in place
+        __syncthreads();
     }
     if (tree) tile_tree_qm31(lds, lds + 4 * T, T, base, tree, n);
 }
@@ -714,7 +767,8 @@ void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint3
 void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
 {
     if (!blocks) return;
-    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), (size_t)16 << max_log_t, st, t, n);
+    // T elements + 2^k <= T twiddle words
+    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), (size_t)20 << max_log_t, st, t, n);
 }
 
 void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
