@@ -151,6 +151,44 @@ def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 96) -> di
     }
 
 
+FFT_BUTTERFLY_OPS = 74  # VALU instructions of one QM31 butterfly in this build's tile loop (gfx950 assembly)
+
+
+def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
+    """The codeword and fold-chain plans on their own at a size far beyond the 256 MB
+    MALL: Circle FFT of 2^26 QM31 coefficients (1 GiB) and one fold 2^26 -> 2^25.
+    Inputs are the 2^22-coefficient seeded block repeated (the kernels' work does not
+    depend on the values); parity of both plans is covered by the tests at <= 2^20."""
+    n = 1 << log_n
+    block = inputs.circle_fft_coefficients(seed=3, log_n=22)
+    cin, cout = ctx.buffer(16 * n), ctx.buffer(16 * n)
+    for k in range(n >> 22):
+        cin.upload(block, offset=k * block.nbytes)
+    fft = azk.CircleCodewordPlan(ctx, log_n)
+    fft.execute_resident(cin, cout)
+    t_fft = float(np.median([fft.execute_resident(cin, cout)["gpu_ns"] for _ in range(reps)])) * 1e-9
+    fold = azk.CircleFRIFoldChainPlan(ctx, log_n, 1)
+    fout = ctx.buffer(16 * n // 2)
+    ch = np.array([[123456789, 987654321, 55555, 7]], np.uint32)
+    fold.execute_resident(cout, ch, fout)
+    t_fold = float(np.median([fold.execute_resident(cout, ch, fout)["gpu_ns"] for _ in range(reps)])) * 1e-9
+    fold_bytes = 16 * n + 4 * (n // 2) + 16 * (n // 2)  # pairs in, M31 twiddles in, outputs
+    fft_ops = log_n * (n // 2) * FFT_BUTTERFLY_OPS
+    for b in (cin, cout, fout):
+        b.free()
+    return {
+        "fold": {"shape": f"2^{log_n} -> 2^{log_n - 1} QM31", "ms": t_fold * 1e3,
+                 "roofline": {"bound": "hbm", "achieved": fold_bytes / t_fold / 1e9, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": fold_bytes / t_fold / 1e9 / HBM_PEAK_GBS,
+                              "algorithmic_bytes": fold_bytes}},
+        "fft": {"shape": f"2^{log_n} QM31 codeword", "ms": t_fft * 1e3,
+                "algorithmic_gbps": 32 * n / t_fft / 1e9,
+                "roofline": {"bound": "valu", "achieved": fft_ops / t_fft / 1e12, "peak": VALU_PEAK_TOPS,
+                             "unit": "TOP/s", "frac": fft_ops / t_fft / 1e12 / VALU_PEAK_TOPS,
+                             "ops": fft_ops, "ops_per_butterfly": FFT_BUTTERFLY_OPS}},
+    }
+
+
 def calibrate_copy(ctx, log_count: int = 23) -> None:
     """A plain copy of known size for scaling the PMC byte counters: k_copy moves
     2^23 x 32 bytes (256 MiB) in and out (the prehashed commit's first kernel)."""
@@ -169,6 +207,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=2, help="commits timed on the CPU oracle (0 = skip)")
     ap.add_argument("--batched", type=int, default=1)
     ap.add_argument("--serving", type=int, default=1)
+    ap.add_argument("--standalone", type=int, default=1, help="FFT / fold plans at 2^26")
+    ap.add_argument("--standalone-only", type=int, default=0,
+                    help="profiling aid: only the 2^26 FFT / fold plans (one call each) and the calibration copy")
     ap.add_argument("--calibrate", type=int, default=0, help="run the known-size copy kernel (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -178,6 +219,10 @@ def main() -> int:
 
     ctx = azk.Context(0)
     caps = ctx.capabilities()
+    if args.standalone_only:
+        calibrate_copy(ctx)
+        print(json.dumps({"standalone": standalone_form(ctx, reps=1)}), flush=True)
+        return 0
     if args.calibrate:
         calibrate_copy(ctx)
     coeffs = inputs.circle_fft_coefficients(seed=1, log_n=LOG_N)
@@ -246,13 +291,15 @@ def main() -> int:
         "gbps": nbytes * commits_per_s / 1e9,
         "latency_ms": {"p50": 1e3 * pct(lat, 50), "p99": 1e3 * pct(lat, 99), "min": 1e3 * min(lat)},
         "keccak_perms_per_s": commit_perms(LOG_N, ROUNDS) * commits_per_s,
-        "phases_ms": phases, "bit_exact_vs_oracle": checked, "device": caps["name"],
+        "phases_ms": phases, "bit_exact_vs_oracle": checked, "device": caps["name"] or caps["arch"],
         "roofline": roofline,
     }
     if args.batched:
         line["batched"] = batched_form(ctx)
     if args.serving:
         line["serving"] = serving_form(ctx)
+    if args.standalone:
+        line["standalone"] = standalone_form(ctx)
     if args.cpu_sample:
         line["cpu_baseline"] = cpu_baseline(coeffs, args.cpu_sample)
     print(json.dumps(line), flush=True)
