@@ -193,6 +193,7 @@ def load() -> C.CDLL:
         "azk_sumcheck_prove": (st, [vp, vp, sz, vp, vp, sz, C.POINTER(sz), sp]),
         "azk_sumcheck_prove_resident": (st, [vp, vp, sz, vp, vp, C.POINTER(SumcheckResult), sp]),
         "azk_sumcheck_verify": (st, [vp, sz, C.POINTER(C.c_int)]),
+        "azk_circle_verify": (st, [vp, sz, vp, C.POINTER(C.c_int)]),
         "azk_circle_prover_set_exec_plan": (st, [vp, C.POINTER(ExecPlan)]),
         "azk_circle_prover_get_exec_plan": (st, [vp, C.POINTER(ExecPlan)]),
         "azk_circle_prover_tune": (st, [vp, vp, sz, C.POINTER(TuningConfig), C.POINTER(TuningResult)]),
@@ -730,6 +731,18 @@ class M31MultilinearSumcheckPlan(_Plan):
         self.last_stats = _stats_dict(s)
         return {"claimed_sum": r.claimed_sum, "final_evaluation": r.final_evaluation,
                 "initial_digest": bytes(r.initial_digest), "round_polys": polys, "challenges": ch}
+
+
+def circle_verify(proof: bytes, public_input_digest: bytes) -> bool:
+    """CirclePCSFRIProofCodecV1.decode + CirclePCSFRIProofVerifierV1.verify on the host."""
+    lib = load()
+    if len(public_input_digest) != 32:
+        raise AZKError(6, "public input digest must be 32 bytes")
+    b = np.frombuffer(proof, np.uint8) if len(proof) else np.zeros(1, np.uint8)
+    d = np.frombuffer(public_input_digest, np.uint8)
+    ok = C.c_int(0)
+    _ck(lib.azk_circle_verify(b.ctypes.data, len(proof), d.ctypes.data, C.byref(ok)))
+    return bool(ok.value)
 
 
 def sumcheck_verify(proof: bytes) -> bool:

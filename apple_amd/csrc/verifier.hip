@@ -1,0 +1,368 @@
+// Host verifier of Circle PCS/FRI proofs in CirclePCSFRIProofCodecV1 bytes: the service
+// a downstream consumer runs on what the prover emits.
+//
+// Reference: CirclePCSFRIProofCodecV1.decode (Sources/AppleZKProver/CircleProofFormat.swift:
+// 537-645) with the proof's construction rules (CirclePCSFRIProofV1.init :355-476, the
+// domain descriptor codec :71-127, CircleFRISecurityParametersV1 :129-158), the
+// transcript CircleFRITranscriptV1.derive (:684-983, grinding :753-794) and
+// CirclePCSFRIProofVerifierV1.verify (:1397-1546).  Malformed bytes are a layout error
+// (the reference's decode throws); a well-formed proof that does not verify is
+// reported as not accepted (the reference returns false).
+#include <vector>
+
+#include "host_sha3.hpp"
+#include "runtime_common.hpp"
+
+using namespace azk;
+using namespace azk::rt;
+
+namespace {
+
+// ---------------------------------------------------------------- field
+uint32_t fadd(uint32_t a, uint32_t b) { uint32_t s = a + b; return s >= kP ? s - kP : s; }
+uint32_t fsub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + (kP - b); }
+uint32_t fmul(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) % kP); }
+uint32_t fpow(uint32_t b, uint32_t e)
+{
+    uint32_t r = 1;
+    for (; e; e >>= 1, b = fmul(b, b))
+        if (e & 1) r = fmul(r, b);
+    return r;
+}
+uint32_t finv(uint32_t a) { return fpow(a, kP - 2); }
+
+struct Q { uint32_t a, b, c, d; };
+bool operator==(const Q &x, const Q &y) { return x.a == y.a && x.b == y.b && x.c == y.c && x.d == y.d; }
+Q qadd(Q x, Q y) { return {fadd(x.a, y.a), fadd(x.b, y.b), fadd(x.c, y.c), fadd(x.d, y.d)}; }
+Q qsub(Q x, Q y) { return {fsub(x.a, y.a), fsub(x.b, y.b), fsub(x.c, y.c), fsub(x.d, y.d)}; }
+// QM31 = CM31[u]/(u^2 - (2 + i)): (A + Bu)(C + Du) = AC + (2 + i)BD + (AD + BC)u
+Q qmul(Q x, Q y)
+{
+    auto cm = [](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, uint32_t &r0, uint32_t &r1) {
+        r0 = fsub(fmul(x0, y0), fmul(x1, y1));
+        r1 = fadd(fmul(x0, y1), fmul(x1, y0));
+    };
+    uint32_t ac0, ac1, bd0, bd1, ad0, ad1, bc0, bc1;
+    cm(x.a, x.b, y.a, y.b, ac0, ac1);
+    cm(x.c, x.d, y.c, y.d, bd0, bd1);
+    cm(x.a, x.b, y.c, y.d, ad0, ad1);
+    cm(x.c, x.d, y.a, y.b, bc0, bc1);
+    const uint32_t n0 = fsub(fadd(bd0, bd0), bd1), n1 = fadd(bd0, fadd(bd1, bd1));
+    return {fadd(ac0, n0), fadd(ac1, n1), fadd(ad0, bc0), fadd(ad1, bc1)};
+}
+bool canonical(const Q &x) { return x.a < kP && x.b < kP && x.c < kP && x.d < kP; }
+
+// ---------------------------------------------------------------- circle domain
+struct Pt { uint32_t x, y; };
+Pt padd(Pt p, Pt q) { return {fsub(fmul(p.x, q.x), fmul(p.y, q.y)), fadd(fmul(p.x, q.y), fmul(p.y, q.x))}; }
+// generator^idx, generator (2, 1268011823) of order 2^31 (CircleDomain.swift:31-60)
+Pt point_at(uint32_t idx)
+{
+    Pt r{1, 0}, g{2, 1268011823u};
+    for (idx &= 0x7fffffffu; idx; idx >>= 1, g = padd(g, g))
+        if (idx & 1) r = padd(r, g);
+    return r;
+}
+uint32_t double_x(uint32_t x) { return fsub(fadd(fmul(x, x), fmul(x, x)), 1); }
+
+struct Domain {
+    uint32_t log_n = 0, init = 0, half_log = 0;
+    // Circle point of storage index s (bit-reversed order): natural i < N/2 is
+    // init + step * i, the upper half the conjugates (CircleDomain.swift:273-331).
+    Pt storage_point(uint64_t s) const
+    {
+        uint64_t nat = 0, v = s;
+        for (uint32_t b = 0; b < log_n; ++b, v >>= 1) nat = (nat << 1) | (v & 1);
+        const uint64_t half = 1ull << half_log, step = 1ull << (31 - half_log);
+        uint32_t idx;
+        if (nat < half) idx = (uint32_t)((init + step * nat) & 0x7fffffffu);
+        else idx = (uint32_t)((0x80000000ull - ((init + step * (nat - half)) & 0x7fffffffu)) & 0x7fffffffu);
+        return point_at(idx);
+    }
+    // Round r's domain coordinate of pair p: y of the pair's left point for r = 0, x of
+    // the left point doubled r - 1 times afterwards (CircleFRILayerOracleV1.inverseDomainLayers,
+    // CircleDomain.swift:469-508); the fold multiplies by its inverse.
+    uint32_t fold_coordinate(uint32_t r, uint64_t p) const
+    {
+        if (r == 0) return storage_point(2 * p).y;
+        uint32_t x = storage_point(p << (r + 1)).x;
+        for (uint32_t k = 1; k < r; ++k) x = double_x(x);
+        return x;
+    }
+};
+
+// ---------------------------------------------------------------- decode
+struct Reader {
+    const uint8_t *p;
+    size_t n, at = 0;
+    bool ok = true;
+    bool take(size_t k) { if (!ok || k > n - at) { ok = false; return false; } return true; }
+    uint32_t u32() { if (!take(4)) return 0; uint32_t v; memcpy(&v, p + at, 4); at += 4; return v; }
+    uint64_t u64() { if (!take(8)) return 0; uint64_t v; memcpy(&v, p + at, 8); at += 8; return v; }
+    const uint8_t *bytes(size_t k) { if (!take(k)) return nullptr; const uint8_t *q = p + at; at += k; return q; }
+    Q qm() { Q q{u32(), u32(), u32(), u32()}; return q; }
+    bool end() const { return at == n; }
+};
+
+struct Opening { uint64_t leaf; Q value; const uint8_t *siblings; uint32_t count; };
+struct Layer { uint32_t index; uint64_t pair; Opening left, right; };
+struct Query { uint64_t pair; std::vector<Layer> layers; };
+
+struct Proof {
+    const uint8_t *descriptor = nullptr;  // the 44 bytes as read
+    Domain dom;
+    uint32_t log_blowup = 0, queries = 0, folding = 0, grinding = 0;
+    const uint8_t *digest = nullptr;
+    std::vector<const uint8_t *> roots;
+    std::vector<Q> final_layer;
+    std::vector<Query> qs;
+    std::vector<Opening> claimed;
+    bool has_nonce = false;
+    uint64_t nonce = 0;
+};
+
+bool read_opening(Reader &r, Opening &o)
+{
+    o.leaf = r.u64();
+    o.value = r.qm();
+    o.count = r.u32();
+    if (!r.ok || o.count > 64) return false;
+    o.siblings = r.bytes((size_t)32 * o.count);
+    return r.ok && canonical(o.value);
+}
+
+const uint8_t kProofMagic[8] = {0x41, 0x5a, 0x4b, 0x50, 0x46, 0x56, 0x31, 0x00};
+const uint8_t kDescMagic[8] = {0x41, 0x5a, 0x4b, 0x43, 0x44, 0x56, 0x31, 0x00};
+
+azk_status decode(const uint8_t *data, size_t len, Proof &pf)
+{
+    auto bad = [](const char *what) { return fail(AZK_ERR_INVALID_LAYOUT, "Circle PCS proof: %s", what); };
+    Reader r{data, len};
+    const uint8_t *magic = r.bytes(8);
+    if (!magic || memcmp(magic, kProofMagic, 8)) return bad("magic");
+    if (r.u32() != 1 || r.u32() != 1) return bad("proof or transcript version");
+    // domain descriptor (44 bytes, length-prefixed)
+    if (r.u32() != 44) return bad("descriptor length");
+    pf.descriptor = r.bytes(44);
+    if (!pf.descriptor) return bad("descriptor");
+    {
+        Reader d{pf.descriptor, 44};
+        const uint8_t *m = d.bytes(8);
+        if (memcmp(m, kDescMagic, 8) || d.u32() != 1 || d.u32() != kP || d.u32() != 31 || d.u32() != 2 ||
+            d.u32() != 1268011823u)
+            return bad("descriptor header");
+        pf.dom.log_n = d.u32();
+        pf.dom.init = d.u32() & 0x7fffffffu;
+        pf.dom.half_log = d.u32();
+        const uint32_t storage = d.u32();
+        if (pf.dom.log_n < 1 || pf.dom.log_n > 30 || pf.dom.half_log != pf.dom.log_n - 1 || storage != 1)
+            return bad("descriptor shape");
+        // isCanonical: 4 * halfCosetInitialIndex = subgroup generator of the half coset
+        if (((uint64_t)pf.dom.init * 4 & 0x7fffffffu) != ((1ull << (31 - pf.dom.half_log)) & 0x7fffffffu))
+            return bad("descriptor coset");
+    }
+    pf.log_blowup = r.u32();
+    pf.queries = r.u32();
+    pf.folding = r.u32();
+    pf.grinding = r.u32();
+    if (!r.ok || !pf.log_blowup || !pf.queries || !pf.folding || pf.grinding > 64 ||
+        (uint64_t)pf.log_blowup * pf.queries + pf.grinding > 0xffffffffull || pf.folding != 1)
+        return bad("security parameters");
+    if (r.u32() != 32) return bad("public input digest length");
+    pf.digest = r.bytes(32);
+    const uint32_t R = r.u32();
+    if (!r.ok || R == 0 || R > pf.dom.log_n) return bad("commitment count");
+    for (uint32_t k = 0; k < R; ++k) pf.roots.push_back(r.bytes(32));
+    const uint64_t nfin = r.u64();
+    if (!r.ok || nfin != (1ull << pf.dom.log_n) >> R) return bad("final layer size");
+    pf.final_layer.resize(nfin);
+    for (auto &q : pf.final_layer) {
+        q = r.qm();
+        if (!canonical(q)) return bad("non-canonical final layer");
+    }
+    const uint32_t nq = r.u32();
+    if (!r.ok || nq != pf.queries) return bad("query count");
+    const uint64_t n = 1ull << pf.dom.log_n;
+    pf.qs.resize(nq);
+    for (Query &q : pf.qs) {
+        q.pair = r.u64();
+        const uint32_t nl = r.u32();
+        if (!r.ok || nl != R || q.pair >= n / 2) return bad("query shape");
+        q.layers.resize(nl);
+        for (uint32_t k = 0; k < nl; ++k) {  // validateQueryShape (:420-446)
+            Layer &l = q.layers[k];
+            l.index = r.u32();
+            l.pair = r.u64();
+            if (!read_opening(r, l.left) || !read_opening(r, l.right)) return bad("opening");
+            const uint64_t m = n >> k;
+            const uint32_t depth = pf.dom.log_n - k;
+            if (l.index != k || l.pair != q.pair >> k || m <= 1 || l.pair >= m / 2 || l.left.leaf != 2 * l.pair ||
+                l.right.leaf != 2 * l.pair + 1 || l.left.count != depth || l.right.count != depth)
+                return bad("query layer shape");
+        }
+    }
+    if (!r.end()) {  // claimed evaluation openings, then the grinding nonce
+        const uint32_t nc = r.u32();
+        if (!r.ok || nc > n) return bad("claimed openings");
+        uint64_t prev = 0;
+        for (uint32_t k = 0; k < nc; ++k) {
+            Opening o;
+            if (!read_opening(r, o) || o.leaf >= n || o.count != pf.dom.log_n || (k && o.leaf <= prev))
+                return bad("claimed opening shape");
+            prev = o.leaf;
+            pf.claimed.push_back(o);
+        }
+        if (!r.end()) {
+            pf.nonce = r.u64();
+            pf.has_nonce = true;
+        }
+    }
+    if (!r.ok || !r.end()) return bad("trailing or truncated bytes");
+    if ((pf.grinding > 0) != pf.has_nonce) return bad("grinding nonce shape");
+    return AZK_OK;
+}
+
+// ---------------------------------------------------------------- transcript
+void put32(std::vector<uint8_t> &f, uint32_t v) { for (int i = 0; i < 4; ++i) f.push_back((uint8_t)(v >> (8 * i))); }
+void put64(std::vector<uint8_t> &f, uint64_t v) { for (int i = 0; i < 8; ++i) f.push_back((uint8_t)(v >> (8 * i))); }
+std::vector<uint8_t> frame(uint8_t type)
+{
+    static const char dom[] = "AppleZKProver.CircleFRI.PCS.V1";
+    std::vector<uint8_t> f;
+    put32(f, (uint32_t)(sizeof dom - 1));
+    f.insert(f.end(), dom, dom + sizeof dom - 1);
+    put32(f, 1);
+    f.push_back(type);
+    return f;
+}
+
+// CircleFRITranscriptV1.derive; false when the grinding nonce misses its target.
+bool derive(const Proof &pf, std::vector<Q> &challenges, std::vector<uint32_t> &pairs)
+{
+    host::Transcript t;
+    auto absorb = [&](const std::vector<uint8_t> &f) { t.absorb(f.data(), f.size()); };
+    std::vector<uint8_t> f = frame(0);
+    for (uint32_t w : {1u, 1u, kP, 2u, 1268011823u}) put32(f, w);
+    absorb(f);
+    // the descriptor as CircleDomainDescriptorCodecV1.encode writes it (:75-93)
+    f.assign(kDescMagic, kDescMagic + 8);
+    for (uint32_t w : {1u, kP, 31u, 2u, 1268011823u, pf.dom.log_n, pf.dom.init, pf.dom.half_log, 1u}) put32(f, w);
+    absorb(f);
+    f = frame(1);
+    for (uint32_t w : {pf.log_blowup, pf.queries, pf.folding, pf.grinding, pf.log_blowup * pf.queries + pf.grinding})
+        put32(f, w);
+    absorb(f);
+    f = frame(2);
+    put32(f, 32);
+    absorb(f);
+    t.absorb(pf.digest, 32);
+    for (uint32_t k = 0; k < pf.roots.size(); ++k) {
+        f = frame(3);
+        put32(f, k);
+        put32(f, 32);
+        absorb(f);
+        t.absorb(pf.roots[k], 32);
+        f = frame(4);
+        put32(f, k);
+        put32(f, kP);
+        put32(f, 4);
+        absorb(f);
+        uint32_t l[4];
+        t.squeeze_u32(4, kP, l);
+        challenges.push_back({l[0], l[1], l[2], l[3]});
+    }
+    f = frame(5);
+    put64(f, pf.final_layer.size());
+    put64(f, pf.final_layer.size() * 16);
+    absorb(f);
+    std::vector<uint8_t> fl(pf.final_layer.size() * 16);
+    for (size_t k = 0; k < pf.final_layer.size(); ++k) memcpy(fl.data() + 16 * k, &pf.final_layer[k], 16);
+    t.absorb(fl.data(), fl.size());
+    if (pf.grinding) {  // applyGrindingIfNeeded: frame, nonce, then 32 squeezed bytes
+        f = frame(7);
+        put32(f, pf.grinding);
+        put32(f, 8);
+        absorb(f);
+        uint8_t nb[8];
+        memcpy(nb, &pf.nonce, 8);
+        t.absorb(nb, 8);
+        uint8_t dg[32];
+        memcpy(dg, t.s, 32);
+        uint32_t bits = pf.grinding, i = 0;
+        for (; bits >= 8; bits -= 8, ++i)
+            if (dg[i]) return false;
+        if (bits && (dg[i] & (uint8_t)(0xff << (8 - bits)))) return false;
+    }
+    f = frame(6);
+    put32(f, pf.queries);
+    put64(f, (1ull << pf.dom.log_n) / 2);
+    absorb(f);
+    pairs.resize(pf.queries);
+    t.squeeze_u32(pf.queries, (uint32_t)((1ull << pf.dom.log_n) / 2), pairs.data());
+    return true;
+}
+
+// MerkleOracle.verifySHA3_256 of a 16-byte QM31 leaf.
+bool verify_opening(const Opening &o, const uint8_t *root)
+{
+    uint8_t node[32], msg[64];
+    host::sha3_256(reinterpret_cast<const uint8_t *>(&o.value), 16, node);
+    uint64_t idx = o.leaf;
+    for (uint32_t k = 0; k < o.count; ++k, idx >>= 1) {
+        const uint8_t *sib = o.siblings + 32 * k;
+        if (idx & 1) { memcpy(msg, sib, 32); memcpy(msg + 32, node, 32); }
+        else { memcpy(msg, node, 32); memcpy(msg + 32, sib, 32); }
+        host::sha3_256(msg, 64, node);
+    }
+    return memcmp(node, root, 32) == 0;
+}
+
+bool verify(const Proof &pf, const uint8_t *expected_digest)
+{
+    if (memcmp(pf.digest, expected_digest, 32)) return false;
+    std::vector<Q> ch;
+    std::vector<uint32_t> pairs;
+    if (!derive(pf, ch, pairs)) return false;
+    const Q inv2{1u << 30, 0, 0, 0};
+    for (size_t qi = 0; qi < pf.qs.size(); ++qi) {
+        const Query &q = pf.qs[qi];
+        if (q.pair != pairs[qi]) return false;
+        uint64_t pair = pairs[qi];
+        bool have = false;
+        uint64_t exp_index = 0;
+        Q exp_value{};
+        for (uint32_t r = 0; r < q.layers.size(); ++r) {
+            const Layer &l = q.layers[r];
+            if (!verify_opening(l.left, pf.roots[r]) || !verify_opening(l.right, pf.roots[r])) return false;
+            if (have) {
+                if (exp_index == 2 * pair) { if (!(exp_value == l.left.value)) return false; }
+                else if (exp_index == 2 * pair + 1) { if (!(exp_value == l.right.value)) return false; }
+                else return false;
+            }
+            const uint32_t coord = pf.dom.fold_coordinate(r, pair);
+            if (coord == 0) return false;
+            const Q t{finv(coord), 0, 0, 0};
+            const Q even = qadd(l.left.value, l.right.value), odd = qsub(l.left.value, l.right.value);
+            exp_value = qmul(qadd(even, qmul(ch[r], qmul(odd, t))), inv2);
+            exp_index = pair;
+            have = true;
+            pair >>= 1;
+        }
+        if (!have || exp_index >= pf.final_layer.size() || !(pf.final_layer[exp_index] == exp_value)) return false;
+    }
+    return true;
+}
+
+} // namespace
+
+extern "C" azk_status azk_circle_verify(const uint8_t *proof, size_t len, const uint8_t *public_input_digest,
+                                        int *accepted)
+{
+    CHECK_ARG(proof && public_input_digest && accepted);
+    *accepted = 0;
+    Proof pf;
+    azk_status s = decode(proof, len, pf);
+    if (s) return s;
+    *accepted = verify(pf, public_input_digest) ? 1 : 0;
+    return AZK_OK;
+}

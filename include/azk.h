@@ -272,6 +272,12 @@ typedef struct azk_drift_status {
 } azk_drift_status;
 azk_status azk_circle_prover_drift(const azk_circle_prover *prover, azk_drift_status *out);
 
+/* CirclePCSFRIProofCodecV1.decode + CirclePCSFRIProofVerifierV1.verify
+ * (CircleProofFormat.swift:537-645,1397-1546) on the host: *accepted = 1 when the proof
+ * verifies against the 32-byte public-input digest, 0 when it does not; malformed bytes
+ * give AZK_ERR_INVALID_LAYOUT (the reference's decode throws). */
+azk_status azk_circle_verify(const uint8_t *proof, size_t len, const uint8_t *public_input_digest, int *accepted);
+
 /* ---- Batched form: many independent commits of uneven sizes in one captured graph.
  * No reference counterpart (the reference runs one command buffer per job); each
  * job's proof is byte-identical to its plain single-job prove.  A batch plan captures
