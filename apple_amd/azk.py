@@ -97,7 +97,7 @@ def _plan_dict(x: ExecPlan) -> dict:
 
 
 class ServerDesc(C.Structure):
-    _fields_ = [("device", C.c_int), ("max_batch", C.c_uint32), ("sumcheck_lanes", C.c_uint32),
+    _fields_ = [("device", C.c_int), ("max_batch", C.c_uint32), ("aux_lanes", C.c_uint32),
                 ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64)]
 
 
@@ -202,6 +202,7 @@ def load() -> C.CDLL:
         "azk_server_destroy": (None, [vp]),
         "azk_server_submit_commit": (st, [vp, C.POINTER(ProverDesc), vp, sz, C.POINTER(C.c_uint64)]),
         "azk_server_submit_sumcheck": (st, [vp, vp, sz, C.POINTER(C.c_uint64)]),
+        "azk_server_submit_fold": (st, [vp, u32, u32, vp, sz, vp, C.POINTER(C.c_uint64)]),
         "azk_server_wait": (st, [vp, C.c_uint64, C.POINTER(JobInfo)]),
         "azk_server_take": (st, [vp, C.c_uint64, vp, sz, C.POINTER(sz)]),
         "azk_now_ns": (C.c_uint64, []),
@@ -800,9 +801,9 @@ def prover_desc(log_n: int, rounds: int, security=(2, 32, 1, 0), public_input_di
 class Server:
     """Serving form (azk_server_*): commits and sum-checks submitted at any time."""
 
-    def __init__(self, device: int = 0, max_batch: int = 64, sumcheck_lanes: int = 4, hbm_budget: int = 0):
+    def __init__(self, device: int = 0, max_batch: int = 64, aux_lanes: int = 4, hbm_budget: int = 0):
         self.lib = load()
-        d = ServerDesc(device, max_batch, sumcheck_lanes, 0, hbm_budget)
+        d = ServerDesc(device, max_batch, aux_lanes, 0, hbm_budget)
         h = vp()
         _ck(self.lib.azk_server_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -820,6 +821,14 @@ class Server:
         a = np.ascontiguousarray(evals, dtype=np.uint32)
         t = C.c_uint64(0)
         _ck(self.lib.azk_server_submit_sumcheck(self.h, a.ctypes.data, a.size, C.byref(t)))
+        self._keep[t.value] = a
+        return t.value
+
+    def submit_fold(self, log_n: int, rounds: int, evals: np.ndarray, challenges) -> int:
+        a = np.ascontiguousarray(evals, dtype=np.uint32)
+        ch = np.ascontiguousarray(np.asarray(challenges, dtype=np.uint32).reshape(-1))
+        t = C.c_uint64(0)
+        _ck(self.lib.azk_server_submit_fold(self.h, log_n, rounds, a.ctypes.data, a.nbytes, ch.ctypes.data, C.byref(t)))
         self._keep[t.value] = a
         return t.value
 

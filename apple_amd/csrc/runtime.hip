@@ -799,7 +799,7 @@ extern "C" azk_status azk_codeword_read_twiddles(azk_codeword_plan *p, uint32_t 
 struct azk_fold_plan {
     azk_context *ctx;
     uint32_t log_n, rounds;
-    DeviceMem fft_tw, fold_tw, a, b, ch;
+    DeviceMem fft_tw, fold_tw, a, b, ch, in, out;  // in/out: staging of the host-array variant
     OneShot shot;
     std::mutex mu;
 };
@@ -809,7 +809,7 @@ extern "C" azk_status azk_fold_plan_create(azk_context *ctx, uint32_t log_n, uin
     CHECK_ARG(ctx && out);
     if (log_n < 1 || log_n > 28 || rounds < 1 || rounds > log_n || rounds > (uint32_t)kMaxRounds)
         return fail(AZK_ERR_INVALID_LAYOUT, "fold chain log_n %u rounds %u", log_n, rounds);
-    auto *p = new azk_fold_plan{ctx, log_n, rounds, {}, {}, {}, {}, {}, {}, {}};
+    auto *p = new azk_fold_plan{ctx, log_n, rounds, {}, {}, {}, {}, {}, {}, {}, {}, {}};
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> gc(ctx->mu);
     azk_status s = materialize_twiddles(ctx, log_n, rounds, p->fft_tw, &p->fold_tw);
@@ -881,14 +881,21 @@ extern "C" azk_status azk_fold_execute(azk_fold_plan *p, const void *evals, size
     CHECK_ARG(p && evals && challenges && out);
     const size_t ib = (size_t)16 << p->log_n, ob = (size_t)16 << (p->log_n - p->rounds);
     if (in_bytes < ib || out_bytes < ob) return fail(AZK_ERR_INVALID_LAYOUT, "host arrays too short");
-    azk_buffer *bi = nullptr, *bo = nullptr;
-    azk_status s = azk_buffer_alloc(p->ctx, ib, &bi);
-    if (!s) s = azk_buffer_alloc(p->ctx, ob, &bo);
-    if (!s) s = azk_buffer_upload(bi, 0, evals, ib);
-    if (!s) s = azk_fold_execute_resident(p, bi, 0, challenges, bo, 0, stats);
-    if (!s) s = azk_buffer_download(bo, 0, out, ob);
-    azk_buffer_free(bi);
-    azk_buffer_free(bo);
-    return s;
+    azk_buffer bi{}, bo{};
+    {
+        std::lock_guard<std::mutex> g(p->mu);
+        HIP_TRY(hipSetDevice(p->ctx->device));
+        azk_status s = p->in.ensure(ib);  // staging kept by the plan between calls
+        if (!s) s = p->out.ensure(ob);
+        if (s) return s;
+        bi = azk_buffer{p->ctx, p->in.p, ib, false};
+        bo = azk_buffer{p->ctx, p->out.p, ob, false};
+        HIP_TRY(hipMemcpy(p->in.p, evals, ib, hipMemcpyHostToDevice));
+    }
+    azk_status s = azk_fold_execute_resident(p, &bi, 0, challenges, &bo, 0, stats);
+    if (s) return s;
+    std::lock_guard<std::mutex> g(p->mu);
+    HIP_TRY(hipMemcpy(out, p->out.p, ob, hipMemcpyDeviceToHost));
+    return AZK_OK;
 }
 

@@ -11,9 +11,10 @@
 //    the next arrivals and starts their coefficient uploads on a separate copy stream,
 //    so the next iteration starts on resident inputs.  Provers (their layers, trees,
 //    twiddles and input staging) stay cached per shape and are reused.
-//  * sum-check lanes (threads, one stream each): a job's words are hashed on the host
-//    (the serial initial digest) while its device rounds run.  An idle lane keeps its
-//    plan for the next job of the same size unless another job waits for memory.
+//  * auxiliary lanes (threads, one stream each) run sum-check jobs (the words are hashed
+//    on the host, the serial initial digest, while the device rounds run) and fold-chain
+//    jobs.  An idle lane keeps its plan and buffers for the next job of the same shape
+//    unless another job waits for memory.
 //  * admission: a job is admitted when the bytes of live device state (cached provers,
 //    sum-check plans) plus its own need fit the budget; idle cached state is evicted
 //    first when it does not.  A job larger than the whole budget fails with
@@ -32,7 +33,7 @@ using namespace azk::prv;
 
 namespace {
 
-enum { KIND_COMMIT = 0, KIND_SUMCHECK = 1 };
+enum { KIND_COMMIT = 0, KIND_SUMCHECK = 1, KIND_FOLD = 2 };
 
 struct SJob {
     uint64_t ticket = 0;
@@ -40,6 +41,8 @@ struct SJob {
     azk_circle_prover_desc desc{};
     const void *input = nullptr;
     size_t count = 0;  // sum-check words
+    uint32_t log_n = 0, rounds = 0;   // sum-check and fold shapes
+    std::vector<uint32_t> challenges;  // fold: 4 words per round, copied at submit
     size_t need = 0;   // device bytes of the state it needs
     azk_status status = AZK_OK;
     std::string err;
@@ -57,21 +60,46 @@ struct ShapeKey {
 };
 
 size_t sumcheck_need(uint32_t log_n) { return ((size_t)4 << log_n) * 7 / 4 + 64 * 1024; }
+// fold chain plan: FFT + fold twiddles, two ping-pong layers, input and output buffers
+size_t fold_need(uint32_t log_n, uint32_t rounds)
+{
+    const size_t n = (size_t)1 << log_n;
+    return 8 * n + 16 * n + 16 * n + 16 * (n >> rounds) + 64 * 1024;
+}
+
+// The device state an auxiliary lane keeps between jobs of the same shape.
+struct AuxCache {
+    uint32_t kind = ~0u, log_n = 0, rounds = 0;
+    size_t bytes = 0;
+    azk_sumcheck_plan *sc = nullptr;
+    azk_fold_plan *fold = nullptr;
+    azk_buffer *in = nullptr, *out = nullptr;
+    bool holds(const SJob *j) const { return kind == j->kind && log_n == j->log_n && rounds == j->rounds; }
+    bool empty() const { return kind == ~0u; }
+    void release()
+    {
+        if (sc) azk_sumcheck_plan_destroy(sc);
+        if (fold) azk_fold_plan_destroy(fold);
+        if (in) azk_buffer_free(in);
+        if (out) azk_buffer_free(out);
+        *this = AuxCache{};
+    }
+};
 
 } // namespace
 
 struct azk_server {
     azk_server_desc d{};
     azk_context *ctx = nullptr;               // commit lane
-    std::vector<azk_context *> sc_ctx;        // one per sum-check lane
+    std::vector<azk_context *> aux_ctx;        // one per sum-check lane
     hipStream_t upload = nullptr;
     std::mutex mu;
-    std::condition_variable cv_commit, cv_sc, cv_done;
-    std::deque<SJob *> pend_commit, pend_sc;
+    std::condition_variable cv_commit, cv_aux, cv_done;
+    std::deque<SJob *> pend_commit, pend_aux;
     std::map<uint64_t, std::unique_ptr<SJob>> jobs;
     uint64_t next_ticket = 1;
     size_t budget = 0, held = 0;  // held: bytes of live provers and plans
-    size_t sc_waiting = 0;        // bytes a blocked sum-check job needs (eviction request)
+    size_t aux_waiting = 0;        // bytes a blocked sum-check job needs (eviction request)
     bool commit_blocked = false;  // the head commit waits for memory (idle plans are released)
     bool stop = false;
     // commit lane state (commit thread only, except `held` under mu)
@@ -79,7 +107,7 @@ struct azk_server {
     std::deque<SJob *> ready;  // admitted, upload enqueued
     Runner runner;
     std::thread commit_thr;
-    std::vector<std::thread> sc_thr;
+    std::vector<std::thread> aux_thr;
 };
 
 namespace {
@@ -205,19 +233,19 @@ void commit_lane(azk_server *s)
     for (;;) {
         {
             std::unique_lock<std::mutex> g(s->mu);
-            s->cv_commit.wait(g, [&] { return s->stop || !s->pend_commit.empty() || !s->ready.empty() || s->sc_waiting; });
+            s->cv_commit.wait(g, [&] { return s->stop || !s->pend_commit.empty() || !s->ready.empty() || s->aux_waiting; });
             if (s->stop && s->pend_commit.empty() && s->ready.empty()) break;
         }
         admit_commits(s);
         size_t sc_want;
         {
             std::lock_guard<std::mutex> g(s->mu);
-            sc_want = s->sc_waiting;
+            sc_want = s->aux_waiting;
         }
         if (sc_want) {
             evict_idle(s, sc_want);
             std::lock_guard<std::mutex> g(s->mu);
-            s->sc_waiting = 0;  // served as far as idle provers allow; the lane asks again if still blocked
+            s->aux_waiting = 0;  // served as far as idle provers allow; the lane asks again if still blocked
         }
         if (s->ready.empty()) {
             // Nothing admissible: free idle provers for the head job, or fail it if it can
@@ -242,7 +270,7 @@ void commit_lane(azk_server *s)
                 } else {
                     std::unique_lock<std::mutex> g(s->mu);  // sum-check lanes hold the memory
                     s->commit_blocked = true;
-                    s->cv_sc.notify_all();
+                    s->cv_aux.notify_all();
                     s->cv_commit.wait_for(g, std::chrono::milliseconds(1));
                 }
                 continue;
@@ -268,45 +296,87 @@ void commit_lane(azk_server *s)
     }
 }
 
-void sumcheck_lane(azk_server *s, size_t lane)
+// Runs one auxiliary job with the lane's cached state (built when the shape changes).
+azk_status run_aux(azk_server *s, size_t lane, AuxCache &c, SJob *j)
+{
+    azk_context *ctx = s->aux_ctx[lane];
+    azk_status st = AZK_OK;
+    if (!c.holds(j)) {
+        c.release();
+        c.kind = j->kind;
+        c.log_n = j->log_n;
+        c.rounds = j->rounds;
+        c.bytes = j->need;
+        if (j->kind == KIND_SUMCHECK) {
+            st = azk_sumcheck_plan_create(ctx, j->log_n, &c.sc);
+        } else {
+            st = azk_fold_plan_create(ctx, j->log_n, j->rounds, &c.fold);
+            if (!st) st = azk_buffer_alloc(ctx, (size_t)16 << j->log_n, &c.in);
+            if (!st) st = azk_buffer_alloc(ctx, (size_t)16 << (j->log_n - j->rounds), &c.out);
+        }
+        if (st) {
+            const std::string m = azk_last_error();
+            c.release();
+            return fail(st, "%s", m.c_str());
+        }
+    }
+    j->t_start = now_ns();
+    if (j->kind == KIND_SUMCHECK) {
+        j->result.resize(azk_sumcheck_proof_size(j->log_n));
+        size_t len = 0;
+        st = azk_sumcheck_prove(c.sc, (const uint32_t *)j->input, j->count, nullptr, j->result.data(), j->result.size(),
+                                &len, nullptr);
+        j->result.resize(st ? 0 : len);
+        return st;
+    }
+    const size_t ib = (size_t)16 << j->log_n, ob = (size_t)16 << (j->log_n - j->rounds);
+    st = azk_buffer_upload(c.in, 0, j->input, ib);
+    if (!st) st = azk_fold_execute_resident(c.fold, c.in, 0, j->challenges.data(), c.out, 0, nullptr);
+    if (!st) {
+        j->result.resize(ob);
+        st = azk_buffer_download(c.out, 0, j->result.data(), ob);
+    }
+    return st;
+}
+
+void aux_lane(azk_server *s, size_t lane)
 {
     (void)hipSetDevice(s->d.device);
-    azk_sumcheck_plan *plan = nullptr;
-    uint32_t plan_log = 0;
+    AuxCache cache;
     for (;;) {
         SJob *j = nullptr;
         {
             std::unique_lock<std::mutex> g(s->mu);
             for (;;) {
-                if (!s->pend_sc.empty()) {
-                    SJob *c = s->pend_sc.front();
-                    const uint32_t log_n = ilog2(c->count);
-                    const size_t freed = (plan && plan_log != log_n) ? sumcheck_need(plan_log) : 0;
-                    const size_t add = (plan && plan_log == log_n) ? 0 : c->need;
+                if (!s->pend_aux.empty()) {
+                    SJob *c = s->pend_aux.front();
+                    const bool same = cache.holds(c);
+                    const size_t freed = (!cache.empty() && !same) ? cache.bytes : 0;
+                    const size_t add = same ? 0 : c->need;
                     if (s->held - freed + add <= s->budget || c->need > s->budget) {
                         j = c;
-                        s->pend_sc.pop_front();
-                        if (s->sc_waiting == c->need) s->sc_waiting = 0;
+                        s->pend_aux.pop_front();
+                        if (s->aux_waiting == c->need) s->aux_waiting = 0;
                         if (c->need <= s->budget) s->held = s->held - freed + add;
                         break;
                     }
-                    s->sc_waiting = c->need;  // ask the commit lane to evict idle provers
+                    s->aux_waiting = c->need;  // ask the commit lane to evict idle provers
                     s->cv_commit.notify_one();
-                    s->cv_sc.wait_for(g, std::chrono::milliseconds(1));
+                    s->cv_aux.wait_for(g, std::chrono::milliseconds(1));
                     continue;
                 }
                 if (s->stop) break;
-                if (plan && (s->commit_blocked || s->sc_waiting)) {
-                    // idle while another job waits for memory: give the plan's bytes back
+                if (!cache.empty() && (s->commit_blocked || s->aux_waiting)) {
+                    // idle while another job waits for memory: give the cached bytes back
+                    const size_t b = cache.bytes;
                     g.unlock();
-                    azk_sumcheck_plan_destroy(plan);
-                    plan = nullptr;
+                    cache.release();
                     g.lock();
-                    s->held -= sumcheck_need(plan_log);
+                    s->held -= b;
                     continue;
                 }
-                if (plan) s->cv_sc.wait_for(g, std::chrono::milliseconds(2));
-                else s->cv_sc.wait(g);
+                if (!cache.empty()) s->cv_aux.wait_for(g, std::chrono::milliseconds(2));
+                else s->cv_aux.wait(g);
             }
         }
         if (!j) break;
@@ -314,39 +384,22 @@ void sumcheck_lane(azk_server *s, size_t lane)
             finish(s, j, AZK_ERR_ALLOCATION, "job exceeds the server's HBM budget");
             continue;
         }
-        const uint32_t log_n = ilog2(j->count);
         j->t_admit = now_ns();
-        if (plan && plan_log != log_n) {
-            azk_sumcheck_plan_destroy(plan);
-            plan = nullptr;
-        }
-        azk_status st = AZK_OK;
-        if (!plan) {
-            st = azk_sumcheck_plan_create(s->sc_ctx[lane], log_n, &plan);
-            if (st) {
-                plan = nullptr;
-                std::lock_guard<std::mutex> g(s->mu);
-                s->held -= j->need;
-            } else {
-                plan_log = log_n;
-            }
-        }
-        if (!st) {
-            j->t_start = now_ns();
-            j->result.resize(azk_sumcheck_proof_size(log_n));
-            size_t len = 0;
-            st = azk_sumcheck_prove(plan, (const uint32_t *)j->input, j->count, nullptr, j->result.data(),
-                                    j->result.size(), &len, nullptr);
-            j->result.resize(st ? 0 : len);
+        const bool had = cache.holds(j);
+        azk_status st = run_aux(s, lane, cache, j);
+        if (st && !had && cache.empty()) {  // the cache was not built: its reservation goes back
+            std::lock_guard<std::mutex> g(s->mu);
+            s->held -= j->need;
         }
         j->batch_jobs = 1;
         std::string m = st ? azk_last_error() : "";
         finish(s, j, st, m.c_str());
     }
-    if (plan) {
-        azk_sumcheck_plan_destroy(plan);
+    if (!cache.empty()) {
+        const size_t b = cache.bytes;
+        cache.release();
         std::lock_guard<std::mutex> g(s->mu);
-        s->held -= sumcheck_need(plan_log);
+        s->held -= b;
     }
 }
 
@@ -376,8 +429,8 @@ azk_status azk_server_create(const azk_server_desc *desc, azk_server **out)
     auto *s = new azk_server();
     s->d = *desc;
     if (!s->d.max_batch) s->d.max_batch = 64;
-    if (!s->d.sumcheck_lanes) s->d.sumcheck_lanes = 4;
-    if (s->d.sumcheck_lanes > 16) s->d.sumcheck_lanes = 16;
+    if (!s->d.aux_lanes) s->d.aux_lanes = 4;
+    if (s->d.aux_lanes > 16) s->d.aux_lanes = 16;
     azk_status st = azk_context_create(s->d.device, &s->ctx);
     if (st) { delete s; return st; }
     s->budget = s->d.hbm_budget;
@@ -386,21 +439,21 @@ azk_status azk_server_create(const azk_server_desc *desc, azk_server **out)
         if (hipMemGetInfo(&free_b, &total) != hipSuccess) free_b = (size_t)8 << 30;
         s->budget = free_b / 2;
     }
-    for (uint32_t k = 0; k < s->d.sumcheck_lanes && !st; ++k) {
+    for (uint32_t k = 0; k < s->d.aux_lanes && !st; ++k) {
         azk_context *c = nullptr;
         st = azk_context_create(s->d.device, &c);
-        if (!st) s->sc_ctx.push_back(c);
+        if (!st) s->aux_ctx.push_back(c);
     }
     if (!st && hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) != hipSuccess)
         st = fail(AZK_ERR_ALLOCATION, "hipStreamCreate failed");
     if (st) {
-        for (auto *c : s->sc_ctx) azk_context_destroy(c);
+        for (auto *c : s->aux_ctx) azk_context_destroy(c);
         azk_context_destroy(s->ctx);
         delete s;
         return st;
     }
     s->commit_thr = std::thread(commit_lane, s);
-    for (size_t k = 0; k < s->sc_ctx.size(); ++k) s->sc_thr.emplace_back(sumcheck_lane, s, k);
+    for (size_t k = 0; k < s->aux_ctx.size(); ++k) s->aux_thr.emplace_back(aux_lane, s, k);
     *out = s;
     return AZK_OK;
 }
@@ -413,16 +466,16 @@ void azk_server_destroy(azk_server *s)
         s->stop = true;
     }
     s->cv_commit.notify_all();
-    s->cv_sc.notify_all();
+    s->cv_aux.notify_all();
     s->commit_thr.join();
-    for (auto &t : s->sc_thr) t.join();
+    for (auto &t : s->aux_thr) t.join();
     (void)hipSetDevice(s->d.device);
     (void)hipStreamSynchronize(s->upload);
     for (auto &kv : s->idle) azk_circle_prover_destroy(kv.second);
     for (auto &kv : s->jobs)
         if (kv.second->uploaded) (void)hipEventDestroy(kv.second->uploaded);
     (void)hipStreamDestroy(s->upload);
-    for (auto *c : s->sc_ctx) azk_context_destroy(c);
+    for (auto *c : s->aux_ctx) azk_context_destroy(c);
     azk_context *ctx = s->ctx;
     delete s;  // the Runner's staging and streams go first
     azk_context_destroy(ctx);
@@ -463,17 +516,47 @@ azk_status azk_server_submit_sumcheck(azk_server *s, const uint32_t *evals, size
     j->kind = KIND_SUMCHECK;
     j->input = evals;
     j->count = count;
-    j->need = sumcheck_need(ilog2(count));
+    j->log_n = ilog2(count);
+    j->need = sumcheck_need(j->log_n);
     j->t_submit = now_ns();
     {
         std::lock_guard<std::mutex> g(s->mu);
         if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
         j->ticket = s->next_ticket++;
         *ticket = j->ticket;
-        s->pend_sc.push_back(j.get());
+        s->pend_aux.push_back(j.get());
         s->jobs.emplace(j->ticket, std::move(j));
     }
-    s->cv_sc.notify_one();
+    s->cv_aux.notify_one();
+    return AZK_OK;
+}
+
+azk_status azk_server_submit_fold(azk_server *s, uint32_t log_n, uint32_t rounds, const void *evals, size_t bytes,
+                                  const uint32_t *challenges, uint64_t *ticket)
+{
+    CHECK_ARG(s && evals && challenges && ticket);
+    if (log_n < 1 || log_n > 28 || rounds < 1 || rounds > log_n || rounds > 30)
+        return fail(AZK_ERR_INVALID_LAYOUT, "fold chain log_n %u rounds %u", log_n, rounds);
+    if (bytes < ((size_t)16 << log_n)) return fail(AZK_ERR_INVALID_LAYOUT, "evaluation bytes %zu < %zu", bytes, (size_t)16 << log_n);
+    for (uint32_t i = 0; i < 4 * rounds; ++i)
+        if (challenges[i] >= kP) return fail(AZK_ERR_INVALID_LAYOUT, "non-canonical challenge limb");
+    auto j = std::make_unique<SJob>();
+    j->kind = KIND_FOLD;
+    j->input = evals;
+    j->log_n = log_n;
+    j->rounds = rounds;
+    j->challenges.assign(challenges, challenges + 4 * rounds);
+    j->need = fold_need(log_n, rounds);
+    j->t_submit = now_ns();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
+        j->ticket = s->next_ticket++;
+        *ticket = j->ticket;
+        s->pend_aux.push_back(j.get());
+        s->jobs.emplace(j->ticket, std::move(j));
+    }
+    s->cv_aux.notify_one();
     return AZK_OK;
 }
 

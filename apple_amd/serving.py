@@ -1,11 +1,13 @@
-"""Serving harness: a seeded Poisson stream of Circle PCS commits and M31 multilinear
-sum-checks driven through the C ABI server (azk_server_*), and the same stream run as
-plain sequential calls for comparison.
+"""Serving harness: a seeded Poisson stream of Circle PCS commits, M31 multilinear
+sum-checks and FRI fold chains driven through the C ABI server (azk_server_*), and the
+same stream run as plain sequential calls for comparison.
 
 Workload (per job, seeded SplitMix64): commits with log N uniform in [12, 22], 10 FRI
 rounds (fewer when log N is smaller), security (log blowup 2, 32 queries, folding 1, no
 grinding), public-input digest 0xa0 + i; sum-checks with n = 2^k, k uniform in [16, 22],
-canonical M31 words.  Half the jobs are of each kind.
+canonical M31 words; fold chains of 2^k QM31 evaluations, k uniform in [16, 22], 10 rounds
+with fixed challenges.  The kind shares are parameters (default 40% commits, 40%
+sum-checks, 20% folds).
 """
 from __future__ import annotations
 
@@ -24,8 +26,16 @@ def commit_rounds(log_n: int) -> int:
     return min(10, log_n - 1)
 
 
-def make_trace(seed: int, jobs: int, rate: float, commit_logs=(12, 22), sumcheck_logs=(16, 22),
-               commit_share: float = 0.5) -> list[tuple[float, str, int]]:
+FOLD_ROUNDS = 10
+
+
+def fold_challenges(rounds: int = FOLD_ROUNDS) -> np.ndarray:
+    """Canonical QM31 challenges, one per round (fixed; a fold job carries its own)."""
+    return inputs.random_m31(0xF01D, 4 * rounds).reshape(rounds, 4)
+
+
+def make_trace(seed: int, jobs: int, rate: float, commit_logs=(12, 22), sumcheck_logs=(16, 22), fold_logs=(16, 22),
+               commit_share: float = 0.4, fold_share: float = 0.2) -> list[tuple[float, str, int]]:
     """(arrival seconds, kind, log size) for `jobs` Poisson arrivals at `rate` jobs/s
     (rate <= 0: every job arrives at time 0)."""
     rng = np.random.default_rng(seed)
@@ -33,8 +43,11 @@ def make_trace(seed: int, jobs: int, rate: float, commit_logs=(12, 22), sumcheck
     for _ in range(jobs):
         if rate > 0:
             t += float(rng.exponential(1.0 / rate))
-        if rng.random() < commit_share:
+        u = rng.random()
+        if u < commit_share:
             out.append((t, "commit", int(rng.integers(commit_logs[0], commit_logs[1] + 1))))
+        elif u < commit_share + fold_share:
+            out.append((t, "fold", int(rng.integers(fold_logs[0], fold_logs[1] + 1))))
         else:
             out.append((t, "sumcheck", int(rng.integers(sumcheck_logs[0], sumcheck_logs[1] + 1))))
     return out
@@ -51,6 +64,8 @@ class Inputs:
                 continue
             if kind == "commit":
                 host = inputs.circle_fft_coefficients(seed + lg, lg)
+            elif kind == "fold":
+                host = inputs.random_m31(seed + 200 + lg, 4 << lg)
             else:
                 host = inputs.random_m31(seed + 100 + lg, 1 << lg)
             pin = azk.PinnedArray(host.shape, np.uint32)
@@ -102,6 +117,8 @@ def run_server(server: azk.Server, trace, data: Inputs, collect: bool = False) -
             if d is None:
                 d = descs[lg] = azk.prover_desc(lg, commit_rounds(lg), SECURITY, DIGEST)
             tk = server.submit_commit(d, data.get(kind, lg))
+        elif kind == "fold":
+            tk = server.submit_fold(lg, FOLD_ROUNDS, data.get(kind, lg), fold_challenges())
         else:
             tk = server.submit_sumcheck(data.get(kind, lg))
         with lock:
@@ -133,6 +150,11 @@ def service_times(ctx: azk.Context, trace, data: Inputs, repeats: int = 1) -> li
                 p = provers[lg] = azk.CircleCodewordPCSFRIProverV1(ctx, lg, commit_rounds(lg), SECURITY, DIGEST)
                 p.prove_fft_coefficients(data.get(kind, lg))  # warm: builds its program
                 f = (lambda p=p, a=data.get(kind, lg): p.prove_fft_coefficients(a))
+            elif kind == "fold":
+                fp = azk.CircleFRIFoldChainPlan(ctx, lg, FOLD_ROUNDS)
+                plans[("fold", lg)] = fp
+                fp.execute(data.get(kind, lg), fold_challenges())
+                f = (lambda fp=fp, a=data.get(kind, lg): fp.execute(a, fold_challenges()))
             else:
                 q = plans[lg] = azk.M31MultilinearSumcheckPlan(ctx, lg)
                 q.prove(data.get(kind, lg))

@@ -116,13 +116,15 @@ def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
 
 
 def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 96) -> dict:
-    """Poisson stream of commits (log N 12..22) and multilinear sum-checks (2^16..2^22),
-    half each, through the C ABI server; rate = 0.8 x the server's saturated throughput.
-    Compared with the same stream run as plain sequential calls (one job at a time)."""
+    """Poisson stream of commits (log N 12..22, 40%), multilinear sum-checks (2^16..2^22,
+    40%) and fold chains (2^16..2^22 QM31, 10 rounds, 20%) through the C ABI server;
+    rate = 0.8 x the server's saturated throughput.  Compared with the same stream run as
+    plain sequential calls (one job at a time)."""
     from apple_amd import serving
-    shapes = [(0.0, "commit", lg) for lg in range(12, 23)] + [(0.0, "sumcheck", lg) for lg in range(16, 23)]
+    shapes = ([(0.0, "commit", lg) for lg in range(12, 23)] + [(0.0, "sumcheck", lg) for lg in range(16, 23)] +
+              [(0.0, "fold", lg) for lg in range(16, 23)])
     data = serving.Inputs(shapes, seed=seed)
-    srv = azk.Server(0, max_batch=64, sumcheck_lanes=8)
+    srv = azk.Server(0, max_batch=64, aux_lanes=8)
     serving.run_server(srv, shapes + shapes, data)  # warm: provers and plans of every shape
     sat = serving.run_server(srv, serving.make_trace(seed, sat_jobs, rate=0), data)
     rate = 0.8 * sat["jobs_per_s"]
@@ -139,8 +141,8 @@ def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 96) -> di
     seq = serving.sequential_latencies(trace, service)
     seq_sat = len(trace) / sum(service)
     return {
-        "workload": "poisson mix: commits log N U{12..22} (10 rounds, 32 queries) + multilinear sum-checks 2^U{16..22}, "
-                    "50/50, seed %d" % seed,
+        "workload": f"poisson mix: 40% commits log N U{{12..22}} (10 rounds, 32 queries), 40% multilinear sum-checks "
+                    f"2^U{{16..22}}, 20% fold chains 2^U{{16..22}} x 10 rounds; seed {seed}",
         "jobs": jobs, "rate_jobs_per_s": rate, "saturated_jobs_per_s": sat["jobs_per_s"],
         "jobs_per_s": run["jobs_per_s"], "p50_ms": run["p50_ms"], "p99_ms": run["p99_ms"],
         "mean_commit_batch": run["mean_commit_batch"], "failed": run["failed"],

@@ -328,8 +328,8 @@ azk_status azk_sumcheck_verify(const uint8_t *proof, size_t len, int *accepted);
 
 /* ---- Serving form: jobs arrive at any time; no reference counterpart (the reference runs
  * one synchronous command buffer per call).  Commit jobs are run by continuous batching
- * (each device program covers every admitted commit, up to max_batch); sum-check jobs run
- * on their own lanes.  A job is admitted when the device state it needs fits the HBM
+ * (each device program covers every admitted commit, up to max_batch); sum-check and
+ * fold-chain jobs run on their own lanes.  A job is admitted when the device state it needs fits the HBM
  * budget next to what is live; its result is byte-identical to the plain call's.
  * Inputs are read in place: the caller keeps them unchanged until the job is done
  * (azk_server_wait).  Pinned inputs (azk_host_alloc) upload asynchronously. */
@@ -337,13 +337,13 @@ typedef struct azk_server azk_server;
 typedef struct azk_server_desc {
     int device;
     uint32_t max_batch;       /* commits per device program; 0 = 64 */
-    uint32_t sumcheck_lanes;  /* concurrent sum-check jobs, 1..16; 0 = 4 */
+    uint32_t aux_lanes;       /* concurrent sum-check / fold-chain jobs, 1..16; 0 = 4 */
     uint32_t reserved;
     uint64_t hbm_budget;      /* bytes of live job state; 0 = half the free device memory */
 } azk_server_desc;
 typedef struct azk_job_info {
     azk_status status;
-    uint32_t kind;            /* 0 = Circle PCS commit, 1 = multilinear sum-check */
+    uint32_t kind;            /* 0 = Circle PCS commit, 1 = multilinear sum-check, 2 = fold chain */
     uint32_t batch_jobs;      /* commits in the device program that ran the job */
     size_t result_bytes;      /* proof bytes (CirclePCSFRIProofCodecV1 / M31MultilinearSumcheckProofCodecV1) */
     uint64_t submit_ns, admit_ns, start_ns, done_ns;  /* azk_now_ns clock */
@@ -354,6 +354,11 @@ void azk_server_destroy(azk_server *server);
 azk_status azk_server_submit_commit(azk_server *server, const azk_circle_prover_desc *desc, const void *coeffs,
                                     size_t bytes, uint64_t *ticket);
 azk_status azk_server_submit_sumcheck(azk_server *server, const uint32_t *evals, size_t count, uint64_t *ticket);
+/* A CircleFRIFoldChainPlan job (CircleFRIFoldPlan.swift:132-280): 2^log_n QM31 evaluations
+ * folded `rounds` times with the given challenges (4 words per round, copied at submit);
+ * the result is the 2^(log_n - rounds) folded QM31 values. */
+azk_status azk_server_submit_fold(azk_server *server, uint32_t log_n, uint32_t rounds, const void *evals, size_t bytes,
+                                  const uint32_t *challenges, uint64_t *ticket);
 /* Blocks until the job is done; returns the job's status. */
 azk_status azk_server_wait(azk_server *server, uint64_t ticket, azk_job_info *info);
 /* Copies the result (out may be NULL) and forgets the ticket; waits if needed. */

@@ -13,9 +13,11 @@ def test_trace_is_seeded_and_poisson():
     gaps = np.diff([0.0] + [t for t, _, _ in a])
     assert 0.007 < gaps.mean() < 0.013          # mean inter-arrival 1 / rate
     kinds = [k for _, k, _ in a]
-    assert 150 < kinds.count("commit") < 250     # half of each kind
+    assert 120 < kinds.count("commit") < 200     # 40% commits, 40% sum-checks, 20% folds
+    assert 120 < kinds.count("sumcheck") < 200
+    assert 50 < kinds.count("fold") < 110
     assert all(12 <= lg <= 22 for _, k, lg in a if k == "commit")
-    assert all(16 <= lg <= 22 for _, k, lg in a if k == "sumcheck")
+    assert all(16 <= lg <= 22 for _, k, lg in a if k in ("sumcheck", "fold"))
     assert all(t == 0.0 for t, _, _ in serving.make_trace(5, 10, rate=0))
 
 
@@ -35,15 +37,19 @@ def ctx():
 
 
 def small_trace():
-    return [(0.0, "commit", 10), (0.0, "sumcheck", 12), (0.0, "commit", 13), (0.0, "commit", 10),
-            (0.0, "sumcheck", 9), (0.0, "commit", 11), (0.0, "sumcheck", 14), (0.0, "commit", 12),
-            (0.0, "commit", 14), (0.0, "sumcheck", 12), (0.0, "commit", 9), (0.0, "commit", 13)]
+    return [(0.0, "commit", 10), (0.0, "sumcheck", 12), (0.0, "commit", 13), (0.0, "fold", 12),
+            (0.0, "commit", 10), (0.0, "sumcheck", 9), (0.0, "commit", 11), (0.0, "sumcheck", 14),
+            (0.0, "fold", 14), (0.0, "commit", 12), (0.0, "commit", 14), (0.0, "sumcheck", 12),
+            (0.0, "fold", 12), (0.0, "commit", 9), (0.0, "commit", 13)]
 
 
 def alone(ctx, kind, lg, data):
     if kind == "commit":
         p = azk.CircleCodewordPCSFRIProverV1(ctx, lg, serving.commit_rounds(lg), serving.SECURITY, serving.DIGEST)
         return p.prove_fft_coefficients(data.get(kind, lg))
+    if kind == "fold":
+        f = azk.CircleFRIFoldChainPlan(ctx, lg, serving.FOLD_ROUNDS)
+        return f.execute(data.get(kind, lg), serving.fold_challenges()).tobytes()
     return azk.M31MultilinearSumcheckPlan(ctx, lg).prove(data.get(kind, lg))
 
 
@@ -51,7 +57,7 @@ def alone(ctx, kind, lg, data):
 def test_server_mix_matches_plain_calls(ctx):
     trace = small_trace()
     data = serving.Inputs(trace, seed=3)
-    srv = azk.Server(0, max_batch=64, sumcheck_lanes=2)
+    srv = azk.Server(0, max_batch=64, aux_lanes=2)
     print("server mix of", len(trace), "jobs", flush=True)
     r = serving.run_server(srv, trace, data, collect=True)
     srv.close()
@@ -59,13 +65,15 @@ def test_server_mix_matches_plain_calls(ctx):
     want = {(k, lg): alone(ctx, k, lg, data) for _, k, lg in trace}
     for kind, lg, info, proof in r["results"]:
         assert proof == want[(kind, lg)], (kind, lg)
-        assert info["kind"] == (0 if kind == "commit" else 1)
+        assert info["kind"] == {"commit": 0, "sumcheck": 1, "fold": 2}[kind]
     # one commit and one sum-check also against the oracle
     c = data.get("commit", 10)
     cw = O.circle_fft(c, 10)
     ref, _ = O.fri_prove(cw.reshape(-1, 4), 10, serving.SECURITY, serving.DIGEST, serving.commit_rounds(10))
     assert want[("commit", 10)] == ref
     assert want[("sumcheck", 12)] == O.sumcheck_multilinear_proof(data.get("sumcheck", 12))
+    folded = O.fold_chain(data.get("fold", 12).reshape(-1, 4), 12, serving.fold_challenges())
+    assert want[("fold", 12)] == np.asarray(folded, np.uint32).tobytes()
     assert r["mean_commit_batch"] > 1  # jobs submitted together share device programs
 
 
@@ -75,7 +83,7 @@ def test_server_admission_limits_concurrency(ctx):
     data = serving.Inputs(trace, seed=4)
     # room for one 2^14 commit's state (2.9 MiB) and the sum-check lanes, not two commits
     budget = 4 << 20
-    srv = azk.Server(0, max_batch=64, sumcheck_lanes=2, hbm_budget=budget)
+    srv = azk.Server(0, max_batch=64, aux_lanes=2, hbm_budget=budget)
     print("server tight budget", flush=True)
     r = serving.run_server(srv, trace, data, collect=True)
     srv.close()
