@@ -168,6 +168,10 @@ def load() -> C.CDLL:
         "azk_codeword_execute_resident": (st, [vp, vp, sz, vp, sz, sp]),
         "azk_codeword_execute": (st, [vp, vp, sz, vp, sz, sp]),
         "azk_codeword_read_twiddles": (st, [vp, vp, sz]),
+        "azk_witness_basis_plan_create": (st, [vp, u32, C.POINTER(vp)]),
+        "azk_witness_basis_plan_destroy": (None, [vp]),
+        "azk_witness_basis_execute_resident": (st, [vp, vp, sz, u32, vp, sz, u32, vp, sz, sp]),
+        "azk_witness_basis_execute": (st, [vp, vp, u32, vp, u32, vp, sz, sp]),
         "azk_fold_plan_create": (st, [vp, u32, u32, C.POINTER(vp)]),
         "azk_fold_plan_destroy": (None, [vp]),
         "azk_fold_execute_resident": (st, [vp, vp, sz, vp, vp, sz, sp]),
@@ -517,6 +521,39 @@ class CircleCodewordPlan(_Plan):
         out = np.zeros(max(n, 1), np.uint32)
         _ck(self.ctx._lib.azk_codeword_read_twiddles(self.h, _ptr(out), n))
         return out[:n]
+
+
+class CircleWitnessToFFTBasisPlanV1(_Plan):
+    """CircleWitnessFFTBasis.swift:230-517: x / y monomial witness columns (QM31 rows) ->
+    2^log_n interleaved Circle-FFT-basis coefficients (CircleCodeword.swift:100-127)."""
+    _destroy = "azk_witness_basis_plan_destroy"
+
+    def __init__(self, ctx: Context, log_n: int):
+        self.ctx, self.log_n = ctx, log_n
+        h = vp()
+        _ck(ctx._lib.azk_witness_basis_plan_create(ctx.h, log_n, C.byref(h)))
+        self.h = h
+        _track(self)
+        self.last_stats: dict = {}
+
+    def execute(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+        xa = np.ascontiguousarray(np.asarray(x, dtype=np.uint32).reshape(-1, 4))
+        ya = np.ascontiguousarray(np.asarray(y, dtype=np.uint32).reshape(-1, 4))
+        out = np.zeros(4 << self.log_n, np.uint32)
+        s = Stats()
+        _ck(self.ctx._lib.azk_witness_basis_execute(self.h, _ptr(xa) if len(xa) else None, len(xa),
+                                                     _ptr(ya) if len(ya) else None, len(ya), _ptr(out), out.size,
+                                                     C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return out
+
+    def execute_resident(self, x: Buffer | None, x_count: int, y: Buffer | None, y_count: int, out: Buffer,
+                         x_offset: int = 0, y_offset: int = 0, out_offset: int = 0) -> dict:
+        s = Stats()
+        _ck(self.ctx._lib.azk_witness_basis_execute_resident(
+            self.h, x.h if x is not None else None, x_offset, x_count, y.h if y is not None else None, y_offset,
+            y_count, out.h, out_offset, C.byref(s)))
+        return _stats_dict(s)
 
 
 class CircleFRIFoldChainPlan(_Plan):

@@ -167,6 +167,24 @@ azk_status azk_codeword_execute(azk_codeword_plan *plan, const void *coeffs, siz
 /* FFT twiddles (N - 1 words) as materialized on the device (readCodewordTwiddles :818-820). */
 azk_status azk_codeword_read_twiddles(azk_codeword_plan *plan, uint32_t *out, size_t words);
 
+/* ---- Monomial witness columns -> Circle-FFT-basis coefficients
+ * CircleWitnessToFFTBasisPlanV1(context:domain:) (CircleWitnessFFTBasis.swift:230-517) and
+ * CircleCodewordOracle.circleFFTCoefficients (CircleCodeword.swift:100-127): x / y monomial
+ * coefficients (QM31, counts <= N/2 each, not both zero, canonical) -> N interleaved
+ * line-basis coefficients (out[2i] = x-line i, out[2i+1] = y-line i), the codeword plan's
+ * input.  O(N log^2 N) on the device; the reference's dense matrix stops at N/2 = 4096.
+ * Non-canonical input limbs give AZK_ERR_INVALID_LAYOUT (validateResident). */
+typedef struct azk_basis_plan azk_basis_plan;
+azk_status azk_witness_basis_plan_create(azk_context *ctx, uint32_t log_n, azk_basis_plan **out);
+void azk_witness_basis_plan_destroy(azk_basis_plan *plan);
+/* executeResident(xWitnessCoefficientBuffer:...:outputCircleCoefficientOffset:) :320-428 */
+azk_status azk_witness_basis_execute_resident(azk_basis_plan *plan, const azk_buffer *x, size_t x_offset,
+                                              uint32_t x_count, const azk_buffer *y, size_t y_offset,
+                                              uint32_t y_count, azk_buffer *out, size_t out_offset, azk_stats *stats);
+/* host arrays: x_count / y_count QM31 (4 LE u32 each); out: 4N words */
+azk_status azk_witness_basis_execute(azk_basis_plan *plan, const uint32_t *x, uint32_t x_count, const uint32_t *y,
+                                     uint32_t y_count, uint32_t *out, size_t out_words, azk_stats *stats);
+
 /* ---- Circle FRI fold chain with explicit challenges
  * CircleFRIFoldPlan / CircleFRIFoldChainPlan (CircleFRIFoldPlan.swift:5-280). */
 typedef struct azk_fold_plan azk_fold_plan;

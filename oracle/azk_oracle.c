@@ -598,6 +598,30 @@ int ora_circle_evaluate_direct(const uint32_t *x, uint32_t nx, const uint32_t *y
     return 0;
 }
 
+/* The same Horner evaluation at `count` selected storage indices only (a
+ * size-independent check of a large codeword).                    :27-70,173-185 */
+int ora_circle_evaluate_at(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny, uint32_t log_n,
+                           const uint32_t *idx, size_t count, uint32_t *out)
+{
+    if (log_n < 1 || log_n > 24) return -1;
+    size_t n = (size_t)1 << log_n;
+    if (nx > n / 2 || ny > n / 2) return -1;
+    for (size_t i = 0; i < count; ++i)
+        if (idx[i] >= n) return -1;
+    uint32_t *pts = (uint32_t *)malloc(n * 2 * sizeof(uint32_t));
+    ora_circle_domain_points(log_n, pts);
+    for (size_t i = 0; i < count; ++i) {
+        uint32_t px = pts[2 * (size_t)idx[i]], py = pts[2 * (size_t)idx[i] + 1];
+        qm31 ax = {{0, 0, 0, 0}}, ay = {{0, 0, 0, 0}};
+        for (uint32_t k = nx; k-- > 0;) ax = qm_add(qm_scale(ax, px), qm_load(x + 4 * (size_t)k));
+        for (uint32_t k = ny; k-- > 0;) ay = qm_add(qm_scale(ay, px), qm_load(y + 4 * (size_t)k));
+        qm31 v = qm_add(ax, qm_scale(ay, py));
+        memcpy(out + 4 * i, v.v, 16);
+    }
+    free(pts);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* FRI fold                                            QM31FRIFold.swift:11-44 */
 /* ------------------------------------------------------------------------- */

@@ -63,6 +63,8 @@ int ora_circle_basis_from_monomial(const uint32_t *x, uint32_t nx, const uint32_
                                    uint32_t log_n, uint32_t *out);
 int ora_circle_evaluate_direct(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny,
                                uint32_t log_n, uint32_t *out);
+int ora_circle_evaluate_at(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny, uint32_t log_n,
+                           const uint32_t *idx, size_t count, uint32_t *out);
 
 /* ---- fold ---- */
 int ora_fold_layer(const uint32_t *in, size_t n, const uint32_t *tw, const uint32_t ch[4], uint32_t *out);

@@ -78,6 +78,7 @@ def _declare(l: C.CDLL) -> None:
         "ora_circle_fft": (C.c_int, [u32p, C.c_uint32, u32p]),
         "ora_circle_basis_from_monomial": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p]),
         "ora_circle_evaluate_direct": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p]),
+        "ora_circle_evaluate_at": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, u32p, sz, u32p]),
         "ora_fold_layer": (C.c_int, [u32p, sz, u32p, u32p, u32p]),
         "ora_circle_fold_chain": (C.c_int, [u32p, C.c_uint32, u32p, C.c_uint32, u32p]),
         "ora_merkle_root": (C.c_int, [u8p, sz, sz, sz, u8p]),
@@ -269,6 +270,16 @@ def evaluate_direct(x, y, log_n: int) -> np.ndarray:
     o = np.zeros(4 << log_n, np.uint32)
     _check(lib().ora_circle_evaluate_direct(_p32(xa), len(x), _p32(ya), len(y), log_n, _p32(o)), "direct eval")
     return o
+
+
+def evaluate_at(x, y, log_n: int, idx) -> np.ndarray:
+    """P(x) + y Q(x) at the given storage indices (Horner per point)."""
+    xa, ya = _qm_arr(x), _qm_arr(y)
+    ia = np.ascontiguousarray(idx, dtype=np.uint32)
+    o = np.zeros(4 * len(ia), np.uint32)
+    _check(lib().ora_circle_evaluate_at(_p32(xa), len(x), _p32(ya), len(y), log_n, _p32(ia), len(ia), _p32(o)),
+           "evaluate at")
+    return o.reshape(-1, 4)
 
 
 def fold_layer(vals: np.ndarray, tw: np.ndarray, ch) -> np.ndarray:
