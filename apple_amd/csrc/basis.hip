@@ -86,13 +86,21 @@ __device__ __forceinline__ uint4 q_scale(uint4 x, uint32_t s)
 // both CM31 lanes of x times w
 __device__ __forceinline__ uint4 q_cmul(uint4 x, uint2 w)
 {
+    const uint32_t w1n = m31_neg(w.y);
     uint4 r;
-    cm_mul(x.x, x.y, w.x, w.y, r.x, r.y);
-    cm_mul(x.z, x.w, w.x, w.y, r.z, r.w);
+    cm_mul_lazy(x.x, x.y, w.x, w.y, w1n, r.x, r.y);
+    cm_mul_lazy(x.z, x.w, w.x, w.y, w1n, r.z, r.w);
     return r;
 }
-// ... times conj(w) = w^-1 for a root of unity
-__device__ __forceinline__ uint4 q_cmul_conj(uint4 x, uint2 w) { return q_cmul(x, make_uint2(w.x, m31_neg(w.y))); }
+// ... times conj(w) = w^-1 for a root of unity: conj(w) = w0 + (p - w1) i
+__device__ __forceinline__ uint4 q_cmul_conj(uint4 x, uint2 w)
+{
+    const uint32_t w1n = m31_neg(w.y);
+    uint4 r;
+    cm_mul_lazy(x.x, x.y, w.x, w1n, w.y, r.x, r.y);
+    cm_mul_lazy(x.z, x.w, w.x, w1n, w.y, r.z, r.w);
+    return r;
+}
 
 // x * 2^e for canonical x, 0 <= e < 31: a rotation of the 31-bit value (2^31 = 1 mod p).
 __device__ __forceinline__ uint32_t m31_rotl(uint32_t x, uint32_t e)

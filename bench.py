@@ -153,7 +153,7 @@ def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 96) -> di
     }
 
 
-FFT_BUTTERFLY_OPS = 74  # VALU instructions of one QM31 butterfly in this build's tile loop (gfx950 assembly)
+FFT_BUTTERFLY_OPS = 70  # VALU instructions of one QM31 butterfly in this build's tile loop (gfx950 assembly)
 
 
 def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
