@@ -50,6 +50,9 @@ constexpr uint32_t kLogPassTile = 11;  // global pass tile (rows x columns)
 constexpr uint32_t kLogMinCols = 3;    // >= 8 contiguous QM31 (128 B) per tile row
 constexpr uint32_t kDirectMax = 32;    // shifts of length <= this are direct sums
 constexpr uint32_t kMaxLogN = 26;
+// the fused bottom pass (stages 2..0) sits below the gathered first stage: chunks of >= 16
+// and tail transforms of >= 16 (H > kDirectMax >= 4)
+static_assert(kLogChunk >= 4 && kDirectMax >= 4 && kLogTail < kLogChunk, "stage layout");
 
 enum : uint32_t { kPrepare = 1, kFinalize = 2, kForwardOnly = 4 };
 
@@ -149,6 +152,203 @@ __device__ __forceinline__ void scatter_half(const BsArgs &a, uint32_t t, uint32
     a.dst[((size_t)f << a.log_n) + ((size_t)bs << (a.log_s - 1)) + j] = q_scale(g, a.ifact[j]);
 }
 
+// ---------------------------------------------------------------- radix-2^K stage groups
+// A DIF group covers K consecutive stages, half-sizes (R/2) q, ..., 2q, q (R = 2^K), on
+// blocks of R q positions: for j < q it takes x_m = X[j + m q], runs the R-point DFT with
+// its internal factors (+-1, +-i and powers of w8; for the generator (2, 1268011823)
+// w4 = -i and w8 = 2^15 (1 - i)), and multiplies slot m by w_{Rq}^(j rev_K(m)).  That is
+// the same map as K radix-2 DIF stages, with R - 1 general products per R elements instead
+// of K R / 2 and one LDS round trip instead of K.  The DIT group is its inverse: conjugate
+// external factors first, then the internal stages in reverse order.
+
+__device__ __forceinline__ void cm_negi(uint32_t &a, uint32_t &b) { const uint32_t t = a; a = b; b = m31_neg(t); }  // * -i
+__device__ __forceinline__ void cm_posi(uint32_t &a, uint32_t &b) { const uint32_t t = a; a = m31_neg(b); b = t; }  // * i
+// * w8 = 2^15 (1 - i):        2^15 ((a + b) + (b - a) i)
+__device__ __forceinline__ void cm_w8(uint32_t &a, uint32_t &b)
+{
+    const uint32_t s = m31_add(a, b), d = m31_sub(b, a);
+    a = m31_rotl(s, 15);
+    b = m31_rotl(d, 15);
+}
+// * w8^3 = 2^15 (-1 - i):     2^15 ((b - a) - (a + b) i)
+__device__ __forceinline__ void cm_w8_3(uint32_t &a, uint32_t &b)
+{
+    const uint32_t s = m31_add(a, b), d = m31_sub(b, a);
+    a = m31_rotl(d, 15);
+    b = m31_rotl(m31_neg(s), 15);
+}
+// * conj(w8) = 2^15 (1 + i):  2^15 ((a - b) + (a + b) i)
+__device__ __forceinline__ void cm_w8c(uint32_t &a, uint32_t &b)
+{
+    const uint32_t s = m31_add(a, b), d = m31_sub(a, b);
+    a = m31_rotl(d, 15);
+    b = m31_rotl(s, 15);
+}
+// * conj(w8^3) = 2^15 (-1 + i): 2^15 (-(a + b) + (a - b) i)
+__device__ __forceinline__ void cm_w8c_3(uint32_t &a, uint32_t &b)
+{
+    const uint32_t s = m31_add(a, b), d = m31_sub(a, b);
+    a = m31_rotl(m31_neg(s), 15);
+    b = m31_rotl(d, 15);
+}
+#define AZK_LANES(op, v) do { op((v).x, (v).y); op((v).z, (v).w); } while (0)
+
+// x * w_{2h}^e (or its conjugate) for e < 2h; the table holds w_{2h}^t for t < h and
+// w_{2h}^(t + h) = -w_{2h}^t.
+template <bool CONJ>
+__device__ __forceinline__ uint4 q_mul_pow(uint4 x, const uint2 *tw, uint32_t h, uint32_t e)
+{
+    const bool neg = e >= h;
+    const uint2 w = tw[h + (neg ? e - h : e)];
+    const uint32_t w0 = neg ? m31_neg(w.x) : w.x, w1 = neg ? m31_neg(w.y) : w.y;
+    return CONJ ? q_cmul_conj(x, make_uint2(w0, w1)) : q_cmul(x, make_uint2(w0, w1));
+}
+
+__device__ __forceinline__ void bfly(uint4 &a, uint4 &c)
+{
+    const uint4 s = q_add(a, c), d = q_sub(a, c);
+    a = s;
+    c = d;
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t rev_slot(int m)
+{
+    return K == 3 ? ((m & 1) << 2 | (m & 2) | (m >> 2)) : K == 2 ? ((m & 1) << 1 | (m >> 1)) : (uint32_t)m;
+}
+
+// K DIF stages on the R slots; EXT = false when q = 1 (every external factor is 1).
+template <int K, bool EXT>
+__device__ __forceinline__ void dif_core(uint4 (&x)[1 << K], uint32_t j, uint32_t log_q, const uint2 *tw)
+{
+    constexpr int R = 1 << K;
+    if constexpr (K == 3) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) bfly(x[m], x[m + 4]);
+        AZK_LANES(cm_w8, x[5]);
+        AZK_LANES(cm_negi, x[6]);
+        AZK_LANES(cm_w8_3, x[7]);
+    }
+    if constexpr (K >= 2) {
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+            if (!(m & 2)) {
+                bfly(x[m], x[m + 2]);
+                if (m & 1) AZK_LANES(cm_negi, x[m + 2]);
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < R; m += 2) bfly(x[m], x[m + 1]);
+    if constexpr (EXT) {
+        const uint32_t h = 1u << (log_q + K - 1);
+#pragma unroll
+        for (int m = 1; m < R; ++m) x[m] = q_mul_pow<false>(x[m], tw, h, j * rev_slot<K>(m));
+    }
+}
+
+template <int K, bool EXT>
+__device__ __forceinline__ void dit_core(uint4 (&x)[1 << K], uint32_t j, uint32_t log_q, const uint2 *tw)
+{
+    constexpr int R = 1 << K;
+    if constexpr (EXT) {
+        const uint32_t h = 1u << (log_q + K - 1);
+#pragma unroll
+        for (int m = 1; m < R; ++m) x[m] = q_mul_pow<true>(x[m], tw, h, j * rev_slot<K>(m));
+    }
+#pragma unroll
+    for (int m = 0; m < R; m += 2) bfly(x[m], x[m + 1]);
+    if constexpr (K >= 2) {
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+            if (!(m & 2)) {
+                if (m & 1) AZK_LANES(cm_posi, x[m + 2]);
+                bfly(x[m], x[m + 2]);
+            }
+    }
+    if constexpr (K == 3) {
+        AZK_LANES(cm_w8c, x[5]);
+        AZK_LANES(cm_posi, x[6]);
+        AZK_LANES(cm_w8c_3, x[7]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) bfly(x[m], x[m + 4]);
+    }
+}
+
+// One group of K stages over n LDS elements: slots at LDS distance 2^l_lds, position q =
+// 2^log_q; jpos(i0) = (position of LDS element i0) mod q.
+template <int K, bool INV, class J>
+__device__ __forceinline__ void lds_group(uint4 *sh, uint32_t n, uint32_t l_lds, uint32_t log_q, const uint2 *tw, J &&jpos)
+{
+    for (uint32_t g = threadIdx.x; g < (n >> K); g += kBsThreads) {
+        const uint32_t i0 = ((g >> l_lds) << (l_lds + K)) | (g & ((1u << l_lds) - 1));
+        uint4 x[1 << K];
+#pragma unroll
+        for (int m = 0; m < (1 << K); ++m) x[m] = sh[i0 + ((uint32_t)m << l_lds)];
+        const uint32_t j = jpos(i0);
+        if constexpr (INV) dit_core<K, true>(x, j, log_q, tw);
+        else dif_core<K, true>(x, j, log_q, tw);
+#pragma unroll
+        for (int m = 0; m < (1 << K); ++m) sh[i0 + ((uint32_t)m << l_lds)] = x[m];
+    }
+    __syncthreads();
+}
+
+template <bool INV, class J>
+__device__ __forceinline__ void lds_group_k(uint32_t k, uint4 *sh, uint32_t n, uint32_t l_lds, uint32_t log_q,
+                                            const uint2 *tw, J &&jpos)
+{
+    if (k == 3) lds_group<3, INV>(sh, n, l_lds, log_q, tw, jpos);
+    else if (k == 2) lds_group<2, INV>(sh, n, l_lds, log_q, tw, jpos);
+    else lds_group<1, INV>(sh, n, l_lds, log_q, tw, jpos);
+}
+
+// DIF stages (position logs) hi .. lo over n LDS elements, top group takes the remainder;
+// LDS log distance of position log l is l + d_off.
+template <class J>
+__device__ __forceinline__ void dif_stages(uint4 *sh, uint32_t n, int hi, int lo, int d_off, const uint2 *tw, J &&jpos_of)
+{
+    if (hi < lo) return;
+    const uint32_t count = hi - lo + 1, rem = count % 3;
+    int top = hi;
+    if (rem) {
+        const uint32_t lq = top - rem + 1;
+        lds_group_k<false>(rem, sh, n, lq + d_off, lq, tw, jpos_of(lq));
+        top -= rem;
+    }
+    for (; top >= lo + 2; top -= 3) lds_group<3, false>(sh, n, top - 2 + d_off, top - 2, tw, jpos_of(top - 2));
+}
+
+// DIT stages lo .. hi, bottom-up, the remainder last.
+template <class J>
+__device__ __forceinline__ void dit_stages(uint4 *sh, uint32_t n, int lo, int hi, int d_off, const uint2 *tw, J &&jpos_of)
+{
+    if (hi < lo) return;
+    const uint32_t count = hi - lo + 1, rem = count % 3;
+    int bot = lo;
+    for (; bot + 2 <= hi - (int)rem; bot += 3) lds_group<3, true>(sh, n, bot + d_off, bot, tw, jpos_of(bot));
+    if (rem) lds_group_k<true>(rem, sh, n, bot + d_off, bot, tw, jpos_of(bot));
+}
+
+// Stages 2, 1, 0 (DIF) of transforms laid out contiguously, the pointwise product with
+// K (kv(i) = K at LDS index i), and DIT stages 0, 1, 2: one register pass.
+template <class KV>
+__device__ __forceinline__ void fused_bottom(uint4 *sh, uint32_t n, KV &&kv)
+{
+    for (uint32_t g = threadIdx.x; g < (n >> 3); g += kBsThreads) {
+        const uint32_t i0 = g << 3;
+        uint4 x[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) x[m] = sh[i0 + m];
+        dif_core<3, false>(x, 0, 0, nullptr);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) x[m] = q_cmul(x[m], kv(i0 + m));
+        dit_core<3, false>(x, 0, 0, nullptr);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) sh[i0 + m] = x[m];
+    }
+    __syncthreads();
+}
+
 // Forward DIF passes over stages h = 2^lt .. 2^lb (lb >= kLogChunk) of transforms of size
 // 2^log_s: a tile is 2^(lt-lb+1) rows at stride 2^lb times 2^log_c contiguous columns.
 // kPrepare (lt = log_s - 1): rows of the upper half are the zero padding, so the first
@@ -164,7 +364,7 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_fwd(BsArgs a)
     const size_t base = ((size_t)(g >> log_groups) << (a.lt + 1)) + ((size_t)(g & ((1u << log_groups) - 1)) << a.log_c);
     const uint32_t j0 = (uint32_t)(base & ((1u << a.lb) - 1));
     uint4 *x = a.w + ((size_t)t << a.log_s);
-    uint32_t l_first = a.lt;
+    int l_first = (int)a.lt;
     if (a.mode & kPrepare) {
         // h = 2^lt = S/2: base < h (one block of 2h per transform), p = r * 2^lb + j0 + c < H
         const uint32_t half = tile >> 1, H = 1u << a.lt;
@@ -175,7 +375,7 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_fwd(BsArgs a)
             sh[e] = v;
             sh[e + half] = q_cmul(v, a.tw[H + p]);
         }
-        l_first = a.lt - 1;
+        l_first = (int)a.lt - 1;
     } else {
         for (uint32_t e = threadIdx.x; e < tile; e += kBsThreads) {
             const uint32_t r = e >> a.log_c, c = e & (C - 1);
@@ -183,19 +383,10 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_fwd(BsArgs a)
         }
     }
     __syncthreads();
-    for (int l = (int)l_first; l >= (int)a.lb; --l) {
-        const uint32_t lr = l - a.lb, h = 1u << l;
-        for (uint32_t u = threadIdx.x; u < tile / 2; u += kBsThreads) {
-            const uint32_t c = u & (C - 1), rr = u >> a.log_c;
-            const uint32_t r0 = ((rr >> lr) << (lr + 1)) | (rr & ((1u << lr) - 1));
-            const uint32_t i0 = (r0 << a.log_c) | c, i1 = i0 + (1u << (lr + a.log_c));
-            const uint32_t j = ((rr & ((1u << lr) - 1)) << a.lb) + j0 + c;
-            uint4 p = sh[i0], q = sh[i1];
-            sh[i0] = q_add(p, q);
-            sh[i1] = q_cmul(q_sub(p, q), a.tw[h + j]);
-        }
-        __syncthreads();
-    }
+    const uint32_t lb = a.lb, log_c = a.log_c;
+    dif_stages(sh, tile, l_first, (int)lb, (int)log_c - (int)lb, a.tw, [&](uint32_t lq) {
+        return [=](uint32_t i0) { return (((i0 >> log_c) & ((1u << (lq - lb)) - 1)) << lb) + j0 + (i0 & (C - 1)); };
+    });
     for (uint32_t e = threadIdx.x; e < tile; e += kBsThreads) {
         const uint32_t r = e >> a.log_c, c = e & (C - 1);
         x[base + ((size_t)r << a.lb) + c] = sh[e];
@@ -220,20 +411,11 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_inv(BsArgs a)
         sh[e] = x[base + ((size_t)r << a.lb) + c];
     }
     __syncthreads();
-    const uint32_t l_end = (a.mode & kFinalize) ? a.lt : a.lt + 1;
-    for (uint32_t l = a.lb; l < l_end; ++l) {
-        const uint32_t lr = l - a.lb, h = 1u << l;
-        for (uint32_t u = threadIdx.x; u < tile / 2; u += kBsThreads) {
-            const uint32_t c = u & (C - 1), rr = u >> a.log_c;
-            const uint32_t r0 = ((rr >> lr) << (lr + 1)) | (rr & ((1u << lr) - 1));
-            const uint32_t i0 = (r0 << a.log_c) | c, i1 = i0 + (1u << (lr + a.log_c));
-            const uint32_t j = ((rr & ((1u << lr) - 1)) << a.lb) + j0 + c;
-            uint4 p = sh[i0], q = q_cmul_conj(sh[i1], a.tw[h + j]);
-            sh[i0] = q_add(p, q);
-            sh[i1] = q_sub(p, q);
-        }
-        __syncthreads();
-    }
+    const int l_last = (a.mode & kFinalize) ? (int)a.lt - 1 : (int)a.lt;
+    const uint32_t lb = a.lb, log_c = a.log_c;
+    dit_stages(sh, tile, (int)lb, l_last, (int)log_c - (int)lb, a.tw, [&](uint32_t lq) {
+        return [=](uint32_t i0) { return (((i0 >> log_c) & ((1u << (lq - lb)) - 1)) << lb) + j0 + (i0 & (C - 1)); };
+    });
     if (a.mode & kFinalize) {
         const uint32_t half = tile >> 1, H = 1u << a.lt;
         for (uint32_t e = threadIdx.x; e < half; e += kBsThreads) {
@@ -274,34 +456,17 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_chunk(BsArgs a)
         for (uint32_t e = threadIdx.x; e < tc; e += kBsThreads) sh[e] = x[e];
     }
     __syncthreads();
-    for (int l = l_first; l >= 0; --l) {
-        const uint32_t h = 1u << l;
-        for (uint32_t u = threadIdx.x; u < tc / 2; u += kBsThreads) {
-            const uint32_t j = u & (h - 1), i0 = ((u >> l) << (l + 1)) | j;
-            uint4 p = sh[i0], q = sh[i0 + h];
-            sh[i0] = q_add(p, q);
-            sh[i0 + h] = q_cmul(q_sub(p, q), a.tw[h + j]);
-        }
-        __syncthreads();
-    }
+    auto jpos = [](uint32_t lq) { return [=](uint32_t i0) { return i0 & ((1u << lq) - 1); }; };
     if (a.mode & kForwardOnly) {
+        dif_stages(sh, tc, l_first, 0, 0, a.tw, jpos);
         for (uint32_t e = threadIdx.x; e < tc; e += kBsThreads) x[e] = sh[e];
         return;
     }
+    dif_stages(sh, tc, l_first, 3, 0, a.tw, jpos);
     const uint2 *k = a.kh + ((size_t)1 << a.log_s) + pbase;
-    for (uint32_t e = threadIdx.x; e < tc; e += kBsThreads) sh[e] = q_cmul(sh[e], k[e]);
-    __syncthreads();
-    const uint32_t l_end = (a.mode & kFinalize) ? log_tc - 1 : log_tc;
-    for (uint32_t l = 0; l < l_end; ++l) {
-        const uint32_t h = 1u << l;
-        for (uint32_t u = threadIdx.x; u < tc / 2; u += kBsThreads) {
-            const uint32_t j = u & (h - 1), i0 = ((u >> l) << (l + 1)) | j;
-            uint4 p = sh[i0], q = q_cmul_conj(sh[i0 + h], a.tw[h + j]);
-            sh[i0] = q_add(p, q);
-            sh[i0 + h] = q_sub(p, q);
-        }
-        __syncthreads();
-    }
+    fused_bottom(sh, tc, [&](uint32_t i) { return k[i]; });
+    const int l_last = (a.mode & kFinalize) ? (int)log_tc - 2 : (int)log_tc - 1;
+    dit_stages(sh, tc, 3, l_last, 0, a.tw, jpos);
     if (a.mode & kFinalize) {
         const uint32_t H = tc >> 1;
         for (uint32_t j = threadIdx.x; j < H; j += kBsThreads)
@@ -322,6 +487,7 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_tail(BsArgs a, uint4 *__restr
     const uint32_t f = blockIdx.x >> a.lvl, b = blockIdx.x & ((1u << a.lvl) - 1);
     for (uint32_t n = threadIdx.x; n < B; n += kBsThreads) D[n] = load_entry(a, f, b, n);
     __syncthreads();
+    auto jpos = [](uint32_t lq) { return [=](uint32_t i0) { return i0 & ((1u << lq) - 1); }; };
     for (uint32_t lb = a.log_s; lb >= 2; --lb) {  // sub-blocks of 2^lb, halves of H
         const uint32_t Bc = 1u << lb, H = Bc >> 1;
         if (H <= kDirectMax) {
@@ -350,7 +516,7 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_tail(BsArgs a, uint4 *__restr
             __syncthreads();
             continue;
         }
-        // transforms t' = 2 * sub-block + sigma of size Bc at W + t' Bc
+        // transforms t' = 2 * sub-block + sigma of size Bc at W + t' Bc (2B elements)
         for (uint32_t e = threadIdx.x; e < B; e += kBsThreads) {
             const uint32_t tp = e >> (lb - 1), k = e & (H - 1);
             const uint32_t sub = (tp >> 1) << lb, sigma = tp & 1;
@@ -359,29 +525,10 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_tail(BsArgs a, uint4 *__restr
             W[(tp << lb) + H + k] = q_cmul(v, a.tw[H + k]);
         }
         __syncthreads();
-        for (int l = (int)lb - 2; l >= 0; --l) {
-            const uint32_t h = 1u << l;
-            for (uint32_t u = threadIdx.x; u < B; u += kBsThreads) {
-                const uint32_t j = u & (h - 1), i0 = ((u >> l) << (l + 1)) | j;
-                uint4 p = W[i0], q = W[i0 + h];
-                W[i0] = q_add(p, q);
-                W[i0 + h] = q_cmul(q_sub(p, q), a.tw[h + j]);
-            }
-            __syncthreads();
-        }
+        dif_stages(W, 2 * B, (int)lb - 2, 3, 0, a.tw, jpos);
         const uint2 *k = a.kh + Bc;
-        for (uint32_t e = threadIdx.x; e < 2 * B; e += kBsThreads) W[e] = q_cmul(W[e], k[e & (Bc - 1)]);
-        __syncthreads();
-        for (uint32_t l = 0; l + 1 < lb; ++l) {
-            const uint32_t h = 1u << l;
-            for (uint32_t u = threadIdx.x; u < B; u += kBsThreads) {
-                const uint32_t j = u & (h - 1), i0 = ((u >> l) << (l + 1)) | j;
-                uint4 p = W[i0], q = q_cmul_conj(W[i0 + h], a.tw[h + j]);
-                W[i0] = q_add(p, q);
-                W[i0 + h] = q_sub(p, q);
-            }
-            __syncthreads();
-        }
+        fused_bottom(W, 2 * B, [&](uint32_t i) { return k[i & (Bc - 1)]; });
+        dit_stages(W, 2 * B, 3, (int)lb - 2, 0, a.tw, jpos);
         for (uint32_t e = threadIdx.x; e < B; e += kBsThreads) {
             const uint32_t tp = e >> (lb - 1), j = e & (H - 1);
             const uint4 g = q_add(W[(tp << lb) + j], q_cmul_conj(W[(tp << lb) + H + j], a.tw[H + j]));
