@@ -114,10 +114,10 @@ __device__ __forceinline__ uint4 q_rotl(uint4 v, uint32_t e)
 {
     return {m31_rotl(v.x, e), m31_rotl(v.y, e), m31_rotl(v.z, e), m31_rotl(v.w, e)};
 }
-// acc < 2^37 -> acc mod p
-__device__ __forceinline__ uint32_t fold37(uint64_t acc)
+// acc < 2^40 -> acc mod p
+__device__ __forceinline__ uint32_t fold40(uint64_t acc)
 {
-    const uint32_t r = (uint32_t)(acc & M31_P) + (uint32_t)(acc >> 31);  // < 2^31 + 2^6
+    const uint32_t r = (uint32_t)(acc & M31_P) + (uint32_t)(acc >> 31);  // < 2^31 + 2^9
     return r >= M31_P ? r - M31_P : r;
 }
 __device__ __forceinline__ uint32_t inv_pow2_exp(uint32_t n) { return (31u - n % 31u) % 31u; } // 2^-n = 2^this
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_tail(BsArgs a, uint4 *__restr
                 for (uint32_t m = 0; m + j < H; ++m) {
                     const uint4 v = D[sub + 2 * (j + m) + sigma];
                     const uint32_t r = a.rt[m];
-                    // each folded product is < 2^32: 32 of them cannot overflow 64 bits
+                    // each folded product is < 2^32, so the <= 32-term sums stay below 2^37
                     uint64_t t0 = (uint64_t)v.x * r, t1 = (uint64_t)v.y * r, t2 = (uint64_t)v.z * r,
                              t3 = (uint64_t)v.w * r;
                     acc0 += (t0 & M31_P) + (t0 >> 31);
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(kBsThreads) void k_bs_tail(BsArgs a, uint4 *__restr
                     acc2 += (t2 & M31_P) + (t2 >> 31);
                     acc3 += (t3 & M31_P) + (t3 >> 31);
                 }
-                const uint4 g = {fold37(acc0), fold37(acc1), fold37(acc2), fold37(acc3)};
+                const uint4 g = {fold40(acc0), fold40(acc1), fold40(acc2), fold40(acc3)};
                 W[o] = q_scale(g, a.ifact[j]);
             }
             __syncthreads();

@@ -191,6 +191,44 @@ def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
     }
 
 
+def witness_basis_form(ctx, log_n: int = 21, reps: int = 5, small_log_n: int = 13) -> dict:
+    """Monomial witness columns -> Circle-FFT basis (azk_witness_basis_*), resident: two full
+    columns of 2^(log_n - 1) QM31 monomial coefficients -> 2^log_n interleaved basis
+    coefficients.  The reference's route is O(n^2) on the CPU (the resident plan's dense matrix
+    stops at 4096 per column); at 2^small_log_n the oracle's C restatement of that route is
+    timed beside the device for scale.  Parity is covered by the tests (oracle to 2^15,
+    FFT-of-basis vs Horner evaluation at 2^17 .. 2^23)."""
+    half = 1 << (log_n - 1)
+    rng = np.random.default_rng(5)
+    x = ctx.upload(rng.integers(0, 2**31 - 1, size=(half, 4), dtype=np.uint32))
+    y = ctx.upload(rng.integers(0, 2**31 - 1, size=(half, 4), dtype=np.uint32))
+    out = ctx.buffer(32 * half)
+    plan = azk.CircleWitnessToFFTBasisPlanV1(ctx, log_n)
+    plan.execute_resident(x, half, y, half, out)
+    t = float(np.median([plan.execute_resident(x, half, y, half, out)["gpu_ns"] for _ in range(reps)])) * 1e-9
+    small = 1 << (small_log_n - 1)
+    xs = rng.integers(0, 2**31 - 1, size=(small, 4), dtype=np.uint32)
+    ys = rng.integers(0, 2**31 - 1, size=(small, 4), dtype=np.uint32)
+    sp = azk.CircleWitnessToFFTBasisPlanV1(ctx, small_log_n)
+    sp.execute(xs, ys)
+    ts = []
+    for _ in range(reps):
+        sp.execute(xs, ys)
+        ts.append(sp.last_stats["gpu_ns"])  # host arrays: upload + conversion + download
+    t_small = float(np.median(ts)) * 1e-9
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O  # the CPU baseline for this row (C restatement of the reference's O(n^2) route)
+    c0 = time.perf_counter()
+    O.basis_from_monomial(xs, ys, small_log_n)
+    t_cpu = time.perf_counter() - c0
+    for b in (x, y, out):
+        b.free()
+    return {"shape": f"2 x 2^{log_n - 1} monomial QM31 -> 2^{log_n} Circle-FFT basis", "ms": t * 1e3,
+            "coefficients_per_s": 2 * half / t,
+            "small": {"shape": f"2 x 2^{small_log_n - 1} -> 2^{small_log_n}", "gpu_ms": t_small * 1e3,
+                      "cpu_oracle_ms": t_cpu * 1e3, "cpu_cores": 1}}
+
+
 def calibrate_copy(ctx, log_count: int = 23) -> None:
     """A plain copy of known size for scaling the PMC byte counters: k_copy moves
     2^23 x 32 bytes (256 MiB) in and out (the prehashed commit's first kernel)."""
@@ -213,6 +251,8 @@ def main() -> int:
     ap.add_argument("--standalone-only", type=int, default=0,
                     help="profiling aid: only the 2^26 FFT / fold plans (one call each) and the calibration copy")
     ap.add_argument("--calibrate", type=int, default=0, help="run the known-size copy kernel (PMC scaling)")
+    ap.add_argument("--basis-only", type=int, default=0,
+                    help="profiling aid: only the witness -> Circle-FFT-basis conversion at 2^21")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard whole per GPU; "
@@ -221,6 +261,9 @@ def main() -> int:
 
     ctx = azk.Context(0)
     caps = ctx.capabilities()
+    if args.basis_only:
+        print(json.dumps({"witness_basis": witness_basis_form(ctx)}), flush=True)
+        return 0
     if args.standalone_only:
         calibrate_copy(ctx)
         print(json.dumps({"standalone": standalone_form(ctx, reps=1)}), flush=True)
@@ -313,6 +356,7 @@ def main() -> int:
         line["serving"] = serving_form(ctx)
     if args.standalone:
         line["standalone"] = standalone_form(ctx)
+        line["standalone"]["witness_basis"] = witness_basis_form(ctx)
     if args.cpu_sample:
         line["cpu_baseline"] = cpu_baseline(coeffs, args.cpu_sample)
     print(json.dumps(line), flush=True)
