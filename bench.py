@@ -229,6 +229,79 @@ def witness_basis_form(ctx, log_n: int = 21, reps: int = 5, small_log_n: int = 1
                       "cpu_oracle_ms": t_cpu * 1e3, "cpu_cores": 1}}
 
 
+def monomial_input_form(ctx, prover, steps: int = 20) -> dict:
+    """The commit workload with P and Q given as monomial coefficients (2^19 M31 each, lifted to
+    QM31): the device witness -> Circle-FFT-basis conversion, then the same commit, both on
+    resident buffers (wall time per commit).  The reference's monomial route is O(n^2) on the
+    CPU, so its own benchmark times the basis-coefficient input."""
+    h = 1 << (LOG_N - 2)
+    pq = inputs.random_m31(1, 2 * h)
+    x = np.zeros((h, 4), np.uint32)
+    y = np.zeros((h, 4), np.uint32)
+    x[:, 0], y[:, 0] = pq[:h], pq[h:]
+    bx, by = ctx.upload(x), ctx.upload(y)
+    basis = ctx.buffer(16 << LOG_N)
+    plan = azk.CircleWitnessToFFTBasisPlanV1(ctx, LOG_N)
+    conv = []
+
+    def step():
+        conv.append(plan.execute_resident(bx, h, by, h, basis)["gpu_ns"])
+        return prover.prove_fft_coefficients_resident(basis)
+
+    first = step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        last = step()
+    dt = (time.perf_counter() - t0) / steps
+    for b in (bx, by, basis):
+        b.free()
+    return {"shape": f"P, Q of 2^{LOG_N - 2} monomial M31 coefficients -> 2^{LOG_N} commit", "steps": steps,
+            "ms_per_commit": dt * 1e3, "commits_per_s": 1.0 / dt,
+            "conversion_ms": float(np.median(conv[1:])) * 1e-6, "deterministic": last == first}
+
+
+KECCAK_PERM_OPS = 4555  # VALU instructions of one one-lane Keccak-f[1600] (gfx950 assembly)
+
+
+def keccak_form(ctx, log_count: int = 22, reps: int = 10, warm: int = 30) -> dict:
+    """Keccak-f[1600] on 2^22 independent states (200 B each) and SHA3-256 / Keccak-256 of 2^22
+    one-block messages of 136 bytes (padding makes two permutations per message at the full
+    rate block), resident, uniform random bytes from a seed.  SHA3 digests are spot-checked
+    against hashlib."""
+    count = 1 << log_count
+    rng = np.random.default_rng(2)
+    states = ctx.upload(rng.integers(0, 256, size=200 * count, dtype=np.uint8))
+    sout = ctx.buffer(200 * count)
+    kp = azk.KeccakF1600PermutationBatchPlan(ctx, count)
+    for _ in range(warm):  # the clock ramps over the first calls after the uploads
+        kp.permute_resident(states, sout)
+    t_perm = float(np.median([kp.permute_resident(states, sout)["gpu_ns"] for _ in range(reps)])) * 1e-9
+    states.free()
+    sout.free()
+    msgs_host = rng.integers(0, 256, size=136 * count, dtype=np.uint8)
+    msgs = ctx.upload(msgs_host)
+    dout = ctx.buffer(32 * count)
+    res = {}
+    for name, cls in (("sha3_256", azk.SHA3FixedOneBlockHashPlan), ("keccak_256", azk.Keccak256FixedOneBlockHashPlan)):
+        hp = cls(ctx, count, 136, 136, 32)
+        for _ in range(warm):
+            hp.hash_resident(msgs, dout)
+        t = float(np.median([hp.hash_resident(msgs, dout)["gpu_ns"] for _ in range(reps)])) * 1e-9
+        res[name] = {"ms": t * 1e3, "messages_per_s": count / t, "gbps_in": 136 * count / t / 1e9}
+        if name == "sha3_256":
+            got = dout.download(32 * 1024).reshape(1024, 32)
+            res[name]["spot_check_1024_vs_hashlib"] = all(
+                hashlib.sha3_256(msgs_host[136 * i:136 * (i + 1)].tobytes()).digest() == got[i].tobytes()
+                for i in range(1024))
+    msgs.free()
+    dout.free()
+    ops = KECCAK_PERM_OPS * count
+    return {"states": count, "permute_ms": t_perm * 1e3, "perms_per_s": count / t_perm,
+            "roofline": {"bound": "valu", "achieved": ops / t_perm / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                         "frac": ops / t_perm / 1e12 / VALU_PEAK_TOPS, "ops_per_perm": KECCAK_PERM_OPS},
+            **res}
+
+
 def calibrate_copy(ctx, log_count: int = 23) -> None:
     """A plain copy of known size for scaling the PMC byte counters: k_copy moves
     2^23 x 32 bytes (256 MiB) in and out (the prehashed commit's first kernel)."""
@@ -251,6 +324,9 @@ def main() -> int:
     ap.add_argument("--standalone-only", type=int, default=0,
                     help="profiling aid: only the 2^26 FFT / fold plans (one call each) and the calibration copy")
     ap.add_argument("--calibrate", type=int, default=0, help="run the known-size copy kernel (PMC scaling)")
+    ap.add_argument("--monomial", type=int, default=1, help="the commit from monomial coefficients")
+    ap.add_argument("--keccak", type=int, default=1, help="Keccak-f / one-block hash batches of 2^22")
+    ap.add_argument("--keccak-only", type=int, default=0, help="profiling aid: only the Keccak batches (one call each)")
     ap.add_argument("--basis-only", type=int, default=0,
                     help="profiling aid: only the witness -> Circle-FFT-basis conversion at 2^21")
     args = ap.parse_args()
@@ -261,6 +337,9 @@ def main() -> int:
 
     ctx = azk.Context(0)
     caps = ctx.capabilities()
+    if args.keccak_only:
+        print(json.dumps({"keccak_batch": keccak_form(ctx, reps=1, warm=1)}), flush=True)
+        return 0
     if args.basis_only:
         print(json.dumps({"witness_basis": witness_basis_form(ctx)}), flush=True)
         return 0
@@ -354,6 +433,10 @@ def main() -> int:
         line["batched"] = batched_form(ctx)
     if args.serving:
         line["serving"] = serving_form(ctx)
+    if args.keccak:
+        line["keccak_batch"] = keccak_form(ctx)
+    if args.monomial:
+        line["monomial_input"] = monomial_input_form(ctx, prover)
     if args.standalone:
         line["standalone"] = standalone_form(ctx)
         line["standalone"]["witness_basis"] = witness_basis_form(ctx)
