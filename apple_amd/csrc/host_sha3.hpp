@@ -11,37 +11,66 @@ namespace azk::host {
 
 inline uint64_t rol(uint64_t v, int n) { return (v << n) | (v >> (64 - n)); }
 
-// Unrolled Keccak-f[1600] (FIPS 202), lanes a[x + 5y].
+constexpr uint64_t kRoundConstants[24] = {
+    0x0000000000000001ull, 0x0000000000008082ull, 0x800000000000808Aull, 0x8000000080008000ull,
+    0x000000000000808Bull, 0x0000000080000001ull, 0x8000000080008081ull, 0x8000000000008009ull,
+    0x000000000000008Aull, 0x0000000000000088ull, 0x0000000080008009ull, 0x000000008000000Aull,
+    0x000000008000808Bull, 0x800000000000008Bull, 0x8000000000008089ull, 0x8000000000008003ull,
+    0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
+    0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
+
+// One round from the 25 lanes S<xy> (x + 5y numbering: S00..S24) into D<xy>.  The five
+// chi rows are formed from their pi/rho sources directly: row y of the output takes lanes
+// (x + 3y, x) of the input for x = 0..4, each with its theta column term and rotation.
+#define AZK_KECCAK_ROUND(S, D, rc)                                                                              \
+    do {                                                                                                        \
+        const uint64_t c0 = S##00 ^ S##05 ^ S##10 ^ S##15 ^ S##20, c1 = S##01 ^ S##06 ^ S##11 ^ S##16 ^ S##21;    \
+        const uint64_t c2 = S##02 ^ S##07 ^ S##12 ^ S##17 ^ S##22, c3 = S##03 ^ S##08 ^ S##13 ^ S##18 ^ S##23;    \
+        const uint64_t c4 = S##04 ^ S##09 ^ S##14 ^ S##19 ^ S##24;                                               \
+        const uint64_t d0 = c4 ^ rol(c1, 1), d1 = c0 ^ rol(c2, 1), d2 = c1 ^ rol(c3, 1), d3 = c2 ^ rol(c4, 1);    \
+        const uint64_t d4 = c3 ^ rol(c0, 1);                                                                    \
+        uint64_t b0 = S##00 ^ d0, b1 = rol(S##06 ^ d1, 44), b2 = rol(S##12 ^ d2, 43), b3 = rol(S##18 ^ d3, 21);  \
+        uint64_t b4 = rol(S##24 ^ d4, 14);                                                                      \
+        D##00 = b0 ^ (~b1 & b2) ^ (rc); D##01 = b1 ^ (~b2 & b3); D##02 = b2 ^ (~b3 & b4);                       \
+        D##03 = b3 ^ (~b4 & b0); D##04 = b4 ^ (~b0 & b1);                                                       \
+        b0 = rol(S##03 ^ d3, 28); b1 = rol(S##09 ^ d4, 20); b2 = rol(S##10 ^ d0, 3); b3 = rol(S##16 ^ d1, 45);    \
+        b4 = rol(S##22 ^ d2, 61);                                                                               \
+        D##05 = b0 ^ (~b1 & b2); D##06 = b1 ^ (~b2 & b3); D##07 = b2 ^ (~b3 & b4);                              \
+        D##08 = b3 ^ (~b4 & b0); D##09 = b4 ^ (~b0 & b1);                                                       \
+        b0 = rol(S##01 ^ d1, 1); b1 = rol(S##07 ^ d2, 6); b2 = rol(S##13 ^ d3, 25); b3 = rol(S##19 ^ d4, 8);      \
+        b4 = rol(S##20 ^ d0, 18);                                                                               \
+        D##10 = b0 ^ (~b1 & b2); D##11 = b1 ^ (~b2 & b3); D##12 = b2 ^ (~b3 & b4);                              \
+        D##13 = b3 ^ (~b4 & b0); D##14 = b4 ^ (~b0 & b1);                                                       \
+        b0 = rol(S##04 ^ d4, 27); b1 = rol(S##05 ^ d0, 36); b2 = rol(S##11 ^ d1, 10); b3 = rol(S##17 ^ d2, 15);  \
+        b4 = rol(S##23 ^ d3, 56);                                                                               \
+        D##15 = b0 ^ (~b1 & b2); D##16 = b1 ^ (~b2 & b3); D##17 = b2 ^ (~b3 & b4);                              \
+        D##18 = b3 ^ (~b4 & b0); D##19 = b4 ^ (~b0 & b1);                                                       \
+        b0 = rol(S##02 ^ d2, 62); b1 = rol(S##08 ^ d3, 55); b2 = rol(S##14 ^ d4, 39); b3 = rol(S##15 ^ d0, 41);  \
+        b4 = rol(S##21 ^ d1, 2);                                                                                \
+        D##20 = b0 ^ (~b1 & b2); D##21 = b1 ^ (~b2 & b3); D##22 = b2 ^ (~b3 & b4);                              \
+        D##23 = b3 ^ (~b4 & b0); D##24 = b4 ^ (~b0 & b1);                                                       \
+    } while (0)
+
+// Keccak-f[1600] (FIPS 202), lanes a[x + 5y].  The state lives in two sets of 25 locals
+// and alternates between them round by round, so the compiler keeps it in registers
+// (about twice the throughput of an array-indexed loop under clang).
 inline void keccak_f(uint64_t a[25])
 {
-    static const uint64_t rc[24] = {
-        0x0000000000000001ull, 0x0000000000008082ull, 0x800000000000808Aull, 0x8000000080008000ull,
-        0x000000000000808Bull, 0x0000000080000001ull, 0x8000000080008081ull, 0x8000000000008009ull,
-        0x000000000000008Aull, 0x0000000000000088ull, 0x0000000080008009ull, 0x000000008000000Aull,
-        0x000000008000808Bull, 0x800000000000008Bull, 0x8000000000008089ull, 0x8000000000008003ull,
-        0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
-        0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
-    for (int round = 0; round < 24; ++round) {
-        uint64_t c0 = a[0] ^ a[5] ^ a[10] ^ a[15] ^ a[20], c1 = a[1] ^ a[6] ^ a[11] ^ a[16] ^ a[21];
-        uint64_t c2 = a[2] ^ a[7] ^ a[12] ^ a[17] ^ a[22], c3 = a[3] ^ a[8] ^ a[13] ^ a[18] ^ a[23];
-        uint64_t c4 = a[4] ^ a[9] ^ a[14] ^ a[19] ^ a[24];
-        uint64_t d0 = c4 ^ rol(c1, 1), d1 = c0 ^ rol(c2, 1), d2 = c1 ^ rol(c3, 1), d3 = c2 ^ rol(c4, 1),
-                 d4 = c3 ^ rol(c0, 1);
-        uint64_t b0 = a[0] ^ d0, b10 = rol(a[1] ^ d1, 1), b20 = rol(a[2] ^ d2, 62), b5 = rol(a[3] ^ d3, 28),
-                 b15 = rol(a[4] ^ d4, 27), b16 = rol(a[5] ^ d0, 36), b1 = rol(a[6] ^ d1, 44), b11 = rol(a[7] ^ d2, 6),
-                 b21 = rol(a[8] ^ d3, 55), b6 = rol(a[9] ^ d4, 20), b7 = rol(a[10] ^ d0, 3), b17 = rol(a[11] ^ d1, 10),
-                 b2 = rol(a[12] ^ d2, 43), b12 = rol(a[13] ^ d3, 25), b22 = rol(a[14] ^ d4, 39),
-                 b23 = rol(a[15] ^ d0, 41), b8 = rol(a[16] ^ d1, 45), b18 = rol(a[17] ^ d2, 15),
-                 b3 = rol(a[18] ^ d3, 21), b13 = rol(a[19] ^ d4, 8), b14 = rol(a[20] ^ d0, 18),
-                 b24 = rol(a[21] ^ d1, 2), b9 = rol(a[22] ^ d2, 61), b19 = rol(a[23] ^ d3, 56),
-                 b4 = rol(a[24] ^ d4, 14);
-        a[0] = b0 ^ (~b1 & b2) ^ rc[round]; a[1] = b1 ^ (~b2 & b3); a[2] = b2 ^ (~b3 & b4); a[3] = b3 ^ (~b4 & b0); a[4] = b4 ^ (~b0 & b1);
-        a[5] = b5 ^ (~b6 & b7); a[6] = b6 ^ (~b7 & b8); a[7] = b7 ^ (~b8 & b9); a[8] = b8 ^ (~b9 & b5); a[9] = b9 ^ (~b5 & b6);
-        a[10] = b10 ^ (~b11 & b12); a[11] = b11 ^ (~b12 & b13); a[12] = b12 ^ (~b13 & b14); a[13] = b13 ^ (~b14 & b10); a[14] = b14 ^ (~b10 & b11);
-        a[15] = b15 ^ (~b16 & b17); a[16] = b16 ^ (~b17 & b18); a[17] = b17 ^ (~b18 & b19); a[18] = b18 ^ (~b19 & b15); a[19] = b19 ^ (~b15 & b16);
-        a[20] = b20 ^ (~b21 & b22); a[21] = b21 ^ (~b22 & b23); a[22] = b22 ^ (~b23 & b24); a[23] = b23 ^ (~b24 & b20); a[24] = b24 ^ (~b20 & b21);
+    uint64_t s00 = a[0], s01 = a[1], s02 = a[2], s03 = a[3], s04 = a[4], s05 = a[5], s06 = a[6], s07 = a[7];
+    uint64_t s08 = a[8], s09 = a[9], s10 = a[10], s11 = a[11], s12 = a[12], s13 = a[13], s14 = a[14], s15 = a[15];
+    uint64_t s16 = a[16], s17 = a[17], s18 = a[18], s19 = a[19], s20 = a[20], s21 = a[21], s22 = a[22], s23 = a[23];
+    uint64_t s24 = a[24];
+    uint64_t t00, t01, t02, t03, t04, t05, t06, t07, t08, t09, t10, t11, t12, t13, t14, t15, t16, t17, t18, t19, t20,
+        t21, t22, t23, t24;
+    for (int r = 0; r < 24; r += 2) {
+        AZK_KECCAK_ROUND(s, t, kRoundConstants[r]);
+        AZK_KECCAK_ROUND(t, s, kRoundConstants[r + 1]);
     }
+    a[0] = s00; a[1] = s01; a[2] = s02; a[3] = s03; a[4] = s04; a[5] = s05; a[6] = s06; a[7] = s07; a[8] = s08;
+    a[9] = s09; a[10] = s10; a[11] = s11; a[12] = s12; a[13] = s13; a[14] = s14; a[15] = s15; a[16] = s16;
+    a[17] = s17; a[18] = s18; a[19] = s19; a[20] = s20; a[21] = s21; a[22] = s22; a[23] = s23; a[24] = s24;
 }
+#undef AZK_KECCAK_ROUND
 
 inline void xor_block(uint64_t a[25], const uint8_t *p, size_t n)
 {
