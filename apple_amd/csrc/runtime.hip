@@ -73,9 +73,9 @@ azk_status check_exec_plan(const azk_circle_exec_plan &x)
 {
     if (x.log_tile_commit < 6 || x.log_tile_commit > kDefaultLogTile || x.log_tile_fold < 6 ||
         x.log_tile_fold > kDefaultLogTile || x.log_tile_leafwise < 5 || x.log_tile_leafwise > kLeafwiseLogTile ||
-        x.log_leafwise_below > 31 || x.log_tree_chunk < 4 || x.log_tree_chunk > kUpperLogChunk)
+        x.log_leafwise_below > 31 || x.log_tree_chunk < 3 || x.log_tree_chunk > kUpperLogChunk)
         return fail(AZK_ERR_INVALID_KERNEL_CONFIG,
-                    "execution plan outside the supported tiles (commit/fold 6..11, leafwise 5..8, chunk 4..10)");
+                    "execution plan outside the supported tiles (commit/fold 6..11, leafwise 5..8, chunk 3..10)");
     return AZK_OK;
 }
 

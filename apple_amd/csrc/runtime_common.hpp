@@ -78,7 +78,10 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr uint32_t kP = 0x7fffffffu;
 constexpr uint32_t kDefaultLogTile = 11;    // 2048-element tiles: 48 KB LDS per block
-constexpr uint32_t kUpperLogChunk = 10;     // upper-tree reduction chunk
+constexpr uint32_t kUpperLogChunk = 10;     // largest upper-tree reduction chunk
+// Commit trees: 16-node chunks, so every level of a step after the first is cooperative
+// (measured 2.5-3.5% faster per 2^21 commit than 1024-node chunks).
+constexpr uint32_t kCommitLogTreeChunk = 4;
 constexpr uint32_t kLeafwiseLogTile = 8;    // latency form: one leaf per thread, 256 per block
 
 struct DeviceMem {
@@ -234,7 +237,7 @@ struct ExecPlan {
     uint32_t log_tile_fold = kDefaultLogTile;      // committed folds, big-tile form
     uint32_t log_leafwise_below = 19;              // layers < 2^this: one leaf per thread
     uint32_t log_tile_leafwise = kLeafwiseLogTile; // latency-form tile
-    uint32_t log_tree_chunk = kUpperLogChunk;      // upper-tree reduction chunk
+    uint32_t log_tree_chunk = kCommitLogTreeChunk; // upper-tree reduction chunk
 };
 azk_status check_exec_plan(const azk_circle_exec_plan &x);
 ExecPlan to_exec_plan(const azk_circle_exec_plan &x);

@@ -230,7 +230,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
     if (config) c = *config;
     if (!c.warmup_runs) c.warmup_runs = 2;
     if (!c.measured_runs) c.measured_runs = 5;
-    if (!c.max_candidates) c.max_candidates = 16;
+    if (!c.max_candidates) c.max_candidates = 24;
     if (!c.validation_runs) c.validation_runs = 3;
     std::lock_guard<std::mutex> g(p->mu);
     azk_status s = check_range(coeffs, off, (size_t)16 << p->d.log_n);
@@ -272,11 +272,11 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
     // coordinate descent: one knob at a time, the others at the best plan so far
     struct Knob { uint32_t ExecPlan::*field; std::vector<uint32_t> values; };
     const std::vector<Knob> knobs = {
+        {&ExecPlan::log_tree_chunk, {3, 4, 5, 6, 8, 10}},
         {&ExecPlan::log_tile_fold, {9, 10, 11}},
         {&ExecPlan::log_tile_commit, {9, 10, 11}},
         {&ExecPlan::log_leafwise_below, {17, 18, 19, 20, 21}},
         {&ExecPlan::log_tile_leafwise, {6, 7, 8}},
-        {&ExecPlan::log_tree_chunk, {6, 8, 10}},
     };
     uint32_t raced = 1;
     for (const Knob &k : knobs) {

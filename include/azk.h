@@ -254,7 +254,7 @@ typedef struct azk_circle_exec_plan {
     uint32_t log_tile_fold;       /* committed folds, big-tile form: 2^6..2^11 outputs per block (11) */
     uint32_t log_leafwise_below;  /* committed layers below 2^this fold one leaf per thread (19) */
     uint32_t log_tile_leafwise;   /* 2^5..2^8 leaves per block in that form (8) */
-    uint32_t log_tree_chunk;      /* upper Merkle levels reduced per block, 4..10 (10) */
+    uint32_t log_tree_chunk;      /* upper Merkle levels reduced per block, 3..10 (4) */
 } azk_circle_exec_plan;
 azk_status azk_circle_prover_set_exec_plan(azk_circle_prover *prover, const azk_circle_exec_plan *plan);
 azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *prover, azk_circle_exec_plan *plan);
@@ -262,7 +262,7 @@ azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *prover, azk_
 typedef struct azk_tuning_config {
     uint32_t warmup_runs;       /* 2 */
     uint32_t measured_runs;     /* 5 */
-    uint32_t max_candidates;    /* 16 plans raced (coordinate descent from the default) */
+    uint32_t max_candidates;    /* 24 plans raced (coordinate descent from the default) */
     uint32_t validation_runs;   /* 3 random inputs on which the winner must equal the default plan */
     const char *database_path;  /* JSON-lines plan records; NULL = no persistence */
     int force;                  /* 1 = race even when the database holds a winner */

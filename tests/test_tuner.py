@@ -17,6 +17,7 @@ PLANS = [
     dict(log_tile_commit=9, log_tile_fold=9),
     dict(log_tile_commit=10, log_tile_fold=10, log_tree_chunk=6),
     dict(log_tile_commit=6, log_tile_fold=6, log_tile_leafwise=5, log_tree_chunk=4),
+    dict(log_tree_chunk=3),
     dict(log_leafwise_below=21, log_tile_leafwise=6),
     dict(log_leafwise_below=0, log_tile_fold=8),      # every committed layer in the big-tile form
     dict(log_leafwise_below=31, log_tile_leafwise=7),  # every committed layer one leaf per thread
