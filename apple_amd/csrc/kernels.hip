@@ -261,8 +261,11 @@ __global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *_
     reduce_lds(A, B, T / 2, 1, g0 >> 1, tree, m);
 }
 
+__device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops);
+
 // Upper Merkle levels: each block loads 2^log_c nodes of `level` and reduces them
 // by log_c levels (MerkleCommitter.swift:853-893: parents per level + fused upper tree).
+// A block of a job's last step then runs the job's transcript program for that root.
 __global__ __launch_bounds__(256) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
@@ -281,6 +284,10 @@ __global__ __launch_bounds__(256) void k_tree_reduce(const TreeDesc *__restrict_
     }
     __syncthreads();
     reduce_lds(A, B, C, l, g, tree, m);
+    if (d.t_ops) {  // uniform per block; the barrier makes the root's HBM words visible to wave 0
+        __syncthreads();
+        if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
+    }
 }
 
 // Word copies, 1024 words per block.
@@ -610,15 +617,10 @@ __device__ __forceinline__ void t_squeeze(uint32_t lo, uint32_t hi, const CoopLa
     }
 }
 
-// Runs a transcript program on its job's persistent 200-byte state, one wave per job
+// Runs a transcript program on its job's persistent 200-byte state with one whole wave
 // (lanes 0..24 hold the state words; lanes 32..56 compute an unused second state).
-__global__ __launch_bounds__(64) void k_transcript(const TranscriptDesc *__restrict__ tab, uint32_t ntab)
+__device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops)
 {
-    uint32_t unused;
-    const TranscriptDesc &d = find_desc(tab, ntab, unused);
-    const TranscriptOp *ops = d.ops;
-    uint64_t *state = d.state;
-    const uint32_t nops = d.nops;
     const CoopLane c = coop_lane();
     const bool holds = c.word < 25 && __lane_id() < 32;
     uint32_t lo = 0, hi = 0;
@@ -644,6 +646,13 @@ __global__ __launch_bounds__(64) void k_transcript(const TranscriptDesc *__restr
         }
     }
     if (holds) state[c.word] = (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+__global__ __launch_bounds__(64) void k_transcript(const TranscriptDesc *__restrict__ tab, uint32_t ntab)
+{
+    uint32_t unused;
+    const TranscriptDesc &d = find_desc(tab, ntab, unused);
+    run_transcript(d.ops, d.state, d.nops);
 }
 
 // ============================================================== proof emission

@@ -61,7 +61,10 @@ struct FoldDesc {
 struct TreeDesc {
     uint32_t *tree;
     uint64_t m;  // leaves of the tree
-    uint32_t level, log_c, first_block, pad;
+    // transcript program run by the block that computes the root (final step only)
+    const struct TranscriptOp *t_ops;
+    uint64_t *t_state;
+    uint32_t level, log_c, first_block, t_nops;
 };
 
 // A transcript program run by one wave.

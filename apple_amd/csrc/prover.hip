@@ -219,17 +219,26 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
         const ExecPlan &xp = p->xp;
         plan_fft(R.fft_pass, R.fft_final, j.src, p->layers[0], (const uint32_t *)p->fft_tw.p, p->d.log_n, xp.log_tile_commit,
                  j.mode == 0, p->trees[0]);
-        plan_tree(R.tree0, p->trees[0], p->n, std::min(xp.log_tile_commit, p->d.log_n), xp.log_tree_chunk);
+        // round r's transcript program runs in the block that computes root r when the
+        // root comes from an upper-tree step, and as its own launch otherwise
+        auto prog = [&](uint32_t r) { return TranscriptDesc{ops + p->prog_a[r].first, p->tstate, p->prog_a[r].second, 0}; };
+        {
+            const TranscriptDesc t0 = prog(0);
+            if (!plan_tree(R.tree0, p->trees[0], p->n, std::min(xp.log_tile_commit, p->d.log_n), xp.log_tree_chunk, &t0))
+                R.tr_a[0].add(t0, 1);
+        }
         uint64_t off = 0, m = p->n;
         for (uint32_t r = 0; r < R_j; ++r) {
-            R.tr_a[r].add(TranscriptDesc{ops + p->prog_a[r].first, p->tstate, p->prog_a[r].second, 0}, 1);
             const uint64_t n_out = m / 2;
             const bool commit_next = r + 1 < R_j;
             const uint32_t lt = plan_fold(R.fold_big[r], R.fold_leaf[r], r == 0 ? (const void *)j.layer0() : (const void *)p->layers[r],
                                           p->layers[r + 1], (const uint32_t *)p->fold_tw.p + off, p->challenges + 4 * r, n_out,
                                           xp.log_tile_fold, commit_next ? p->trees[r + 1] : nullptr,
                                           xp.log_leafwise_below, xp.log_tile_leafwise);
-            if (commit_next) plan_tree(R.tree_r[r], p->trees[r + 1], n_out, lt, xp.log_tree_chunk);
+            if (commit_next) {
+                const TranscriptDesc tn = prog(r + 1);
+                if (!plan_tree(R.tree_r[r], p->trees[r + 1], n_out, lt, xp.log_tree_chunk, &tn)) R.tr_a[r + 1].add(tn, 1);
+            }
             off += n_out;
             m = n_out;
         }

@@ -164,12 +164,21 @@ uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in
     return lt;
 }
 
-void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level, uint32_t log_chunk)
+bool plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level, uint32_t log_chunk,
+               const TranscriptDesc *after)
 {
     auto sch = tree_schedule(m, level, log_chunk);
     if (steps.size() < sch.size()) steps.resize(sch.size());
-    for (size_t k = 0; k < sch.size(); ++k)
-        steps[k].add(TreeDesc{tree, m, sch[k].level, sch[k].log_c, 0, 0}, (uint32_t)sch[k].chunks, sch[k].log_c);
+    for (size_t k = 0; k < sch.size(); ++k) {
+        TreeDesc d{tree, m, nullptr, nullptr, sch[k].level, sch[k].log_c, 0, 0};
+        if (after && k + 1 == sch.size()) {  // the last step is one chunk: one block computes the root
+            d.t_ops = after->ops;
+            d.t_state = after->state;
+            d.t_nops = after->nops;
+        }
+        steps[k].add(d, (uint32_t)sch[k].chunks, sch[k].log_c);
+    }
+    return after && !sch.empty();
 }
 
 // A one-off program for the single-job plans: tables are built per call, uploaded,

@@ -253,7 +253,10 @@ uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in
                    const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree,
                    uint32_t log_leafwise_below = 19, uint32_t log_t_leaf = kLeafwiseLogTile);
 
-void plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level,
-               uint32_t log_chunk = kUpperLogChunk);
+// Appends the upper-tree steps of one tree.  With `after`, the block of the last step
+// runs that transcript program once the root is written (returns false, and runs
+// nothing, when the tree needs no step: the caller then launches the program itself).
+bool plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level,
+               uint32_t log_chunk = kUpperLogChunk, const TranscriptDesc *after = nullptr);
 
 } // namespace azk::rt
