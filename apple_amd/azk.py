@@ -198,6 +198,15 @@ def load() -> C.CDLL:
         "azk_sumcheck_prove_resident": (st, [vp, vp, sz, vp, vp, C.POINTER(SumcheckResult), sp]),
         "azk_sumcheck_verify": (st, [vp, sz, C.POINTER(C.c_int)]),
         "azk_circle_verify": (st, [vp, sz, vp, C.POINTER(C.c_int)]),
+        "azk_circle_verify_with_coefficients": (st, [vp, sz, vp, vp, sz, C.POINTER(C.c_int)]),
+        "azk_circle_prove_fft_coefficients_verified": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_circle_prove_fft_coefficients_resident_verified": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_codeword_execute_verified": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_fold_execute_verified": (st, [vp, vp, sz, vp, vp, sz, sp]),
+        "azk_witness_basis_execute_verified": (st, [vp, vp, u32, vp, u32, vp, sz, sp]),
+        "azk_host_circle_fft": (st, [vp, u32, vp]),
+        "azk_host_fold_chain": (st, [vp, u32, vp, u32, vp]),
+        "azk_witness_basis_check": (st, [vp, u32, vp, u32, u32, vp, C.POINTER(C.c_int)]),
         "azk_circle_prover_set_exec_plan": (st, [vp, C.POINTER(ExecPlan)]),
         "azk_circle_prover_get_exec_plan": (st, [vp, C.POINTER(ExecPlan)]),
         "azk_circle_prover_tune": (st, [vp, vp, sz, C.POINTER(TuningConfig), C.POINTER(TuningResult)]),
@@ -503,11 +512,12 @@ class CircleCodewordPlan(_Plan):
         _track(self)
         self.last_stats: dict = {}
 
-    def execute(self, coeffs: np.ndarray) -> np.ndarray:
+    def execute(self, coeffs: np.ndarray, verified: bool = False) -> np.ndarray:
         a = _host(np.ascontiguousarray(coeffs, dtype=np.uint32))
         out = np.zeros(4 << self.log_n, np.uint32)
         s = Stats()
-        _ck(self.ctx._lib.azk_codeword_execute(self.h, _ptr(a), a.size, _ptr(out), out.nbytes, C.byref(s)))
+        fn = self.ctx._lib.azk_codeword_execute_verified if verified else self.ctx._lib.azk_codeword_execute
+        _ck(fn(self.h, _ptr(a), a.size, _ptr(out), out.nbytes, C.byref(s)))
         self.last_stats = _stats_dict(s)
         return out
 
@@ -536,14 +546,14 @@ class CircleWitnessToFFTBasisPlanV1(_Plan):
         _track(self)
         self.last_stats: dict = {}
 
-    def execute(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    def execute(self, x: np.ndarray, y: np.ndarray, verified: bool = False) -> np.ndarray:
         xa = np.ascontiguousarray(np.asarray(x, dtype=np.uint32).reshape(-1, 4))
         ya = np.ascontiguousarray(np.asarray(y, dtype=np.uint32).reshape(-1, 4))
         out = np.zeros(4 << self.log_n, np.uint32)
         s = Stats()
-        _ck(self.ctx._lib.azk_witness_basis_execute(self.h, _ptr(xa) if len(xa) else None, len(xa),
-                                                     _ptr(ya) if len(ya) else None, len(ya), _ptr(out), out.size,
-                                                     C.byref(s)))
+        fn = self.ctx._lib.azk_witness_basis_execute_verified if verified else self.ctx._lib.azk_witness_basis_execute
+        _ck(fn(self.h, _ptr(xa) if len(xa) else None, len(xa), _ptr(ya) if len(ya) else None, len(ya), _ptr(out),
+               out.size, C.byref(s)))
         self.last_stats = _stats_dict(s)
         return out
 
@@ -567,12 +577,13 @@ class CircleFRIFoldChainPlan(_Plan):
         self.h = h
         _track(self)
 
-    def execute(self, evals: np.ndarray, challenges) -> np.ndarray:
+    def execute(self, evals: np.ndarray, challenges, verified: bool = False) -> np.ndarray:
         a = _host(np.ascontiguousarray(evals, dtype=np.uint32))
         ch = np.ascontiguousarray(np.asarray(challenges, dtype=np.uint32).reshape(-1))
         out = np.zeros(4 << (self.log_n - self.rounds), np.uint32)
         s = Stats()
-        _ck(self.ctx._lib.azk_fold_execute(self.h, _ptr(a), a.size, _ptr(ch), _ptr(out), out.nbytes, C.byref(s)))
+        fn = self.ctx._lib.azk_fold_execute_verified if verified else self.ctx._lib.azk_fold_execute
+        _ck(fn(self.h, _ptr(a), a.size, _ptr(ch), _ptr(out), out.nbytes, C.byref(s)))
         return out
 
     def execute_resident(self, evals: Buffer, challenges, out: Buffer, evals_offset: int = 0, out_offset: int = 0) -> dict:
@@ -608,10 +619,11 @@ class CircleCodewordPCSFRIProverV1(_Plan):
         self.last_stats = _stats_dict(s)
         return self._out[: n.value].tobytes()
 
-    def prove_fft_coefficients_resident(self, coeffs: Buffer, offset: int = 0) -> bytes:
+    def prove_fft_coefficients_resident(self, coeffs: Buffer, offset: int = 0, verified: bool = False) -> bytes:
         n, s = C.c_size_t(0), Stats()
-        rc = self.ctx._lib.azk_circle_prove_fft_coefficients_resident(self.h, coeffs.h, offset, _ptr(self._out),
-                                                                     self.proof_size, C.byref(n), C.byref(s))
+        fn = (self.ctx._lib.azk_circle_prove_fft_coefficients_resident_verified if verified
+              else self.ctx._lib.azk_circle_prove_fft_coefficients_resident)
+        rc = fn(self.h, coeffs.h, offset, _ptr(self._out), self.proof_size, C.byref(n), C.byref(s))
         return self._finish(rc, n, s)
 
     def prove_evaluations_resident(self, evals: Buffer, offset: int = 0) -> bytes:
@@ -620,11 +632,12 @@ class CircleCodewordPCSFRIProverV1(_Plan):
                                                                 self.proof_size, C.byref(n), C.byref(s))
         return self._finish(rc, n, s)
 
-    def prove_fft_coefficients(self, coeffs: np.ndarray) -> bytes:
+    def prove_fft_coefficients(self, coeffs: np.ndarray, verified: bool = False) -> bytes:
         a = _host(np.ascontiguousarray(coeffs, dtype=np.uint32))
         n, s = C.c_size_t(0), Stats()
-        rc = self.ctx._lib.azk_circle_prove_fft_coefficients(self.h, _ptr(a), a.size, _ptr(self._out), self.proof_size,
-                                                            C.byref(n), C.byref(s))
+        fn = (self.ctx._lib.azk_circle_prove_fft_coefficients_verified if verified
+              else self.ctx._lib.azk_circle_prove_fft_coefficients)
+        rc = fn(self.h, _ptr(a), a.size, _ptr(self._out), self.proof_size, C.byref(n), C.byref(s))
         return self._finish(rc, n, s)
 
     def set_graph(self, enable: bool) -> None:
@@ -780,6 +793,55 @@ def circle_verify(proof: bytes, public_input_digest: bytes) -> bool:
     d = np.frombuffer(public_input_digest, np.uint8)
     ok = C.c_int(0)
     _ck(lib.azk_circle_verify(b.ctypes.data, len(proof), d.ctypes.data, C.byref(ok)))
+    return bool(ok.value)
+
+
+def circle_verify_with_coefficients(proof: bytes, public_input_digest: bytes, coeffs: np.ndarray) -> bool:
+    """circle_verify, and every opened first-layer value equals the host Circle FFT of the
+    line-basis coefficients (the prover's CPU gate)."""
+    lib = load()
+    if len(public_input_digest) != 32:
+        raise AZKError(6, "public input digest must be 32 bytes")
+    b = np.frombuffer(proof, np.uint8) if len(proof) else np.zeros(1, np.uint8)
+    d = np.frombuffer(public_input_digest, np.uint8)
+    c = np.ascontiguousarray(coeffs, dtype=np.uint32).reshape(-1)
+    ok = C.c_int(0)
+    _ck(lib.azk_circle_verify_with_coefficients(b.ctypes.data, len(proof), d.ctypes.data, _ptr(c), c.size, C.byref(ok)))
+    return bool(ok.value)
+
+
+def host_circle_fft(coeffs: np.ndarray, log_n: int) -> np.ndarray:
+    """CircleCodewordOracle.evaluateWithCircleFFT on the host (the codeword plan's CPU gate)."""
+    c = np.ascontiguousarray(coeffs, dtype=np.uint32).reshape(-1)
+    if c.size != 4 << log_n:
+        raise AZKError(6, "need 4 x 2^log_n coefficient words")
+    out = np.zeros(4 << log_n, np.uint32)
+    _ck(load().azk_host_circle_fft(_ptr(c), log_n, _ptr(out)))
+    return out
+
+
+def host_fold_chain(evals: np.ndarray, log_n: int, challenges) -> np.ndarray:
+    """CircleFRILayerOracleV1.fold on the host (the fold chain plan's CPU gate)."""
+    e = np.ascontiguousarray(evals, dtype=np.uint32).reshape(-1)
+    ch = np.ascontiguousarray(np.asarray(challenges, dtype=np.uint32).reshape(-1))
+    rounds = ch.size // 4
+    if e.size != 4 << log_n or not 1 <= rounds <= log_n:
+        raise AZKError(6, "need 4 x 2^log_n evaluation words and 1..log_n challenges")
+    out = np.zeros(4 << (log_n - rounds), np.uint32)
+    _ck(load().azk_host_fold_chain(_ptr(e), log_n, _ptr(ch), rounds, _ptr(out)))
+    return out
+
+
+def witness_basis_check(x: np.ndarray, y: np.ndarray, log_n: int, basis: np.ndarray) -> bool:
+    """Whether the basis and the monomial columns agree at pseudo-random QM31 points."""
+    xa = np.ascontiguousarray(np.asarray(x, dtype=np.uint32).reshape(-1, 4))
+    ya = np.ascontiguousarray(np.asarray(y, dtype=np.uint32).reshape(-1, 4))
+    b = np.ascontiguousarray(basis, dtype=np.uint32).reshape(-1)
+    if b.size != 4 << log_n:
+        raise AZKError(6, "need 4 x 2^log_n basis words")
+    ok = C.c_int(0)
+    _ck(load().azk_witness_basis_check(_ptr(xa) if len(xa) else None, len(xa), _ptr(ya) if len(ya) else None, len(ya),
+                                       log_n, _ptr(b), C.byref(ok)))
     return bool(ok.value)
 
 

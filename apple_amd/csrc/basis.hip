@@ -36,6 +36,7 @@
 //   blocks of B <= 2^kLogTail: k_bs_tail runs every remaining level in LDS and writes
 //     out[2i + family] = 2^popcount(i) D[bitrev(i)].
 #include "device_field.hpp"
+#include "host_gates.hpp"
 #include "runtime_common.hpp"
 
 using namespace azk;
@@ -894,6 +895,21 @@ azk_status azk_witness_basis_execute(azk_basis_plan *p, const uint32_t *x, uint3
         });
     }
     return s ? s : read_flag(p);
+}
+
+// executeVerified (CircleWitnessFFTBasis.swift:483-517): the device basis against the
+// monomial columns at pseudo-random QM31 points on the host (the reference recomputes the
+// basis with its O(n^2) oracle; a point check costs O(n) and misses a wrong basis with
+// probability below 2^-98 per point).
+azk_status azk_witness_basis_execute_verified(azk_basis_plan *p, const uint32_t *x, uint32_t x_count,
+                                              const uint32_t *y, uint32_t y_count, uint32_t *out, size_t out_words,
+                                              azk_stats *stats)
+{
+    azk_status s = azk_witness_basis_execute(p, x, x_count, y, y_count, out, out_words, stats);
+    if (s) return s;
+    if (!host::basis_consistent(x, x_count, y, y_count, p->log_n, out))
+        return fail(AZK_ERR_CORRECTNESS, "witness basis device result did not match the CPU gate");
+    return AZK_OK;
 }
 
 } // extern "C"

@@ -1,0 +1,191 @@
+// CPU gates: host computations of the path's results for the *_verified entry points,
+// and their C-ABI exports (the reference exposes its CPU oracles as public API too:
+// CircleCodewordOracle, CircleFRILayerOracleV1).  Host code only; the device path never
+// calls into this file except to compare after its own result is complete.
+#include <vector>
+
+#include "host_field.hpp"
+#include "host_gates.hpp"
+#include "host_sha3.hpp"
+#include "runtime_common.hpp"
+
+using namespace azk;
+using namespace azk::host;
+
+namespace azk::host {
+
+namespace {
+
+std::vector<Q> load_q(const uint32_t *w, size_t n)
+{
+    std::vector<Q> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = {w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]};
+    return v;
+}
+void store_q(const std::vector<Q> &v, uint32_t *w)
+{
+    for (size_t i = 0; i < v.size(); ++i) {
+        w[4 * i] = v[i].a;
+        w[4 * i + 1] = v[i].b;
+        w[4 * i + 2] = v[i].c;
+        w[4 * i + 3] = v[i].d;
+    }
+}
+bool all_canonical(const uint32_t *w, size_t words)
+{
+    for (size_t i = 0; i < words; ++i)
+        if (w[i] >= kM31) return false;
+    return true;
+}
+
+// Stage s twiddle h: y (s = 0) or x doubled s - 1 times (s >= 1) of the point at storage
+// index h << (s + 1) (CircleCodewordOracle.circleFFTTwiddles, CircleCodeword.swift:129-164).
+uint32_t fft_twiddle(const std::vector<Pt> &pts, uint32_t s, uint64_t h)
+{
+    const Pt p = pts[h << (s + 1)];
+    if (s == 0) return p.y;
+    uint32_t x = p.x;
+    for (uint32_t k = 1; k < s; ++k) x = double_x(x);
+    return x;
+}
+
+// pi(t) = 2 t^2 - 1 = T_2(t)
+Q q_pi(Q t)
+{
+    const Q sq = qmul(t, t);
+    return qsub(qadd(sq, sq), Q{1, 0, 0, 0});
+}
+
+} // namespace
+
+void circle_fft(const uint32_t *coeffs, uint32_t log_n, uint32_t *out)
+{
+    const uint64_t n = 1ull << log_n;
+    std::vector<Q> v = load_q(coeffs, n);
+    const std::vector<Pt> pts = Domain::canonical_of(log_n).storage_points();
+    for (int sk = (int)log_n - 1; sk >= 0; --sk) {  // stages log n - 1 .. 1, then 0
+        const uint32_t s = (uint32_t)sk;
+        const uint64_t cnt = n >> (s + 1), lanes = 1ull << s;
+        for (uint64_t h = 0; h < cnt; ++h) {
+            const uint32_t t = fft_twiddle(pts, s, h);
+            for (uint64_t lane = 0; lane < lanes; ++lane) {
+                const uint64_t l = (h << (s + 1)) + lane, r = l + lanes;
+                const Q sr = qscale(v[r], t), left = v[l];
+                v[l] = qadd(left, sr);
+                v[r] = qsub(left, sr);
+            }
+        }
+    }
+    store_q(v, out);
+}
+
+bool fold_chain(const uint32_t *evals, uint32_t log_n, const uint32_t *challenges, uint32_t rounds, uint32_t *out)
+{
+    const Domain dom = Domain::canonical_of(log_n);
+    const std::vector<Pt> pts = dom.storage_points();
+    std::vector<Q> cur = load_q(evals, 1ull << log_n);
+    const Q inv2{1u << 30, 0, 0, 0};
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint64_t n_out = cur.size() / 2;
+        std::vector<uint32_t> t(n_out);
+        for (uint64_t p = 0; p < n_out; ++p) {  // the pair's domain coordinate, then its inverse
+            if (r == 0) {
+                t[p] = pts[2 * p].y;
+            } else {
+                uint32_t x = pts[p << (r + 1)].x;
+                for (uint32_t k = 1; k < r; ++k) x = double_x(x);
+                t[p] = x;
+            }
+        }
+        for (uint32_t c : t)
+            if (!c) return false;  // inverseDomainLayers refuses a zero coordinate
+        batch_inverse(t);
+        const Q ch{challenges[4 * r], challenges[4 * r + 1], challenges[4 * r + 2], challenges[4 * r + 3]};
+        std::vector<Q> next(n_out);
+        for (uint64_t p = 0; p < n_out; ++p) {  // (l + r + ch ((l - r) t)) / 2, QM31FRIFold.swift:11-44
+            const Q l = cur[2 * p], rr = cur[2 * p + 1];
+            next[p] = qmul(qadd(qadd(l, rr), qmul(ch, qscale(qsub(l, rr), t[p]))), inv2);
+        }
+        cur.swap(next);
+    }
+    store_q(cur, out);
+    return true;
+}
+
+bool basis_consistent(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny, uint32_t log_n,
+                      const uint32_t *basis, uint32_t points)
+{
+    const uint64_t n = 1ull << log_n, half = n >> 1;
+    // points from a sponge over the shape and samples of the inputs and the basis
+    Transcript t;
+    uint32_t hdr[3] = {log_n, nx, ny};
+    t.absorb((const uint8_t *)hdr, sizeof hdr);
+    const size_t sample = std::min<uint64_t>(n, 64);
+    t.absorb((const uint8_t *)basis, 16 * sample);
+    t.absorb((const uint8_t *)(basis + 4 * (n - sample)), 16 * sample);
+    if (nx) t.absorb((const uint8_t *)(x + 4 * (size_t)(nx - 1)), 16);
+    if (ny) t.absorb((const uint8_t *)(y + 4 * (size_t)(ny - 1)), 16);
+    std::vector<uint32_t> w(8 * points);
+    t.squeeze_u32((uint32_t)w.size(), kM31, w.data());
+    std::vector<Q> v(half);
+    for (uint32_t f = 0; f < 2; ++f) {
+        const uint32_t *mono = f ? y : x;
+        const uint32_t cnt = f ? ny : nx;
+        for (uint32_t k = 0; k < points; ++k) {
+            const uint32_t *pw = w.data() + 8 * k + 4 * f;
+            const Q z{pw[0], pw[1], pw[2], pw[3]};
+            Q lhs{0, 0, 0, 0};  // Horner over the monomial column
+            for (uint32_t i = cnt; i-- > 0;)
+                lhs = qadd(qmul(lhs, z), Q{mono[4 * i], mono[4 * i + 1], mono[4 * i + 2], mono[4 * i + 3]});
+            // sum_i c_i prod_k T_{2^k}(z)^(bit k of i): fold the index bits from the lowest
+            for (uint64_t i = 0; i < half; ++i) {
+                const uint32_t *c = basis + 4 * (2 * i + f);
+                v[i] = {c[0], c[1], c[2], c[3]};
+            }
+            Q tk = z;
+            for (uint64_t m = half; m > 1; m >>= 1, tk = q_pi(tk))
+                for (uint64_t j = 0; j < m / 2; ++j) v[j] = qadd(v[2 * j], qmul(tk, v[2 * j + 1]));
+            if (!(lhs == v[0])) return false;
+        }
+    }
+    return true;
+}
+
+} // namespace azk::host
+
+extern "C" {
+
+azk_status azk_host_circle_fft(const uint32_t *coeffs, uint32_t log_n, uint32_t *out)
+{
+    CHECK_ARG(coeffs && out);
+    if (log_n < 1 || log_n > 26) return rt::fail(AZK_ERR_INVALID_LAYOUT, "domain log size %u outside 1..26", log_n);
+    if (!all_canonical(coeffs, (size_t)4 << log_n)) return rt::fail(AZK_ERR_INVALID_LAYOUT, "non-canonical coefficient");
+    circle_fft(coeffs, log_n, out);
+    return AZK_OK;
+}
+
+azk_status azk_host_fold_chain(const uint32_t *evals, uint32_t log_n, const uint32_t *challenges, uint32_t rounds,
+                               uint32_t *out)
+{
+    CHECK_ARG(evals && challenges && out);
+    if (log_n < 1 || log_n > 26 || rounds < 1 || rounds > log_n)
+        return rt::fail(AZK_ERR_INVALID_LAYOUT, "fold chain of %u rounds on 2^%u", rounds, log_n);
+    if (!all_canonical(evals, (size_t)4 << log_n) || !all_canonical(challenges, (size_t)4 * rounds))
+        return rt::fail(AZK_ERR_INVALID_LAYOUT, "non-canonical evaluation or challenge");
+    if (!fold_chain(evals, log_n, challenges, rounds, out)) return rt::fail(AZK_ERR_INVALID_LAYOUT, "zero domain coordinate");
+    return AZK_OK;
+}
+
+azk_status azk_witness_basis_check(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny, uint32_t log_n,
+                                   const uint32_t *basis, int *consistent)
+{
+    CHECK_ARG(basis && consistent && (x || !nx) && (y || !ny));
+    *consistent = 0;
+    if (log_n < 1 || log_n > 26) return rt::fail(AZK_ERR_INVALID_LAYOUT, "domain log size %u outside 1..26", log_n);
+    const uint64_t half = 1ull << (log_n - 1);
+    if (nx > half || ny > half) return rt::fail(AZK_ERR_INVALID_LAYOUT, "more coefficients than the capacity");
+    *consistent = basis_consistent(x, nx, y, ny, log_n, basis) ? 1 : 0;
+    return AZK_OK;
+}
+
+} // extern "C"

@@ -646,6 +646,45 @@ extern "C" azk_status azk_circle_prove_fft_coefficients(azk_circle_prover *p, co
     return prove_single(p, p->coeff_staging.p, 0, proof, cap, len, stats);
 }
 
+// proveCircleFFTCoefficientsResidentVerified (CircleCodeword.swift:1082-1113): the proof
+// must verify and every opened first-layer value must equal the host Circle FFT of the
+// coefficients (the reference rebuilds the whole proof on the CPU).
+static azk_status gate_proof(azk_circle_prover *p, const uint32_t *coeffs, const uint8_t *proof, size_t len)
+{
+    int ok = 0;
+    azk_status s = azk_circle_verify_with_coefficients(proof, len, p->d.public_input_digest, coeffs,
+                                                       (size_t)4 << p->d.log_n, &ok);
+    if (s) return s;
+    if (!ok) return fail(AZK_ERR_CORRECTNESS, "Circle PCS proof rejected by the CPU gate");
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_circle_prove_fft_coefficients_verified(azk_circle_prover *p, const void *coeffs, size_t bytes,
+                                                                 uint8_t *proof, size_t cap, size_t *len,
+                                                                 azk_stats *stats)
+{
+    size_t n = 0;
+    azk_status s = azk_circle_prove_fft_coefficients(p, coeffs, bytes, proof, cap, &n, stats);
+    if (s) return s;
+    if (len) *len = n;
+    return gate_proof(p, (const uint32_t *)coeffs, proof, n);
+}
+
+extern "C" azk_status azk_circle_prove_fft_coefficients_resident_verified(azk_circle_prover *p,
+                                                                          const azk_buffer *coeffs, size_t offset,
+                                                                          uint8_t *proof, size_t cap, size_t *len,
+                                                                          azk_stats *stats)
+{
+    size_t n = 0;
+    azk_status s = azk_circle_prove_fft_coefficients_resident(p, coeffs, offset, proof, cap, &n, stats);
+    if (s) return s;
+    if (len) *len = n;
+    std::vector<uint32_t> host((size_t)4 << p->d.log_n);
+    s = azk_buffer_download(coeffs, offset, host.data(), 4 * host.size());
+    if (s) return s;
+    return gate_proof(p, host.data(), proof, n);
+}
+
 extern "C" azk_status azk_circle_prover_clear(azk_circle_prover *p)
 {
     CHECK_ARG(p);

@@ -4,6 +4,7 @@
 //
 // Plan semantics mirror the reference's Swift plans (cited per entry point in azk.h);
 // buffers, streams and graphs are HIP-native.
+#include "host_gates.hpp"
 #include "runtime_common.hpp"
 #include "host_sha3.hpp"
 
@@ -793,6 +794,23 @@ extern "C" azk_status azk_codeword_execute(azk_codeword_plan *p, const void *coe
     });
 }
 
+// executeVerified (CircleCodeword.swift:476-490): the device codeword against the host
+// Circle FFT of the same coefficients.
+extern "C" azk_status azk_codeword_execute_verified(azk_codeword_plan *p, const void *coeffs, size_t in_bytes,
+                                                    void *evals, size_t out_bytes, azk_stats *stats)
+{
+    azk_status s = azk_codeword_execute(p, coeffs, in_bytes, evals, out_bytes, stats);
+    if (s) return s;
+    const size_t words = (size_t)4 << p->log_n;
+    for (size_t i = 0; i < words; ++i)
+        if (((const uint32_t *)coeffs)[i] >= kP) return fail(AZK_ERR_INVALID_LAYOUT, "non-canonical coefficient");
+    std::vector<uint32_t> want(words);
+    host::circle_fft((const uint32_t *)coeffs, p->log_n, want.data());
+    if (memcmp(want.data(), evals, 4 * words))
+        return fail(AZK_ERR_CORRECTNESS, "Circle codeword device result did not match the CPU gate");
+    return AZK_OK;
+}
+
 extern "C" azk_status azk_codeword_read_twiddles(azk_codeword_plan *p, uint32_t *out, size_t words)
 {
     CHECK_ARG(p && out);
@@ -905,6 +923,24 @@ extern "C" azk_status azk_fold_execute(azk_fold_plan *p, const void *evals, size
     if (s) return s;
     std::lock_guard<std::mutex> g(p->mu);
     HIP_TRY(hipMemcpy(out, p->out.p, ob, hipMemcpyDeviceToHost));
+    return AZK_OK;
+}
+
+// CircleFRIFoldChainPlan.executeVerified (CircleFRIFoldPlan.swift:209-225): the device
+// chain against the host fold chain.
+extern "C" azk_status azk_fold_execute_verified(azk_fold_plan *p, const void *evals, size_t in_bytes,
+                                                const uint32_t *challenges, void *out, size_t out_bytes,
+                                                azk_stats *stats)
+{
+    azk_status s = azk_fold_execute(p, evals, in_bytes, challenges, out, out_bytes, stats);
+    if (s) return s;
+    const size_t words = (size_t)4 << p->log_n, ow = (size_t)4 << (p->log_n - p->rounds);
+    for (size_t i = 0; i < words; ++i)
+        if (((const uint32_t *)evals)[i] >= kP) return fail(AZK_ERR_INVALID_LAYOUT, "non-canonical evaluation");
+    std::vector<uint32_t> want(ow);
+    if (!host::fold_chain((const uint32_t *)evals, p->log_n, challenges, p->rounds, want.data()))
+        return fail(AZK_ERR_INVALID_LAYOUT, "zero domain coordinate");
+    if (memcmp(want.data(), out, 4 * ow)) return fail(AZK_ERR_CORRECTNESS, "fold chain device result did not match the CPU gate");
     return AZK_OK;
 }
 

@@ -10,86 +10,16 @@
 // reported as not accepted (the reference returns false).
 #include <vector>
 
+#include "host_field.hpp"
+#include "host_gates.hpp"
 #include "host_sha3.hpp"
 #include "runtime_common.hpp"
 
 using namespace azk;
 using namespace azk::rt;
+using namespace azk::host;
 
 namespace {
-
-// ---------------------------------------------------------------- field
-uint32_t fadd(uint32_t a, uint32_t b) { uint32_t s = a + b; return s >= kP ? s - kP : s; }
-uint32_t fsub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + (kP - b); }
-uint32_t fmul(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) % kP); }
-uint32_t fpow(uint32_t b, uint32_t e)
-{
-    uint32_t r = 1;
-    for (; e; e >>= 1, b = fmul(b, b))
-        if (e & 1) r = fmul(r, b);
-    return r;
-}
-uint32_t finv(uint32_t a) { return fpow(a, kP - 2); }
-
-struct Q { uint32_t a, b, c, d; };
-bool operator==(const Q &x, const Q &y) { return x.a == y.a && x.b == y.b && x.c == y.c && x.d == y.d; }
-Q qadd(Q x, Q y) { return {fadd(x.a, y.a), fadd(x.b, y.b), fadd(x.c, y.c), fadd(x.d, y.d)}; }
-Q qsub(Q x, Q y) { return {fsub(x.a, y.a), fsub(x.b, y.b), fsub(x.c, y.c), fsub(x.d, y.d)}; }
-// QM31 = CM31[u]/(u^2 - (2 + i)): (A + Bu)(C + Du) = AC + (2 + i)BD + (AD + BC)u
-Q qmul(Q x, Q y)
-{
-    auto cm = [](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, uint32_t &r0, uint32_t &r1) {
-        r0 = fsub(fmul(x0, y0), fmul(x1, y1));
-        r1 = fadd(fmul(x0, y1), fmul(x1, y0));
-    };
-    uint32_t ac0, ac1, bd0, bd1, ad0, ad1, bc0, bc1;
-    cm(x.a, x.b, y.a, y.b, ac0, ac1);
-    cm(x.c, x.d, y.c, y.d, bd0, bd1);
-    cm(x.a, x.b, y.c, y.d, ad0, ad1);
-    cm(x.c, x.d, y.a, y.b, bc0, bc1);
-    const uint32_t n0 = fsub(fadd(bd0, bd0), bd1), n1 = fadd(bd0, fadd(bd1, bd1));
-    return {fadd(ac0, n0), fadd(ac1, n1), fadd(ad0, bc0), fadd(ad1, bc1)};
-}
-bool canonical(const Q &x) { return x.a < kP && x.b < kP && x.c < kP && x.d < kP; }
-
-// ---------------------------------------------------------------- circle domain
-struct Pt { uint32_t x, y; };
-Pt padd(Pt p, Pt q) { return {fsub(fmul(p.x, q.x), fmul(p.y, q.y)), fadd(fmul(p.x, q.y), fmul(p.y, q.x))}; }
-// generator^idx, generator (2, 1268011823) of order 2^31 (CircleDomain.swift:31-60)
-Pt point_at(uint32_t idx)
-{
-    Pt r{1, 0}, g{2, 1268011823u};
-    for (idx &= 0x7fffffffu; idx; idx >>= 1, g = padd(g, g))
-        if (idx & 1) r = padd(r, g);
-    return r;
-}
-uint32_t double_x(uint32_t x) { return fsub(fadd(fmul(x, x), fmul(x, x)), 1); }
-
-struct Domain {
-    uint32_t log_n = 0, init = 0, half_log = 0;
-    // Circle point of storage index s (bit-reversed order): natural i < N/2 is
-    // init + step * i, the upper half the conjugates (CircleDomain.swift:273-331).
-    Pt storage_point(uint64_t s) const
-    {
-        uint64_t nat = 0, v = s;
-        for (uint32_t b = 0; b < log_n; ++b, v >>= 1) nat = (nat << 1) | (v & 1);
-        const uint64_t half = 1ull << half_log, step = 1ull << (31 - half_log);
-        uint32_t idx;
-        if (nat < half) idx = (uint32_t)((init + step * nat) & 0x7fffffffu);
-        else idx = (uint32_t)((0x80000000ull - ((init + step * (nat - half)) & 0x7fffffffu)) & 0x7fffffffu);
-        return point_at(idx);
-    }
-    // Round r's domain coordinate of pair p: y of the pair's left point for r = 0, x of
-    // the left point doubled r - 1 times afterwards (CircleFRILayerOracleV1.inverseDomainLayers,
-    // CircleDomain.swift:469-508); the fold multiplies by its inverse.
-    uint32_t fold_coordinate(uint32_t r, uint64_t p) const
-    {
-        if (r == 0) return storage_point(2 * p).y;
-        uint32_t x = storage_point(p << (r + 1)).x;
-        for (uint32_t k = 1; k < r; ++k) x = double_x(x);
-        return x;
-    }
-};
 
 // ---------------------------------------------------------------- decode
 struct Reader {
@@ -354,6 +284,34 @@ bool verify(const Proof &pf, const uint8_t *expected_digest)
 }
 
 } // namespace
+
+// The proof verifies, and every opened first-layer value equals the Circle FFT of the
+// line-basis coefficients at that leaf: the CPU gate of the prover's *_verified calls.
+extern "C" azk_status azk_circle_verify_with_coefficients(const uint8_t *proof, size_t len,
+                                                          const uint8_t *public_input_digest, const uint32_t *coeffs,
+                                                          size_t coeff_words, int *accepted)
+{
+    CHECK_ARG(proof && public_input_digest && coeffs && accepted);
+    *accepted = 0;
+    Proof pf;
+    azk_status s = decode(proof, len, pf);
+    if (s) return s;
+    const size_t n = (size_t)1 << pf.dom.log_n;
+    if (pf.dom.log_n > 26 || coeff_words != 4 * n)
+        return fail(AZK_ERR_INVALID_LAYOUT, "need %zu coefficient words for the proof's 2^%u domain", 4 * n, pf.dom.log_n);
+    for (size_t i = 0; i < coeff_words; ++i)
+        if (coeffs[i] >= kP) return fail(AZK_ERR_INVALID_LAYOUT, "non-canonical coefficient");
+    if (!verify(pf, public_input_digest)) return AZK_OK;
+    std::vector<uint32_t> cw(4 * n);
+    host::circle_fft(coeffs, pf.dom.log_n, cw.data());
+    auto same = [&](const Opening &o) { return memcmp(&o.value, cw.data() + 4 * o.leaf, 16) == 0; };
+    for (const Query &q : pf.qs)
+        if (!same(q.layers[0].left) || !same(q.layers[0].right)) return AZK_OK;
+    for (const Opening &o : pf.claimed)
+        if (!same(o)) return AZK_OK;
+    *accepted = 1;
+    return AZK_OK;
+}
 
 extern "C" azk_status azk_circle_verify(const uint8_t *proof, size_t len, const uint8_t *public_input_digest,
                                         int *accepted)

@@ -164,6 +164,10 @@ azk_status azk_codeword_execute_resident(azk_codeword_plan *plan, const azk_buff
                                          azk_buffer *out, size_t out_offset, azk_stats *stats);
 azk_status azk_codeword_execute(azk_codeword_plan *plan, const void *coeffs, size_t in_bytes, void *evals,
                                 size_t out_bytes, azk_stats *stats);
+/* executeVerified (:476-490): AZK_ERR_CORRECTNESS unless the evaluations equal the host
+ * Circle FFT of the coefficients (azk_host_circle_fft). */
+azk_status azk_codeword_execute_verified(azk_codeword_plan *plan, const void *coeffs, size_t in_bytes, void *evals,
+                                         size_t out_bytes, azk_stats *stats);
 /* FFT twiddles (N - 1 words) as materialized on the device (readCodewordTwiddles :818-820). */
 azk_status azk_codeword_read_twiddles(azk_codeword_plan *plan, uint32_t *out, size_t words);
 
@@ -184,6 +188,11 @@ azk_status azk_witness_basis_execute_resident(azk_basis_plan *plan, const azk_bu
 /* host arrays: x_count / y_count QM31 (4 LE u32 each); out: 4N words */
 azk_status azk_witness_basis_execute(azk_basis_plan *plan, const uint32_t *x, uint32_t x_count, const uint32_t *y,
                                      uint32_t y_count, uint32_t *out, size_t out_words, azk_stats *stats);
+/* executeVerified (:483-517): AZK_ERR_CORRECTNESS unless azk_witness_basis_check accepts the
+ * result (the reference recomputes the basis with its O(n^2) oracle). */
+azk_status azk_witness_basis_execute_verified(azk_basis_plan *plan, const uint32_t *x, uint32_t x_count,
+                                              const uint32_t *y, uint32_t y_count, uint32_t *out, size_t out_words,
+                                              azk_stats *stats);
 
 /* ---- Circle FRI fold chain with explicit challenges
  * CircleFRIFoldPlan / CircleFRIFoldChainPlan (CircleFRIFoldPlan.swift:5-280). */
@@ -195,6 +204,9 @@ azk_status azk_fold_execute_resident(azk_fold_plan *plan, const azk_buffer *eval
                                      const uint32_t *challenges, azk_buffer *out, size_t out_offset, azk_stats *stats);
 azk_status azk_fold_execute(azk_fold_plan *plan, const void *evals, size_t in_bytes, const uint32_t *challenges,
                             void *out, size_t out_bytes, azk_stats *stats);
+/* executeVerified (CircleFRIFoldPlan.swift:209-225): against azk_host_fold_chain. */
+azk_status azk_fold_execute_verified(azk_fold_plan *plan, const void *evals, size_t in_bytes,
+                                     const uint32_t *challenges, void *out, size_t out_bytes, azk_stats *stats);
 
 /* ---- Circle PCS/FRI commit superstep (the prover)
  * CircleCodewordPCSFRIProverV1 (CircleCodeword.swift:920-1419) and
@@ -226,6 +238,15 @@ azk_status azk_circle_prove_evaluations_resident(azk_circle_prover *prover, cons
 /* Host-array variant: uploads the line-basis coefficients, then proves. */
 azk_status azk_circle_prove_fft_coefficients(azk_circle_prover *prover, const void *coeffs, size_t bytes,
                                              uint8_t *proof, size_t capacity, size_t *proof_len, azk_stats *stats);
+/* proveCircleFFTCoefficientsResidentVerified (CircleCodeword.swift:1082-1113) and the
+ * host-array form: AZK_ERR_CORRECTNESS unless azk_circle_verify_with_coefficients accepts
+ * the proof (the resident form downloads the coefficients for the gate). */
+azk_status azk_circle_prove_fft_coefficients_verified(azk_circle_prover *prover, const void *coeffs, size_t bytes,
+                                                      uint8_t *proof, size_t capacity, size_t *proof_len,
+                                                      azk_stats *stats);
+azk_status azk_circle_prove_fft_coefficients_resident_verified(azk_circle_prover *prover, const azk_buffer *coeffs,
+                                                               size_t offset, uint8_t *proof, size_t capacity,
+                                                               size_t *proof_len, azk_stats *stats);
 /* clearReusableBuffers (CircleCodeword.swift:1238-1253). */
 azk_status azk_circle_prover_clear(azk_circle_prover *prover);
 /* 1 = replay the superstep as a captured hipGraph (default), 0 = eager stream launches. */
@@ -295,6 +316,22 @@ azk_status azk_circle_prover_drift(const azk_circle_prover *prover, azk_drift_st
  * verifies against the 32-byte public-input digest, 0 when it does not; malformed bytes
  * give AZK_ERR_INVALID_LAYOUT (the reference's decode throws). */
 azk_status azk_circle_verify(const uint8_t *proof, size_t len, const uint8_t *public_input_digest, int *accepted);
+/* The same, and every opened first-layer value (query pairs and claimed openings) equals
+ * the host Circle FFT of `coeffs` (4 x 2^logN words, the proof's domain) at its leaf. */
+azk_status azk_circle_verify_with_coefficients(const uint8_t *proof, size_t len, const uint8_t *public_input_digest,
+                                               const uint32_t *coeffs, size_t coeff_words, int *accepted);
+
+/* ---- CPU gates (host code; what the *_verified entry points compare against)
+ * CircleCodewordOracle.evaluateWithCircleFFT (CircleCodeword.swift:72-98): N = 2^log_n
+ * interleaved line-basis coefficients -> N evaluations, canonical domain, storage order. */
+azk_status azk_host_circle_fft(const uint32_t *coeffs, uint32_t log_n, uint32_t *evals);
+/* CircleFRILayerOracleV1.fold (CircleDomain.swift:513-544): rounds x 4 challenge words. */
+azk_status azk_host_fold_chain(const uint32_t *evals, uint32_t log_n, const uint32_t *challenges, uint32_t rounds,
+                               uint32_t *out);
+/* No reference counterpart (its gate is the O(n^2) oracle): *consistent = 1 when the basis
+ * and the monomial columns agree at two pseudo-random QM31 points per column. */
+azk_status azk_witness_basis_check(const uint32_t *x, uint32_t x_count, const uint32_t *y, uint32_t y_count,
+                                   uint32_t log_n, const uint32_t *basis, int *consistent);
 
 /* ---- Batched form: many independent commits of uneven sizes in one captured graph.
  * No reference counterpart (the reference runs one command buffer per job); each
