@@ -11,6 +11,7 @@
 namespace azk {
 
 constexpr int kMaxRounds = 30;
+constexpr uint32_t kMaxGrindingBits = 20;  // CircleFRIGrindingV1.maximumLocalSearchBits
 
 enum TranscriptOpKind : uint32_t { TOP_RESET = 0, TOP_ABSORB = 1, TOP_SQUEEZE_U32 = 2, TOP_LOAD_STATE = 3 };
 

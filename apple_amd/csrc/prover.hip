@@ -169,6 +169,11 @@ azk_status prover_build(azk_circle_prover *p)
     p->query_frame = base_frame(6);
     put32(p->query_frame, d.query_count);
     put64(p->query_frame, n / 2);
+    if (d.grinding_bits) {  // grindingFrame (:956-962)
+        p->grind_frame = base_frame(7);
+        put32(p->grind_frame, d.grinding_bits);
+        put32(p->grind_frame, 8);
+    }
 
     // ---- proof layout, all fields 4-byte multiples (CircleProofFormat.swift:478-535)
     std::vector<uint8_t> head = {0x41, 0x5a, 0x4b, 0x50, 0x46, 0x56, 0x31, 0x00};
@@ -193,6 +198,10 @@ azk_status prover_build(azk_circle_prover *p)
     }
     p->query_words = qw;
     p->proof_words = p->queries_off + (uint64_t)d.query_count * qw;
+    if (d.grinding_bits) {  // tail: u32 0 claimed openings, then the u64 nonce (:516-525)
+        p->nonce_off = p->proof_words + 1;
+        p->proof_words += 3;
+    }
     p->proof_template.assign(p->proof_words, 0);
     memcpy(p->proof_template.data(), head.data(), head.size());
     p->proof_template[nfin_at] = (uint32_t)nfin;
@@ -249,6 +258,7 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
         R.proof_off.push_back(proof_bytes);
         proof_bytes = align_up(proof_bytes + 4 * p->proof_words, 256);
     }
+    R.nonce.assign(jobs.size(), 0);
     azk_status s = R.tail_dev.ensure(tail_bytes);
     if (!s) s = R.qidx_dev.ensure(qidx_bytes);
     if (!s) s = R.proof_dev.ensure(proof_bytes);
@@ -328,9 +338,51 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
     return AZK_OK;
 }
 
-// Host step of one job: final-layer frame + final layer + query frame, then Q indices
-// mod N/2 (CircleFRITranscriptV1.derive, CircleProofFormat.swift:687-722; grinding is 0).
-void host_final(const Runner &R, size_t k)
+// Smallest nonce whose 32-byte squeeze, after absorbing the nonce into state `s`, starts
+// with `bits` zero bits, most significant bit of each byte first
+// (CircleFRITranscriptV1.findGrindingNonce, CircleProofFormat.swift:752-791; digestMeetsTarget
+// :663-681).  Threads take blocks of 4096 nonces in increasing order and stop at the best hit,
+// so every nonce below the result has been tested.
+uint64_t find_nonce(const uint64_t s[25], uint32_t bits, unsigned threads)
+{
+    auto hit = [&](uint64_t nonce) {
+        uint64_t st[25];
+        memcpy(st, s, 200);
+        st[0] ^= nonce;                     // 8 LE nonce bytes
+        st[1] ^= 0x06;                      // SHA3 suffix at byte 8
+        st[16] ^= 0x8000000000000000ull;    // final padding bit at byte 135
+        host::keccak_f(st);
+        uint32_t rem = bits, i = 0;
+        for (; rem >= 8; rem -= 8, ++i)
+            if ((st[i / 8] >> (8 * (i % 8))) & 0xff) return false;
+        const uint32_t byte = (uint32_t)(st[i / 8] >> (8 * (i % 8))) & 0xff;
+        return !rem || (byte & (0xffu << (8 - rem)) & 0xff) == 0;
+    };
+    constexpr uint64_t kBlock = 4096;
+    std::atomic<uint64_t> best{~0ull}, next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const uint64_t lo = next.fetch_add(1) * kBlock;
+            if (lo >= best.load()) return;
+            for (uint64_t n = lo; n < lo + kBlock && n < best.load(std::memory_order_relaxed); ++n)
+                if (hit(n)) {
+                    uint64_t cur = best.load();
+                    while (n < cur && !best.compare_exchange_weak(cur, n)) {}
+                    break;
+                }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    return best.load();
+}
+
+// Host step of one job: final-layer frame + final layer (+ grinding frame and nonce) +
+// query frame, then Q indices mod N/2 (CircleFRITranscriptV1.derive,
+// CircleProofFormat.swift:687-722, grinding :753-794).
+void host_final(Runner &R, size_t k, unsigned threads)
 {
     const azk_circle_prover *p = R.jobs[k].p;
     host::Transcript t;
@@ -338,26 +390,47 @@ void host_final(const Runner &R, size_t k)
     memcpy(t.s, tail, 200);
     t.absorb(p->final_frame.data(), p->final_frame.size());
     t.absorb(tail + 200, 16 * p->nfin);
+    if (p->d.grinding_bits) {
+        t.absorb(p->grind_frame.data(), p->grind_frame.size());
+        const uint64_t nonce = find_nonce(t.s, p->d.grinding_bits, threads);
+        uint8_t nb[8];
+        memcpy(nb, &nonce, 8);
+        t.absorb(nb, 8);
+        R.nonce[k] = nonce;
+    }
     t.absorb(p->query_frame.data(), p->query_frame.size());
     t.squeeze_u32(p->d.query_count, (uint32_t)(p->n / 2), (uint32_t *)((uint8_t *)R.qidx_host.p + R.qidx_off[k]));
 }
 
-void host_finals(const Runner &R)
+void host_finals(Runner &R)
 {
     const size_t jobs = R.jobs.size();
     const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     const size_t nt = std::min(jobs, hw);
+    const unsigned search = (unsigned)std::max<size_t>(1, hw / std::max<size_t>(nt, 1));  // nonce threads per job
     if (nt <= 1) {
-        for (size_t k = 0; k < jobs; ++k) host_final(R, k);
+        for (size_t k = 0; k < jobs; ++k) host_final(R, k, search);
         return;
     }
     std::vector<std::thread> pool;
     for (size_t t = 1; t < nt; ++t)
-        pool.emplace_back([&R, t, nt, jobs]() {
-            for (size_t k = t; k < jobs; k += nt) host_final(R, k);
+        pool.emplace_back([&R, t, nt, jobs, search]() {
+            for (size_t k = t; k < jobs; k += nt) host_final(R, k, search);
         });
-    for (size_t k = 0; k < jobs; k += nt) host_final(R, k);
+    for (size_t k = 0; k < jobs; k += nt) host_final(R, k, search);
     for (auto &th : pool) th.join();
+}
+
+// The grinding nonces go into the proofs after the gather (the emit kernel does not know them).
+void patch_nonces(Runner &R)
+{
+    for (size_t k = 0; k < R.jobs.size(); ++k) {
+        const azk_circle_prover *p = R.jobs[k].p;
+        if (!p->d.grinding_bits) continue;
+        uint32_t *w = (uint32_t *)((uint8_t *)R.proof_host.p + R.proof_off[k]) + p->nonce_off;
+        w[0] = (uint32_t)R.nonce[k];
+        w[1] = (uint32_t)(R.nonce[k] >> 32);
+    }
 }
 
 azk_status enqueue_b(Runner &R, hipStream_t st)
@@ -424,6 +497,7 @@ azk_status run(Runner &R, bool graph, azk_phase_times *phases, azk_stats *stats)
     clk.mark(PH_TAIL);
     HIP_TRY(hipEventRecord(t.b, st));
     HIP_TRY(hipStreamSynchronize(st));
+    patch_nonces(R);
     if (stats) {
         stats->submit_ns = w1 - w0;
         stats->wall_ns = now_ns() - w0;
@@ -492,6 +566,7 @@ void runner_reset(Runner &R)
     R.tail_off.clear();
     R.qidx_off.clear();
     R.proof_off.clear();
+    R.nonce.clear();
 }
 
 size_t prover_footprint(const azk_circle_prover_desc &d)
@@ -513,9 +588,8 @@ azk_status validate_desc(const azk_circle_prover_desc &d)
     if (d.log_blowup == 0 || d.query_count == 0 || d.query_count > 65536 || d.folding_step != 1 ||
         (uint64_t)d.log_blowup * d.query_count + d.grinding_bits > 0xffffffffull)
         return fail(AZK_ERR_INVALID_LAYOUT, "invalid security parameters");
-    if (d.grinding_bits != 0)
-        return fail(AZK_ERR_INVALID_KERNEL_CONFIG, "grinding is not implemented on the device path (grinding_bits = %u)",
-                    d.grinding_bits);
+    if (d.grinding_bits > kMaxGrindingBits)  // CircleFRIGrindingV1.maximumLocalSearchBits (:654)
+        return fail(AZK_ERR_INVALID_LAYOUT, "grinding_bits %u above %u", d.grinding_bits, kMaxGrindingBits);
     return AZK_OK;
 }
 

@@ -43,9 +43,10 @@ struct azk_circle_prover {
     uint32_t *challenges = nullptr;
     uint64_t *tstate = nullptr, *tstate_init = nullptr;
     std::vector<std::pair<uint32_t, uint32_t>> prog_a;  // transcript program of round r: (first op, count)
-    std::vector<uint8_t> final_frame, query_frame;
+    std::vector<uint8_t> final_frame, query_frame, grind_frame;
     std::vector<uint32_t> proof_template;
     uint64_t roots_off = 0, final_off = 0, queries_off = 0, query_words = 0, proof_words = 0;
+    uint64_t nonce_off = 0;  // word of the grinding nonce in the proof (grinding_bits > 0)
     std::vector<uint32_t> layer_off;
     bool use_graph = true, timing = false;
     std::unique_ptr<azk::prv::Runner> runner;  // cached single-job program
@@ -83,6 +84,7 @@ struct Runner {
     DeviceMem tail_dev, qidx_dev, proof_dev;
     PinnedMem tail_host, qidx_host, proof_host;
     std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job
+    std::vector<uint64_t> nonce;                         // grinding nonce per job (grinding_bits > 0)
     hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
     ~Runner()
     {
@@ -97,7 +99,10 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
 // next build.  R must not hold captured graphs.
 void runner_reset(Runner &R);
 azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk);
-void host_finals(const Runner &R);
+// The host step of every job (final-layer absorb, grinding nonce search, query indices).
+void host_finals(Runner &R);
+// Writes the grinding nonces into the gathered proofs (after enqueue_b has completed).
+void patch_nonces(Runner &R);
 azk_status enqueue_b(Runner &R, hipStream_t st);
 const uint8_t *proof_of(const Runner &R, size_t k);
 // One prove through the prover's cached single-job program (rebuilt when the source,

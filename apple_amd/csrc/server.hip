@@ -219,6 +219,7 @@ azk_status run_batch(azk_server *s, std::vector<SJob *> &batch)
     r = enqueue_b(s->runner, st);
     if (r) return r;
     HIP_TRY(hipStreamSynchronize(st));
+    patch_nonces(s->runner);
     for (size_t k = 0; k < batch.size(); ++k) {
         SJob *j = batch[k];
         const uint8_t *pr = proof_of(s->runner, k);

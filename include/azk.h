@@ -220,7 +220,7 @@ typedef struct azk_circle_prover_desc {
     uint32_t log_blowup;
     uint32_t query_count;
     uint32_t folding_step;     /* must be 1 */
-    uint32_t grinding_bits;    /* must be 0 on the device path */
+    uint32_t grinding_bits;    /* 0..20 (maximumLocalSearchBits); the nonce search runs on host threads */
     uint8_t public_input_digest[32];
 } azk_circle_prover_desc;
 typedef struct azk_circle_prover azk_circle_prover;

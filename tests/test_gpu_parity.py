@@ -275,8 +275,26 @@ def test_prover_offsets_timing_and_clear(ctx):
     assert prover.prove_fft_coefficients_resident(buf, 4096) == want
     with pytest.raises(azk.AZKError):
         prover.prove_fft_coefficients_resident(buf, 4100)  # misaligned
-    with pytest.raises(azk.AZKError):
-        azk.CircleCodewordPCSFRIProverV1(ctx, log_n, rounds, (2, 9, 1, 4), digest)  # grinding unsupported
+    with pytest.raises(azk.AZKError) as e:  # above CircleFRIGrindingV1.maximumLocalSearchBits
+        azk.CircleCodewordPCSFRIProverV1(ctx, log_n, rounds, (2, 9, 1, 21), digest)
+    assert e.value.code == 6
+
+
+@pytest.mark.parametrize("log_n,rounds,bits", [(6, 3, 1), (10, 4, 6), (12, 5, 12), (16, 8, 16)])
+def test_prover_grinding_matches_oracle(ctx, log_n, rounds, bits):
+    """The smallest nonce of the reference's sequential search (findGrindingNonce,
+    CircleProofFormat.swift:752-791), found by host threads, in the proof tail."""
+    print(f"grinding 2^{log_n} bits {bits}", flush=True)
+    sec = (2, 9, 1, bits)
+    digest = bytes(0x30 + i for i in range(32))
+    coeffs = inputs.circle_fft_coefficients(seed=700 + bits, log_n=log_n)
+    want = _oracle_proof_from_coeffs(coeffs, log_n, sec, digest, rounds)
+    prover = azk.CircleCodewordPCSFRIProverV1(ctx, log_n, rounds, sec, digest)
+    got = prover.prove_fft_coefficients(coeffs)
+    assert got == want
+    assert azk.circle_verify(got, digest)
+    batch = azk.CircleBatchPlan([prover], [ctx.upload(coeffs)])
+    assert batch.prove() == [want]
 
 
 def test_batch_uneven_jobs_match_plain(ctx):

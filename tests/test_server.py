@@ -115,3 +115,17 @@ def test_server_rejects_and_recovers(ctx):
     srv.close()
     p = azk.CircleCodewordPCSFRIProverV1(ctx, 8, 3, serving.SECURITY, serving.DIGEST)
     assert got == p.prove_fft_coefficients(small)
+
+
+@pytest.mark.gpu
+def test_server_grinding_job(ctx):
+    """A commit with grinding bits through the server equals the plain prover's proof."""
+    sec = (2, 9, 1, 10)
+    coeffs = inputs.circle_fft_coefficients(5, 11)
+    srv = azk.Server(0)
+    t = srv.submit_commit(azk.prover_desc(11, 5, sec, serving.DIGEST), coeffs)
+    got = srv.take(t)
+    srv.close()
+    p = azk.CircleCodewordPCSFRIProverV1(ctx, 11, 5, sec, serving.DIGEST)
+    assert got == p.prove_fft_coefficients(coeffs)
+    assert azk.circle_verify(got, serving.DIGEST)
