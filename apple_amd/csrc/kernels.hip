@@ -41,6 +41,32 @@ __global__ __launch_bounds__(256) void k_keccakf_batch(const uint8_t *__restrict
     for (uint32_t b = 200; b < out_stride; b += 8) *reinterpret_cast<uint2 *>(out + (size_t)i * out_stride + b) = make_uint2(0, 0);
 }
 
+// Grinding search (CircleFRITranscriptV1.findGrindingNonce, CircleProofFormat.swift:752-791):
+// lane i absorbs nonce base + i (8 LE bytes, SHA3 padding) into the state that follows the
+// grinding frame and tests digestMeetsTarget (:663-681) on the first state bytes, most
+// significant bit of each byte first.  The smallest hit of the launch is folded into *best
+// with a 64-bit atomic minimum.  bits <= 32.
+__global__ __launch_bounds__(256) void k_grind(const uint64_t *__restrict__ state, uint32_t bits, uint64_t base,
+                                               unsigned long long *best)
+{
+    const uint64_t nonce = base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    u2 a[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+        const uint64_t v = state[k];
+        a[k] = {(uint32_t)v, (uint32_t)(v >> 32)};
+    }
+    a[0].lo ^= (uint32_t)nonce;
+    a[0].hi ^= (uint32_t)(nonce >> 32);
+    a[1].lo ^= 0x06u;
+    a[16].hi ^= 0x80000000u;
+    kf_v(a);
+    const uint32_t whole = bits >> 3, rem = bits & 7;
+    uint32_t mask = whole >= 4 ? 0xffffffffu : (1u << (8 * whole)) - 1u;  // the zero bytes
+    if (rem && whole < 4) mask |= ((0xffu << (8 - rem)) & 0xffu) << (8 * whole);
+    if ((a[0].lo & mask) == 0) atomicMin(best, (unsigned long long)nonce);
+}
+
 // Loads the first `len` bytes of a message into 34 LE words, zero beyond.
 template <bool kAligned>
 __device__ __forceinline__ void load_message(const uint8_t *__restrict__ p, uint32_t len, uint32_t (&w)[34])
@@ -811,6 +837,11 @@ void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks,
     // 4 waves (one per SIMD): measured faster than 16, whose cooperative levels contend
     // for the CU's LDS crossbar
     hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(256), lds, st, t, n);
+}
+
+void launch_grind(hipStream_t st, const uint64_t *state, uint32_t bits, uint64_t base, uint32_t count, unsigned long long *best)
+{
+    hipLaunchKernelGGL(k_grind, dim3(count / 256), dim3(256), 0, st, state, bits, base, best);
 }
 
 void launch_transcript(hipStream_t st, const TranscriptDesc *t, uint32_t n)

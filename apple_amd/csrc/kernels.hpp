@@ -122,6 +122,9 @@ void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_
 void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
 void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c);
 void launch_transcript(hipStream_t st, const TranscriptDesc *t, uint32_t n);
+// Nonces base .. base + count - 1 (count a multiple of 256) against the grinding state.
+void launch_grind(hipStream_t st, const uint64_t *state, uint32_t bits, uint64_t base, uint32_t count,
+                  unsigned long long *best);
 void launch_copy(hipStream_t st, const CopyDesc *t, uint32_t n, uint32_t blocks);
 void launch_emit_proof(hipStream_t st, const EmitParams *t, uint32_t n, uint32_t blocks);
 

@@ -379,6 +379,33 @@ uint64_t find_nonce(const uint64_t s[25], uint32_t bits, unsigned threads)
     return best.load();
 }
 
+// Nonce search on the device: batches of 2^22 candidates in increasing order until a batch
+// has a hit; that batch's minimum is the smallest nonce.  The stream is the runner's, idle
+// between the two programs.
+constexpr uint32_t kHostGrindBits = 12;  // expected 2^bits candidates: up to 4096 stay on host threads
+constexpr uint32_t kGrindBatch = 1u << 22;
+
+azk_status grind_device(Runner &R, const uint64_t s[25], uint32_t bits, uint64_t &nonce)
+{
+    std::lock_guard<std::mutex> g(R.grind_mu);
+    hipStream_t st = R.ctx->stream;
+    azk_status e = R.grind_dev.ensure(256);
+    if (e) return e;
+    uint64_t *state = (uint64_t *)R.grind_dev.p;
+    unsigned long long *best = (unsigned long long *)((uint8_t *)R.grind_dev.p + 200);
+    unsigned long long none = ~0ull, got = ~0ull;
+    HIP_TRY(hipMemcpyAsync(state, s, 200, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(best, &none, 8, hipMemcpyHostToDevice, st));
+    for (uint64_t base = 0; got == ~0ull; base += kGrindBatch) {
+        launch_grind(st, state, bits, base, kGrindBatch, best);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&got, best, 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    nonce = got;
+    return AZK_OK;
+}
+
 // Host step of one job: final-layer frame + final layer (+ grinding frame and nonce) +
 // query frame, then Q indices mod N/2 (CircleFRITranscriptV1.derive,
 // CircleProofFormat.swift:687-722, grinding :753-794).
@@ -392,7 +419,17 @@ void host_final(Runner &R, size_t k, unsigned threads)
     t.absorb(tail + 200, 16 * p->nfin);
     if (p->d.grinding_bits) {
         t.absorb(p->grind_frame.data(), p->grind_frame.size());
-        const uint64_t nonce = find_nonce(t.s, p->d.grinding_bits, threads);
+        uint64_t nonce = 0;
+        if (p->d.grinding_bits <= kHostGrindBits) {
+            nonce = find_nonce(t.s, p->d.grinding_bits, threads);
+        } else {
+            const azk_status e = grind_device(R, t.s, p->d.grinding_bits, nonce);
+            if (e) {
+                std::lock_guard<std::mutex> g(R.grind_mu);
+                R.grind_err = e;
+                return;
+            }
+        }
         uint8_t nb[8];
         memcpy(nb, &nonce, 8);
         t.absorb(nb, 8);
@@ -402,15 +439,16 @@ void host_final(Runner &R, size_t k, unsigned threads)
     t.squeeze_u32(p->d.query_count, (uint32_t)(p->n / 2), (uint32_t *)((uint8_t *)R.qidx_host.p + R.qidx_off[k]));
 }
 
-void host_finals(Runner &R)
+azk_status host_finals(Runner &R)
 {
+    R.grind_err = AZK_OK;
     const size_t jobs = R.jobs.size();
     const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     const size_t nt = std::min(jobs, hw);
     const unsigned search = (unsigned)std::max<size_t>(1, hw / std::max<size_t>(nt, 1));  // nonce threads per job
     if (nt <= 1) {
         for (size_t k = 0; k < jobs; ++k) host_final(R, k, search);
-        return;
+        return R.grind_err;
     }
     std::vector<std::thread> pool;
     for (size_t t = 1; t < nt; ++t)
@@ -419,6 +457,7 @@ void host_finals(Runner &R)
         });
     for (size_t k = 0; k < jobs; k += nt) host_final(R, k, search);
     for (auto &th : pool) th.join();
+    return R.grind_err;
 }
 
 // The grinding nonces go into the proofs after the gather (the emit kernel does not know them).
@@ -488,7 +527,7 @@ azk_status run(Runner &R, bool graph, azk_phase_times *phases, azk_stats *stats)
     const uint64_t w1 = now_ns();
     HIP_TRY(hipStreamSynchronize(st));
     const uint64_t h0 = now_ns();
-    host_finals(R);
+    if ((s = host_finals(R))) return s;
     const uint64_t h1 = now_ns();
     clk.mark(PH_HOST);
     if (graph) HIP_TRY(hipGraphLaunch(R.graph_b, st));

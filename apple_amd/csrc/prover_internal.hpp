@@ -85,6 +85,9 @@ struct Runner {
     PinnedMem tail_host, qidx_host, proof_host;
     std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job
     std::vector<uint64_t> nonce;                         // grinding nonce per job (grinding_bits > 0)
+    DeviceMem grind_dev;                                 // device nonce search: state + best
+    std::mutex grind_mu;                                 // one device search at a time
+    azk_status grind_err = AZK_OK;
     hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
     ~Runner()
     {
@@ -100,7 +103,7 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
 void runner_reset(Runner &R);
 azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk);
 // The host step of every job (final-layer absorb, grinding nonce search, query indices).
-void host_finals(Runner &R);
+azk_status host_finals(Runner &R);
 // Writes the grinding nonces into the gathered proofs (after enqueue_b has completed).
 void patch_nonces(Runner &R);
 azk_status enqueue_b(Runner &R, hipStream_t st);

@@ -215,7 +215,8 @@ azk_status run_batch(azk_server *s, std::vector<SJob *> &batch)
     if (r) return r;
     admit_commits(s);  // the next batch's uploads overlap this one
     HIP_TRY(hipStreamSynchronize(st));
-    host_finals(s->runner);
+    r = host_finals(s->runner);
+    if (r) return r;
     r = enqueue_b(s->runner, st);
     if (r) return r;
     HIP_TRY(hipStreamSynchronize(st));

@@ -280,10 +280,11 @@ def test_prover_offsets_timing_and_clear(ctx):
     assert e.value.code == 6
 
 
-@pytest.mark.parametrize("log_n,rounds,bits", [(6, 3, 1), (10, 4, 6), (12, 5, 12), (16, 8, 16)])
+@pytest.mark.parametrize("log_n,rounds,bits", [(6, 3, 1), (10, 4, 6), (12, 5, 12), (16, 8, 16), (8, 4, 20)])
 def test_prover_grinding_matches_oracle(ctx, log_n, rounds, bits):
     """The smallest nonce of the reference's sequential search (findGrindingNonce,
-    CircleProofFormat.swift:752-791), found by host threads, in the proof tail."""
+    CircleProofFormat.swift:752-791) in the proof tail: host threads up to 12 bits, the
+    device search above."""
     print(f"grinding 2^{log_n} bits {bits}", flush=True)
     sec = (2, 9, 1, bits)
     digest = bytes(0x30 + i for i in range(32))
