@@ -205,6 +205,11 @@ def load() -> C.CDLL:
         "azk_fold_execute_verified": (st, [vp, vp, sz, vp, vp, sz, sp]),
         "azk_witness_basis_execute_verified": (st, [vp, vp, u32, vp, u32, vp, sz, sp]),
         "azk_host_circle_fft": (st, [vp, u32, vp]),
+        "azk_chain_plan_create": (st, [vp, C.POINTER(ProverDesc), C.POINTER(vp)]),
+        "azk_chain_plan_destroy": (None, [vp]),
+        "azk_chain_execute_resident": (st, [vp, vp, sz, vp, vp, vp, sz, sp]),
+        "azk_chain_execute": (st, [vp, vp, sz, vp, vp, vp, sz, sp]),
+        "azk_chain_execute_verified": (st, [vp, vp, sz, vp, vp, vp, sz, sp]),
         "azk_host_fold_chain": (st, [vp, u32, vp, u32, vp]),
         "azk_witness_basis_check": (st, [vp, u32, vp, u32, u32, vp, C.POINTER(C.c_int)]),
         "azk_circle_prover_set_exec_plan": (st, [vp, C.POINTER(ExecPlan)]),
@@ -591,6 +596,43 @@ class CircleFRIFoldChainPlan(_Plan):
         s = Stats()
         _ck(self.ctx._lib.azk_fold_execute_resident(self.h, evals.h, evals_offset, _ptr(ch), out.h, out_offset, C.byref(s)))
         return _stats_dict(s)
+
+
+class CircleFRIMerkleTranscriptFoldChainPlan(_Plan):
+    """CircleFRIFoldPlan.swift:301-649: per round commit, absorb the root, squeeze the
+    challenge, fold; returns the roots, the challenges and the final layer."""
+    _destroy = "azk_chain_plan_destroy"
+
+    def __init__(self, ctx: Context, log_n: int, rounds: int, security=(2, 32, 1, 0),
+                 public_input_digest: bytes = b""):
+        self.ctx, self.log_n, self.rounds = ctx, log_n, rounds
+        d = prover_desc(log_n, rounds, security, public_input_digest)
+        h = vp()
+        _ck(ctx._lib.azk_chain_plan_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        _track(self)
+        self.last_stats: dict = {}
+
+    def execute(self, evals: np.ndarray, verified: bool = False) -> dict:
+        e = np.ascontiguousarray(evals, dtype=np.uint32).reshape(-1)
+        roots = np.zeros(32 * self.rounds, np.uint8)
+        ch = np.zeros(4 * self.rounds, np.uint32)
+        fin = np.zeros(4 << (self.log_n - self.rounds), np.uint32)
+        s = Stats()
+        fn = self.ctx._lib.azk_chain_execute_verified if verified else self.ctx._lib.azk_chain_execute
+        _ck(fn(self.h, _ptr(e), e.size, _ptr(roots), _ptr(ch), _ptr(fin), fin.size, C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return {"commitments": [roots[32 * r:32 * (r + 1)].tobytes() for r in range(self.rounds)],
+                "challenges": ch.reshape(-1, 4), "values": fin}
+
+    def execute_resident(self, evals: Buffer, out: Buffer, evals_offset: int = 0, out_offset: int = 0) -> dict:
+        roots = np.zeros(32 * self.rounds, np.uint8)
+        ch = np.zeros(4 * self.rounds, np.uint32)
+        s = Stats()
+        _ck(self.ctx._lib.azk_chain_execute_resident(self.h, evals.h, evals_offset, _ptr(roots), _ptr(ch), out.h,
+                                                     out_offset, C.byref(s)))
+        return {"commitments": [roots[32 * r:32 * (r + 1)].tobytes() for r in range(self.rounds)],
+                "challenges": ch.reshape(-1, 4), "stats": _stats_dict(s)}
 
 
 class CircleCodewordPCSFRIProverV1(_Plan):

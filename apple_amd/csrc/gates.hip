@@ -2,6 +2,9 @@
 // and their C-ABI exports (the reference exposes its CPU oracles as public API too:
 // CircleCodewordOracle, CircleFRILayerOracleV1).  Host code only; the device path never
 // calls into this file except to compare after its own result is complete.
+#include <algorithm>
+#include <cstring>
+#include <thread>
 #include <vector>
 
 #include "host_field.hpp"
@@ -79,37 +82,71 @@ void circle_fft(const uint32_t *coeffs, uint32_t log_n, uint32_t *out)
     store_q(v, out);
 }
 
+bool fold_round(const std::vector<Pt> &pts, uint32_t r, const uint32_t *in, uint64_t n_out, const uint32_t ch4[4],
+                uint32_t *out)
+{
+    std::vector<uint32_t> t(n_out);
+    for (uint64_t p = 0; p < n_out; ++p) {  // the pair's domain coordinate, then its inverse
+        if (r == 0) {
+            t[p] = pts[2 * p].y;
+        } else {
+            uint32_t x = pts[p << (r + 1)].x;
+            for (uint32_t k = 1; k < r; ++k) x = double_x(x);
+            t[p] = x;
+        }
+    }
+    for (uint32_t c : t)
+        if (!c) return false;  // inverseDomainLayers refuses a zero coordinate
+    batch_inverse(t);
+    const Q ch{ch4[0], ch4[1], ch4[2], ch4[3]}, inv2{1u << 30, 0, 0, 0};
+    for (uint64_t p = 0; p < n_out; ++p) {  // (l + r + ch ((l - r) t)) / 2, QM31FRIFold.swift:11-44
+        const Q l{in[8 * p], in[8 * p + 1], in[8 * p + 2], in[8 * p + 3]};
+        const Q rr{in[8 * p + 4], in[8 * p + 5], in[8 * p + 6], in[8 * p + 7]};
+        const Q o = qmul(qadd(qadd(l, rr), qmul(ch, qscale(qsub(l, rr), t[p]))), inv2);
+        out[4 * p] = o.a;
+        out[4 * p + 1] = o.b;
+        out[4 * p + 2] = o.c;
+        out[4 * p + 3] = o.d;
+    }
+    return true;
+}
+
 bool fold_chain(const uint32_t *evals, uint32_t log_n, const uint32_t *challenges, uint32_t rounds, uint32_t *out)
 {
-    const Domain dom = Domain::canonical_of(log_n);
-    const std::vector<Pt> pts = dom.storage_points();
-    std::vector<Q> cur = load_q(evals, 1ull << log_n);
-    const Q inv2{1u << 30, 0, 0, 0};
+    const std::vector<Pt> pts = Domain::canonical_of(log_n).storage_points();
+    std::vector<uint32_t> cur(evals, evals + ((size_t)4 << log_n)), next;
     for (uint32_t r = 0; r < rounds; ++r) {
-        const uint64_t n_out = cur.size() / 2;
-        std::vector<uint32_t> t(n_out);
-        for (uint64_t p = 0; p < n_out; ++p) {  // the pair's domain coordinate, then its inverse
-            if (r == 0) {
-                t[p] = pts[2 * p].y;
-            } else {
-                uint32_t x = pts[p << (r + 1)].x;
-                for (uint32_t k = 1; k < r; ++k) x = double_x(x);
-                t[p] = x;
-            }
-        }
-        for (uint32_t c : t)
-            if (!c) return false;  // inverseDomainLayers refuses a zero coordinate
-        batch_inverse(t);
-        const Q ch{challenges[4 * r], challenges[4 * r + 1], challenges[4 * r + 2], challenges[4 * r + 3]};
-        std::vector<Q> next(n_out);
-        for (uint64_t p = 0; p < n_out; ++p) {  // (l + r + ch ((l - r) t)) / 2, QM31FRIFold.swift:11-44
-            const Q l = cur[2 * p], rr = cur[2 * p + 1];
-            next[p] = qmul(qadd(qadd(l, rr), qmul(ch, qscale(qsub(l, rr), t[p]))), inv2);
-        }
+        const uint64_t n_out = cur.size() / 8;
+        next.assign(4 * n_out, 0);
+        if (!fold_round(pts, r, cur.data(), n_out, challenges + 4 * r, next.data())) return false;
         cur.swap(next);
     }
-    store_q(cur, out);
+    memcpy(out, cur.data(), 4 * cur.size());
     return true;
+}
+
+void qm31_merkle_root(const uint32_t *leaves, uint64_t count, uint8_t root[32])
+{
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto parallel = [hw](uint64_t n, auto &&body) {  // body(lo, hi) over [0, n)
+        const unsigned nt = (unsigned)std::min<uint64_t>(hw, std::max<uint64_t>(1, n / 4096));
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nt; ++t) pool.emplace_back([&, t] { body(n * t / nt, n * (t + 1) / nt); });
+        body(0, n / nt);
+        for (auto &th : pool) th.join();
+    };
+    std::vector<uint8_t> level(32 * count), up;
+    parallel(count, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) sha3_256((const uint8_t *)(leaves + 4 * i), 16, level.data() + 32 * i);
+    });
+    for (uint64_t m = count; m > 1; m >>= 1) {
+        up.assign(16 * m, 0);
+        parallel(m / 2, [&](uint64_t lo, uint64_t hi) {
+            for (uint64_t i = lo; i < hi; ++i) sha3_256(level.data() + 64 * i, 64, up.data() + 32 * i);
+        });
+        level.swap(up);
+    }
+    memcpy(root, level.data(), 32);
 }
 
 bool basis_consistent(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny, uint32_t log_n,

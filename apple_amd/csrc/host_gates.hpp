@@ -7,6 +7,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
+#include "host_field.hpp"
+
 namespace azk::host {
 
 // Circle FFT of N = 2^log_n interleaved line-basis coefficients on the canonical
@@ -17,6 +21,15 @@ void circle_fft(const uint32_t *coeffs, uint32_t log_n, uint32_t *out);
 // (CircleFRILayerOracleV1.fold, CircleDomain.swift:513-544): N >> rounds outputs; false
 // on a zero domain coordinate (the reference throws).
 bool fold_chain(const uint32_t *evals, uint32_t log_n, const uint32_t *challenges, uint32_t rounds, uint32_t *out);
+
+// Fold round r of one layer (2 n_out QM31 in, n_out out) with challenge ch, pair coordinates
+// from `pts`, the storage-order points of the original domain; false on a zero coordinate.
+bool fold_round(const std::vector<Pt> &pts, uint32_t r, const uint32_t *in, uint64_t n_out, const uint32_t ch[4],
+                uint32_t *out);
+
+// SHA3-256 Merkle root over `count` (a power of two) 16-byte QM31 leaves: leaf =
+// SHA3(16 bytes), parent = SHA3(left || right) (MerkleOracle.rootSHA3_256, MerkleOracle.swift:4-52).
+void qm31_merkle_root(const uint32_t *leaves, uint64_t count, uint8_t root[32]);
 
 // Whether `basis` (2^log_n interleaved line-basis coefficients) represents the x / y
 // monomial columns: both sides evaluated at `points` pseudo-random QM31 points per column

@@ -23,6 +23,8 @@
 #include <memory>
 #include <thread>
 
+#include "circle_frames.hpp"
+
 #include "host_sha3.hpp"
 #include "prover_internal.hpp"
 
@@ -32,28 +34,10 @@ using namespace azk::prv;
 
 namespace {
 
-void put32(std::vector<uint8_t> &v, uint32_t x) { for (int k = 0; k < 4; ++k) v.push_back((uint8_t)(x >> (8 * k))); }
-void put64(std::vector<uint8_t> &v, uint64_t x) { for (int k = 0; k < 8; ++k) v.push_back((uint8_t)(x >> (8 * k))); }
-
-// Circle transcript V1 base frame (CircleProofFormat.swift:966-973).
-std::vector<uint8_t> base_frame(uint8_t type)
-{
-    static const char dom[] = "AppleZKProver.CircleFRI.PCS.V1";
-    std::vector<uint8_t> f;
-    put32(f, (uint32_t)(sizeof dom - 1));
-    f.insert(f.end(), dom, dom + sizeof dom - 1);
-    put32(f, 1);
-    f.push_back(type);
-    return f;
-}
-
-// 44-byte domain descriptor (CircleDomainDescriptorCodecV1.encode, CircleProofFormat.swift:75-93).
-std::vector<uint8_t> descriptor(uint32_t log_n)
-{
-    std::vector<uint8_t> d = {0x41, 0x5a, 0x4b, 0x43, 0x44, 0x56, 0x31, 0x00};
-    for (uint32_t w : {1u, kP, 31u, 2u, 1268011823u, log_n, 1u << (31 - (log_n + 1)), log_n - 1, 1u}) put32(d, w);
-    return d;
-}
+using frames::base_frame;
+using frames::descriptor;
+using frames::put32;
+using frames::put64;
 
 } // namespace
 
@@ -586,6 +570,44 @@ azk_status prove_single(azk_circle_prover *p, const void *src, int mode, uint8_t
         ++p->drift_samples;
     }
     return AZK_OK;
+}
+
+// The commit / transcript / fold program of one job without the host step and the proof:
+// roots (R x 32 bytes), challenges (R x 4 words) and the final layer (N >> R QM31, copied with
+// `final_kind` to `final_dst`).  Reference: CircleFRIMerkleTranscriptFoldChainPlan
+// (CircleFRIFoldPlan.swift:301-649).
+azk_status chain_single(azk_circle_prover *p, const void *src, int mode, uint8_t *roots, uint32_t *challenges,
+                        void *final_dst, hipMemcpyKind final_kind, azk_stats *stats)
+{
+    if (!p->runner || p->runner->jobs[0].src != src || p->runner->jobs[0].mode != mode) {
+        p->runner.reset();
+        auto r = std::make_unique<Runner>();
+        azk_status s = runner_build(*r, p->ctx, {Job{p, src, mode}});
+        if (s) return s;
+        p->runner = std::move(r);
+    }
+    Runner &R = *p->runner;
+    hipStream_t st = p->ctx->stream;
+    const bool graph = p->use_graph && !p->timing;
+    if (graph) {
+        azk_status s = ensure_graphs(R, st);
+        if (s) return s;
+    }
+    return run_timed(p->ctx, stats, [&]() -> azk_status {
+        if (graph) {
+            HIP_TRY(hipGraphLaunch(R.graph_a, st));
+        } else {
+            PhaseClock none(st, false);
+            azk_status s = enqueue_a(R, st, none);
+            if (s) return s;
+        }
+        const uint32_t rounds = p->d.rounds;
+        for (uint32_t r = 0; r < rounds; ++r)
+            HIP_TRY(hipMemcpyAsync(roots + 32 * r, p->trees[r] + 8 * (2 * (p->n >> r) - 2), 32, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(challenges, p->challenges, 16 * (size_t)rounds, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(final_dst, p->layers[rounds], 16 * p->nfin, final_kind, st));
+        return AZK_OK;
+    });
 }
 
 void runner_reset(Runner &R)

@@ -112,6 +112,10 @@ const uint8_t *proof_of(const Runner &R, size_t k);
 // the mode or the execution plan changed).  Caller holds p->mu and the context lock.
 azk_status prove_single(azk_circle_prover *p, const void *src, int mode, uint8_t *proof, size_t cap, size_t *len,
                         azk_stats *stats);
+// The same program up to the final layer (no host step, no proof): roots, challenges and the
+// final layer.  Caller holds p->mu and the context lock.
+azk_status chain_single(azk_circle_prover *p, const void *src, int mode, uint8_t *roots, uint32_t *challenges,
+                        void *final_dst, hipMemcpyKind final_kind, azk_stats *stats);
 
 // Descriptor rules of CircleFRISecurityParametersV1 / CirclePCSFRIProofV1 plus the
 // device path's limits (folding step 1, no grinding).

@@ -333,6 +333,26 @@ azk_status azk_host_fold_chain(const uint32_t *evals, uint32_t log_n, const uint
 azk_status azk_witness_basis_check(const uint32_t *x, uint32_t x_count, const uint32_t *y, uint32_t y_count,
                                    uint32_t log_n, const uint32_t *basis, int *consistent);
 
+/* ---- Merkle-transcript FRI fold chain
+ * CircleFRIMerkleTranscriptFoldChainPlan (CircleFRIFoldPlan.swift:301-649): per round commit
+ * the layer, absorb the root, squeeze the challenge, fold.  Outputs rounds x 32-byte roots,
+ * rounds x 4 challenge words and the final layer (2^(log_n - rounds) QM31).  The descriptor
+ * is the prover's (its security parameters and digest enter the transcript prefix). */
+typedef struct azk_chain_plan azk_chain_plan;
+azk_status azk_chain_plan_create(azk_context *ctx, const azk_circle_prover_desc *desc, azk_chain_plan **out);
+void azk_chain_plan_destroy(azk_chain_plan *plan);
+/* executeResident (:433-455) */
+azk_status azk_chain_execute_resident(azk_chain_plan *plan, const azk_buffer *evals, size_t evals_offset,
+                                      uint8_t *commitments, uint32_t *challenges, azk_buffer *out, size_t out_offset,
+                                      azk_stats *stats);
+/* execute(evaluations:) (:390-408) */
+azk_status azk_chain_execute(azk_chain_plan *plan, const uint32_t *evals, size_t eval_words, uint8_t *commitments,
+                             uint32_t *challenges, uint32_t *final_layer, size_t final_words, azk_stats *stats);
+/* executeVerified (:410-431): host Merkle roots, host transcript and host folds. */
+azk_status azk_chain_execute_verified(azk_chain_plan *plan, const uint32_t *evals, size_t eval_words,
+                                      uint8_t *commitments, uint32_t *challenges, uint32_t *final_layer,
+                                      size_t final_words, azk_stats *stats);
+
 /* ---- Batched form: many independent commits of uneven sizes in one captured graph.
  * No reference counterpart (the reference runs one command buffer per job); each
  * job's proof is byte-identical to its plain single-job prove.  A batch plan captures
