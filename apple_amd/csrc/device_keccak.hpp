@@ -1,13 +1,16 @@
 // Keccak-f[1600] and fixed-shape SHA3-256 sponges for gfx950.
 //
-// Two forms of the same permutation:
+// Three forms of the same permutation:
 //  * kf_v: one state per lane, each 64-bit lane as a VGPR pair.  Rotations are
 //    v_alignbit_b32 pairs, the 5-way column parities and chi are v_bitop3_b32
 //    (truth tables 0x96 = xor3, chi via the compiler's own bitop3 formation):
-//    ~190 VALU ops per round, 4555 per permutation.
+//    ~190 VALU ops per round, 4555 per permutation.  The throughput form.
+//  * kf_w: one state per wave, one state word per lane (plus copies); DPP and
+//    permlane swaps for theta, one ds_bpermute exchange per round for pi and chi.
+//    The latency form (2.9 us per dependent permutation): Merkle tree tops and the
+//    Fiat-Shamir transcript.
 //  * kf_s: wave-uniform state; the compiler keeps it in SGPRs and runs 64-bit
-//    SALU ops.  Used where one serial sponge sits on the critical path (the
-//    Fiat-Shamir transcript): its latency is several times lower than one VALU wave.
+//    SALU ops.  Used by the sum-check transcript.
 // Semantics follow the reference's SHA3 oracle (Sources/AppleZKProver/SHA3Oracle.swift:
 // 181-280) and its Metal keccak_f1600 (Resources/HashMerkleKernels.metal:308-348);
 // round constants and rho offsets are the FIPS 202 tables.
@@ -114,93 +117,113 @@ __device__ __forceinline__ void kf_v(u2 (&a)[25])
     kround_v(a, 0x80008008u, 0x80000000u);
 }
 
-// ---------------------------------------------------------------- wavefront-cooperative form
-// One state spread over 25 lanes, two states per wave (lanes 0..24 and 32..56); lane
-// l = x + 5y holds A[x, y] as a (lo, hi) VGPR pair, which is also the state's word l
-// in sponge order.  Cross-lane steps pull with ds_bpermute; rho is a per-lane swap +
-// alignbit.  About 45 instructions per round instead of the one-lane form's 190, so one
-// permutation finishes sooner when only a few states are in flight (a transcript step,
-// the top levels of a Merkle tree).  Lanes 25..31 and
-// 57..63 carry a copy of nothing useful and are never read.  Each round takes two
-// exchanges: theta pulls columns x-1 and x+1 over all rows (20 pulls), and pi + chi
-// pull a lane's B[x], B[x+1], B[x+2] straight from their pi sources (6 pulls).
-struct CoopLane {
-    int cm[5], cp[5];    // byte addresses (lane * 4) of A[x-1, k] and A[x+1, k], k = 0..4
-    int pi0, pi1, pi2;   // pi sources of B[x, y], B[x+1, y], B[x+2, y]: A[(X + 3y) % 5, X]
-    uint32_t sh;         // rho: rotl by r = optional half swap, then alignbit by sh
-    bool sw, lead;       // lead: lane 0 of its state (iota)
-    uint32_t word;       // l = x + 5y (25.. for idle lanes)
-};
-
-__device__ __forceinline__ CoopLane coop_lane()
-{
-    // rho offsets r[x + 5y] as (swap, shift) pairs: r >= 32 swaps halves, r' = r mod 32
-    // is alignbit by 32 - r' (r' = 0 flips the swap and shifts by 0).
-    constexpr uint32_t kSwap = 0x00c1d165u;  // bit l: swap for lane l
-    constexpr uint8_t kShift[25] = {0, 31, 2, 4, 5, 28, 20, 26, 9, 12, 29, 22, 21,
-                                    7, 25, 23, 19, 17, 11, 24, 14, 30, 3, 8, 18};
-    const uint32_t lane = __lane_id(), base = lane & 32u, l = lane & 31u;
-    CoopLane c;
-    c.word = l;
-    if (l >= 25) {
-        const int self = (int)(lane * 4);
-        for (int k = 0; k < 5; ++k) c.cm[k] = c.cp[k] = self;
-        c.pi0 = c.pi1 = c.pi2 = self;
-        c.sh = 0;
-        c.sw = false;
-        c.lead = false;
-        return c;
-    }
-    const uint32_t x = l % 5, y = l / 5;
-    auto at = [&](uint32_t xx, uint32_t yy) { return (int)((base + xx % 5 + 5 * (yy % 5)) * 4); };
-    for (uint32_t k = 0; k < 5; ++k) {
-        c.cm[k] = at(x + 4, k);
-        c.cp[k] = at(x + 1, k);
-    }
-    c.pi0 = at(x + 3 * y, x);
-    c.pi1 = at(x + 1 + 3 * y, x + 1);
-    c.pi2 = at(x + 2 + 3 * y, x + 2);
-    c.sh = kShift[l];
-    c.sw = (kSwap >> l) & 1u;
-    c.lead = l == 0;
-    return c;
-}
-
 __device__ __forceinline__ uint32_t bperm(int addr, uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);
 }
 
-__device__ __forceinline__ void kf_coop(uint32_t &lo, uint32_t &hi, const CoopLane &c)
+// ---------------------------------------------------------------- one state per wave
+// Lane l = 8y + p holds A[x(p), y] for y < 5, where x(p) = 4, 0, 1, 2, 3, 4, 0, 1 for
+// p = 0..7: positions 1..5 are the state, 0, 6 and 7 hold copies of columns 4, 0 and 1 so
+// that every column neighbour a real lane needs sits one lane away inside its 8-lane
+// group.  Lanes 40..63 hold zero.  Per round:
+//  * theta: column parities by an xor all-reduce over the eight groups (DPP row_ror:8,
+//    then v_permlane16_swap and v_permlane32_swap), C[x-1] and C[x+1] by DPP row_shr:1
+//    and row_shl:1 -- no LDS traffic;
+//  * rho at the source lane (per-lane half swap + alignbit);
+//  * pi and chi's two row neighbours in one ds_bpermute exchange: every lane (copies
+//    included) pulls B[x], B[x+1], B[x+2] of its row straight from their pi sources;
+//  * the zero lanes are re-zeroed, iota goes to both lanes that hold A[0, 0].
+// One LDS round trip per round: 2.9 us per permutation on one wave, against 5.2 us for a
+// 25-lane layout that also pulls theta's columns with ds_bpermute (two round trips) and
+// ~9 us for the one-lane form.
+struct WaveLane {
+    int s0, s1, s2;   // byte addresses of the pi sources of B[x, y], B[x+1, y], B[x+2, y]
+    uint32_t sh;      // rho: optional half swap, then alignbit by sh
+    uint32_t keep;    // ~0 on lanes 0..39, 0 on the zero lanes
+    uint32_t lead;    // ~0 on the two lanes holding A[0, 0]
+    uint32_t rcl, rch;  // round constant lane % 24 (read back per round with readlane)
+    bool sw;
+    bool own;         // the lane is its word's position 1..5 (not a copy)
+    uint32_t word;    // state word x + 5y held by this lane (25 on the zero lanes)
+};
+
+// Lane of the state's own copy of word w (position 1 + x of group y).
+__device__ __forceinline__ uint32_t wave_lane_of(uint32_t w) { return 8u * (w / 5u) + 1u + w % 5u; }
+
+__device__ __forceinline__ WaveLane wave_lane()
+{
+    constexpr uint32_t kSwap = 0x00c1d165u;  // bit w: rho swaps the halves of word w
+    constexpr uint8_t kShift[25] = {0, 31, 2, 4, 5, 28, 20, 26, 9, 12, 29, 22, 21,
+                                    7, 25, 23, 19, 17, 11, 24, 14, 30, 3, 8, 18};
+    constexpr uint32_t kX = 0x10432104u;     // x(p) in nibble p
+    const uint32_t lane = __lane_id(), y = lane >> 3, x = (kX >> (4 * (lane & 7u))) & 15u;
+    WaveLane c;
+    c.rcl = kRcLo[lane % 24u];
+    c.rch = kRcHi[lane % 24u];
+    if (y >= 5) {
+        c.s0 = c.s1 = c.s2 = (int)(lane * 4);
+        c.sh = 0;
+        c.keep = c.lead = 0;
+        c.sw = false;
+        c.own = false;
+        c.word = 25;
+        return c;
+    }
+    // B[X, y] = rho(A)[(X + 3y) mod 5, X]
+    auto src = [&](uint32_t k) {
+        const uint32_t X = (x + k) % 5u;
+        return (int)(4 * (8 * X + 1 + (X + 3 * y) % 5u));
+    };
+    c.s0 = src(0);
+    c.s1 = src(1);
+    c.s2 = src(2);
+    c.word = x + 5 * y;
+    c.sh = kShift[c.word];
+    c.sw = (kSwap >> c.word) & 1u;
+    c.keep = ~0u;
+    c.lead = c.word == 0 ? ~0u : 0u;
+    c.own = (lane & 7u) - 1u < 5u;
+    return c;
+}
+
+// Column parities of both halves over the eight groups.  Rows of 16 lanes hold groups
+// (0, 1), (2, 3), (4, 5), (6, 7); row_ror:8 adds the two groups of a row, the first
+// permlane16 swap pairs rows (0, 1) and (2, 3) of the low halves with those of the high
+// halves, leaving [lo 0..3, hi 0..3, lo 4..7, hi 4..7] by row; the permlane32 swap adds
+// rows 0 + 2 and 1 + 3, and the last permlane16 swap spreads lo and hi to every row.
+__device__ __forceinline__ void wave_parity(uint32_t lo, uint32_t hi, uint32_t &cl, uint32_t &ch)
+{
+    const uint32_t ul = lo ^ __builtin_amdgcn_mov_dpp(lo, 0x128, 0xf, 0xf, false);  // row_ror:8
+    const uint32_t uh = hi ^ __builtin_amdgcn_mov_dpp(hi, 0x128, 0xf, 0xf, false);
+    const auto a = __builtin_amdgcn_permlane16_swap(ul, uh, false, false);
+    const uint32_t x = a[0] ^ a[1];
+    const auto b = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    const uint32_t z = b[0] ^ b[1];
+    const auto d = __builtin_amdgcn_permlane16_swap(z, z, false, false);
+    cl = d[0];
+    ch = d[1];
+}
+
+__device__ __forceinline__ void kf_w(uint32_t &lo, uint32_t &hi, const WaveLane &c)
 {
 #pragma unroll 2
     for (int r = 0; r < 24; ++r) {
-        // theta: columns x-1 and x+1 over all five rows in one exchange
-        uint32_t ml[5], mh[5], pl[5], ph[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            ml[k] = bperm(c.cm[k], lo);
-            mh[k] = bperm(c.cm[k], hi);
-            pl[k] = bperm(c.cp[k], lo);
-            ph[k] = bperm(c.cp[k], hi);
-        }
-        const uint32_t cml = xor3(xor3(ml[0], ml[1], ml[2]), ml[3], ml[4]);
-        const uint32_t cmh = xor3(xor3(mh[0], mh[1], mh[2]), mh[3], mh[4]);
-        const uint32_t cpl = xor3(xor3(pl[0], pl[1], pl[2]), pl[3], pl[4]);
-        const uint32_t cph = xor3(xor3(ph[0], ph[1], ph[2]), ph[3], ph[4]);
+        uint32_t cl, ch;
+        wave_parity(lo, hi, cl, ch);
+        const uint32_t cml = __builtin_amdgcn_mov_dpp(cl, 0x111, 0xf, 0xf, true);  // row_shr:1
+        const uint32_t cmh = __builtin_amdgcn_mov_dpp(ch, 0x111, 0xf, 0xf, true);
+        const uint32_t cpl = __builtin_amdgcn_mov_dpp(cl, 0x101, 0xf, 0xf, true);  // row_shl:1
+        const uint32_t cph = __builtin_amdgcn_mov_dpp(ch, 0x101, 0xf, 0xf, true);
         lo = xor3(lo, cml, __builtin_amdgcn_alignbit(cpl, cph, 31));
         hi = xor3(hi, cmh, __builtin_amdgcn_alignbit(cph, cpl, 31));
-        // rho at the source lane; pi and chi's two row neighbours in one exchange
         const uint32_t h2 = c.sw ? lo : hi, l2 = c.sw ? hi : lo;
         const uint32_t th = __builtin_amdgcn_alignbit(h2, l2, c.sh), tl = __builtin_amdgcn_alignbit(l2, h2, c.sh);
-        const uint32_t b0l = bperm(c.pi0, tl), b1l = bperm(c.pi1, tl), b2l = bperm(c.pi2, tl);
-        const uint32_t b0h = bperm(c.pi0, th), b1h = bperm(c.pi1, th), b2h = bperm(c.pi2, th);
-        lo = b0l ^ (~b1l & b2l);
-        hi = b0h ^ (~b1h & b2h);
-        if (c.lead) {  // iota
-            lo ^= kRcLo[r];
-            hi ^= kRcHi[r];
-        }
+        const uint32_t b0l = bperm(c.s0, tl), b1l = bperm(c.s1, tl), b2l = bperm(c.s2, tl);
+        const uint32_t b0h = bperm(c.s0, th), b1h = bperm(c.s1, th), b2h = bperm(c.s2, th);
+        const uint32_t rl = __builtin_amdgcn_readlane(c.rcl, r), rh = __builtin_amdgcn_readlane(c.rch, r);
+        lo = ((b0l ^ (~b1l & b2l)) & c.keep) ^ (rl & c.lead);
+        hi = ((b0h ^ (~b1h & b2h)) & c.keep) ^ (rh & c.lead);
     }
 }
 

@@ -155,47 +155,51 @@ __device__ __forceinline__ void ld_node(const uint32_t *p, uint32_t (&d)[8])
     d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
 }
 
-// Parents q < half of the level-l nodes in `src`, two per wave with the wavefront-
-// cooperative permutation (state lanes l < 8 load the 64-byte message, lanes 0..3 hold
-// the digest).  Waves with no parent skip as a whole.
-__device__ __forceinline__ void coop_parents(const uint32_t *src, uint32_t *dst, uint32_t half, int l, uint64_t g,
-                                             uint32_t *tree, uint64_t m, const CoopLane &c)
+// Parents q < half of the level-l nodes in `src`, one per wave at a time with the
+// one-state-per-wave permutation (lanes holding words < 8 load the 64-byte message, the
+// own lanes of words 0..3 write the digest).  The loop is wave-uniform.
+__device__ __forceinline__ void wave_parents(const uint32_t *src, uint32_t *dst, uint32_t half, int l, uint64_t g,
+                                             uint32_t *tree, uint64_t m, const WaveLane &c)
 {
-    const uint32_t wave = threadIdx.x >> 6, q = 2 * wave + ((__lane_id() >> 5) & 1u);
-    if (2 * wave >= half) return;
-    const bool live = q < half;
-    uint32_t lo = 0, hi = 0;
-    if (live && c.word < 8) {
-        lo = src[16 * q + 2 * c.word];
-        hi = src[16 * q + 2 * c.word + 1];
-    }
-    if (c.word == 8) lo = 0x06u;          // SHA3 padding after 64 bytes
-    if (c.word == 16) hi = 0x80000000u;   // final bit of the 136-byte block
-    kf_coop(lo, hi, c);
-    if (live && c.word < 4) {
-        dst[8 * q + 2 * c.word] = lo;
-        dst[8 * q + 2 * c.word + 1] = hi;
-        uint32_t *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
-        reinterpret_cast<uint2 *>(node)[c.word] = make_uint2(lo, hi);
+    const uint32_t waves = blockDim.x >> 6;
+    for (uint32_t q = threadIdx.x >> 6; q < half; q += waves) {
+        uint32_t lo = 0, hi = 0;
+        if (c.word < 8) {
+            lo = src[16 * q + 2 * c.word];
+            hi = src[16 * q + 2 * c.word + 1];
+        }
+        if (c.word == 8) lo = 0x06u;          // SHA3 padding after 64 bytes
+        if (c.word == 16) hi = 0x80000000u;   // final bit of the 136-byte block
+        kf_w(lo, hi, c);
+        if (c.own && c.word < 4) {
+            dst[8 * q + 2 * c.word] = lo;
+            dst[8 * q + 2 * c.word + 1] = hi;
+            uint32_t *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
+            reinterpret_cast<uint2 *>(node)[c.word] = make_uint2(lo, hi);
+        }
     }
 }
 
 // Reduces `cnt` level-`l` nodes held in LDS region A (node q at words 8q..8q+7),
 // whose first node has global index `g` in that level, up to one node.  Every
 // computed level is written to the tree in HBM.  Ping-pong A -> B -> A ... with one
-// barrier per level (B holds cnt/2 nodes).  Levels with at most two parents per wave
-// (the serial top of the tree) use the cooperative permutation: ~5 us per dependent
-// permutation instead of ~9 us for one lane.
+// barrier per level (B holds cnt/2 nodes).  Levels with at most kWaveParents parents per
+// wave (the serial top of the tree) use the one-state-per-wave permutation, 2.9 us per
+// dependent permutation against ~9 us for a level in the one-lane form.
+#ifndef AZK_WAVE_PARENTS
+#define AZK_WAVE_PARENTS 2
+#endif
+constexpr uint32_t kWaveParents = AZK_WAVE_PARENTS;
 __device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t *B, uint32_t cnt, int l, uint64_t g, uint32_t *tree,
                                            uint64_t m)
 {
     uint32_t *src = A, *dst = B;
-    const uint32_t coop_max = 2 * (blockDim.x >> 6);
-    const CoopLane c = coop_lane();
+    const uint32_t wave_max = kWaveParents * (blockDim.x >> 6);
+    const WaveLane c = wave_lane();
     while (cnt > 1) {
         uint32_t half = cnt >> 1;
-        if (half <= coop_max) {
-            coop_parents(src, dst, half, l, g, tree, m, c);
+        if (half <= wave_max) {
+            wave_parents(src, dst, half, l, g, tree, m, c);
         } else {
             for (uint32_t q = threadIdx.x; q < half; q += blockDim.x) {
                 uint32_t lft[8], rgt[8], o[8];
@@ -292,7 +296,10 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
 // Upper Merkle levels: each block loads 2^log_c nodes of `level` and reduces them
 // by log_c levels (MerkleCommitter.swift:853-893: parents per level + fused upper tree).
 // A block of a job's last step then runs the job's transcript program for that root.
-__global__ __launch_bounds__(256) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
+#ifndef AZK_TREE_THREADS
+#define AZK_TREE_THREADS 256
+#endif
+__global__ __launch_bounds__(AZK_TREE_THREADS) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
     uint32_t chunk;
@@ -520,6 +527,10 @@ __global__ __launch_bounds__(256) void k_fold(const FoldDesc *__restrict__ tab, 
     extern __shared__ uint32_t lds[];
     uint32_t tile;
     const FoldDesc &d = find_desc(tab, ntab, tile);
+    if (d.t_ops) {  // block-uniform: a transcript block, no barrier below is reached
+        if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
+        return;
+    }
     const uint32_t T = 1u << d.log_t;
     const uint64_t base = (uint64_t)tile * T;
     const uint32_t *ch = d.ch;
@@ -564,6 +575,10 @@ __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restric
     __shared__ uint32_t A[8 * 256], B[4 * 256];
     uint32_t tile;
     const FoldDesc &d = find_desc(tab, ntab, tile);
+    if (d.t_ops) {  // block-uniform: a transcript block, no barrier below is reached
+        if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
+        return;
+    }
     const uint32_t T = 1u << d.log_t;
     const uint64_t base = (uint64_t)tile * T;
     const uint32_t *ch = d.ch;
@@ -589,11 +604,11 @@ __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restric
 
 // ============================================================== transcript
 
-// Transcript sponge on the cooperative permutation: lane l < 25 of the wave holds state
-// word l.  Absorbs `len` bytes at `src` (8-byte aligned, readable up to the next
+// Transcript sponge on the one-state-per-wave permutation: every lane holding state word w
+// (its own lane and any copy) applies the same message word.  Absorbs `len` bytes at `src` (8-byte aligned, readable up to the next
 // multiple of 8) as one SHA3-padded message.  Reference: TranscriptState.absorb
 // (SHA3Oracle.swift:19-39), Metal transcript_absorb_keccak :1324-1366.
-__device__ __forceinline__ void t_absorb(uint32_t &lo, uint32_t &hi, const CoopLane &c, const uint64_t *src,
+__device__ __forceinline__ void t_absorb(uint32_t &lo, uint32_t &hi, const WaveLane &c, const uint64_t *src,
                                          uint64_t len, uint32_t suffix)
 {
     const bool in_rate = c.word < 17;
@@ -604,7 +619,7 @@ __device__ __forceinline__ void t_absorb(uint32_t &lo, uint32_t &hi, const CoopL
             lo ^= (uint32_t)w;
             hi ^= (uint32_t)(w >> 32);
         }
-        kf_coop(lo, hi, c);
+        kf_w(lo, hi, c);
         off += 136;
     }
     const uint32_t tc = (uint32_t)(len - off);
@@ -620,20 +635,20 @@ __device__ __forceinline__ void t_absorb(uint32_t &lo, uint32_t &hi, const CoopL
         lo ^= (uint32_t)w;
         hi ^= (uint32_t)(w >> 32);
     }
-    kf_coop(lo, hi, c);
+    kf_w(lo, hi, c);
 }
 
 // Rejection-sampled u32 words mod `modulus` from a copy of the state, 34 words per
 // block.  Reference: TranscriptState.squeezeUInt32 (SHA3Oracle.swift:41-70).
-__device__ __forceinline__ void t_squeeze(uint32_t lo, uint32_t hi, const CoopLane &c, uint32_t count,
+__device__ __forceinline__ void t_squeeze(uint32_t lo, uint32_t hi, const WaveLane &c, uint32_t count,
                                           uint32_t modulus, uint32_t *dst)
 {
     const uint64_t limit = (1ull << 32) - ((1ull << 32) % modulus);
     uint32_t got = 0, cand = 0;
     while (got < count) {
-        if (cand > 0 && cand % 34 == 0) kf_coop(lo, hi, c);
-        const uint32_t wi = cand % 34;
-        const uint32_t wl = __builtin_amdgcn_readlane(lo, wi >> 1), wh = __builtin_amdgcn_readlane(hi, wi >> 1);
+        if (cand > 0 && cand % 34 == 0) kf_w(lo, hi, c);
+        const uint32_t wi = cand % 34, at = wave_lane_of(wi >> 1);
+        const uint32_t wl = __builtin_amdgcn_readlane(lo, at), wh = __builtin_amdgcn_readlane(hi, at);
         const uint64_t w = (wi & 1) ? wh : wl;
         ++cand;
         if (w < limit) {
@@ -644,11 +659,12 @@ __device__ __forceinline__ void t_squeeze(uint32_t lo, uint32_t hi, const CoopLa
 }
 
 // Runs a transcript program on its job's persistent 200-byte state with one whole wave
-// (lanes 0..24 hold the state words; lanes 32..56 compute an unused second state).
+// (the one-state-per-wave layout: every lane holding a state word loads it, the own lanes
+// store it back).
 __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops)
 {
-    const CoopLane c = coop_lane();
-    const bool holds = c.word < 25 && __lane_id() < 32;
+    const WaveLane c = wave_lane();
+    const bool holds = c.word < 25;
     uint32_t lo = 0, hi = 0;
     if (holds) {
         const uint64_t v = state[c.word];
@@ -671,7 +687,7 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
             t_squeeze(lo, hi, c, (uint32_t)op.n, op.modulus, reinterpret_cast<uint32_t *>(op.dst));
         }
     }
-    if (holds) state[c.word] = (uint64_t)lo | ((uint64_t)hi << 32);
+    if (c.own) state[c.word] = (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 __global__ __launch_bounds__(64) void k_transcript(const TranscriptDesc *__restrict__ tab, uint32_t ntab)
@@ -834,9 +850,8 @@ void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks,
 {
     if (!blocks) return;
     size_t lds = (size_t)(8u << max_log_c) * 4 + (size_t)(4u << max_log_c) * 4;
-    // 4 waves (one per SIMD): measured faster than 16, whose cooperative levels contend
-    // for the CU's LDS crossbar
-    hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(256), lds, st, t, n);
+    // 4 waves (one per SIMD): 8-wave blocks with 16-node chunks measured no faster
+    hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(AZK_TREE_THREADS), lds, st, t, n);
 }
 
 void launch_grind(hipStream_t st, const uint64_t *state, uint32_t bits, uint64_t base, uint32_t count, unsigned long long *best)

@@ -47,7 +47,8 @@ struct FftFinalDesc {
 };
 
 // One FRI fold round of one job (2 n_out -> n_out), fused with the new layer's
-// leaf hashes and tile subtrees (if `tree`).
+// leaf hashes and tile subtrees (if `tree`).  A descriptor with `t_ops` set is instead
+// one block that runs that transcript program on its first wave, beside the fold.
 struct FoldDesc {
     const uint4 *in;
     uint4 *out;
@@ -56,6 +57,9 @@ struct FoldDesc {
     uint32_t *tree;
     uint64_t n_out;
     uint32_t log_t, first_block;
+    const struct TranscriptOp *t_ops;
+    uint64_t *t_state;
+    uint32_t t_nops, pad;
 };
 
 // Reduction of chunks of 2^log_c nodes of `level` up by log_c levels.

@@ -43,6 +43,7 @@ struct azk_circle_prover {
     uint32_t *challenges = nullptr;
     uint64_t *tstate = nullptr, *tstate_init = nullptr;
     std::vector<std::pair<uint32_t, uint32_t>> prog_a;  // transcript program of round r: (first op, count)
+    std::vector<std::pair<uint32_t, uint32_t>> prog_c;  // commitment frame r + 1, run beside fold r
     std::vector<uint8_t> final_frame, query_frame, grind_frame;
     std::vector<uint32_t> proof_template;
     uint64_t roots_off = 0, final_off = 0, queries_off = 0, query_words = 0, proof_words = 0;

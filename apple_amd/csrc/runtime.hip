@@ -73,10 +73,11 @@ std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t l
 azk_status check_exec_plan(const azk_circle_exec_plan &x)
 {
     if (x.log_tile_commit < 6 || x.log_tile_commit > kDefaultLogTile || x.log_tile_fold < 6 ||
-        x.log_tile_fold > kDefaultLogTile || x.log_tile_leafwise < 5 || x.log_tile_leafwise > kLeafwiseLogTile ||
-        x.log_leafwise_below > 31 || x.log_tree_chunk < 3 || x.log_tree_chunk > kUpperLogChunk)
-        return fail(AZK_ERR_INVALID_KERNEL_CONFIG,
-                    "execution plan outside the supported tiles (commit/fold 6..11, leafwise 5..8, chunk 3..10)");
+        x.log_tile_fold > kDefaultLogTile || x.log_tile_leafwise < kLeafwiseMinLogTile ||
+        x.log_tile_leafwise > kLeafwiseLogTile || x.log_leafwise_below > 31 || x.log_tree_chunk < 3 ||
+        x.log_tree_chunk > kUpperLogChunk || (x.log_leafwise_grid && (x.log_leafwise_grid < 8 || x.log_leafwise_grid > 20)))
+        return fail(AZK_ERR_INVALID_KERNEL_CONFIG, "execution plan outside the supported tiles (commit/fold 6..11, "
+                                                   "leafwise 3..8, chunk 3..10, leafwise grid 0 or 8..20)");
     return AZK_OK;
 }
 
@@ -88,6 +89,7 @@ ExecPlan to_exec_plan(const azk_circle_exec_plan &x)
     e.log_leafwise_below = x.log_leafwise_below;
     e.log_tile_leafwise = x.log_tile_leafwise;
     e.log_tree_chunk = x.log_tree_chunk;
+    e.log_leafwise_grid = x.log_leafwise_grid;
     return e;
 }
 
@@ -99,6 +101,7 @@ azk_circle_exec_plan from_exec_plan(const ExecPlan &e)
     x.log_leafwise_below = e.log_leafwise_below;
     x.log_tile_leafwise = e.log_tile_leafwise;
     x.log_tree_chunk = e.log_tree_chunk;
+    x.log_leafwise_grid = e.log_leafwise_grid;
     return x;
 }
 
@@ -152,11 +155,16 @@ void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &fi
 
 uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in, void *out, const uint32_t *tw,
                    const uint32_t *ch, uint64_t n_out, uint32_t log_t, uint32_t *tree, uint32_t log_leafwise_below,
-                   uint32_t log_t_leaf)
+                   uint32_t log_t_leaf, uint32_t log_grid)
 {
     const uint32_t log_out = ilog2(n_out);
     if (tree && log_out < log_leafwise_below) {
-        const uint32_t lt = std::min(log_t_leaf, log_out);
+        uint32_t lt = log_t_leaf;
+        if (log_grid) {  // small tiles for small layers: fewer in-block levels in the one-lane form
+            const uint32_t want = log_out > log_grid ? log_out - log_grid : 0;
+            lt = std::min(log_t_leaf, std::max(kLeafwiseMinLogTile, want));
+        }
+        lt = std::min(lt, log_out);
         leaf.add(FoldDesc{(const uint4 *)in, (uint4 *)out, tw, ch, tree, n_out, lt, 0}, (uint32_t)(n_out >> lt), lt);
         return lt;
     }

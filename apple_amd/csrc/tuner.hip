@@ -69,7 +69,7 @@ std::string plan_json(const ExecPlan &x)
     std::ostringstream o;
     o << "{\"log_tile_commit\": " << x.log_tile_commit << ", \"log_tile_fold\": " << x.log_tile_fold
       << ", \"log_leafwise_below\": " << x.log_leafwise_below << ", \"log_tile_leafwise\": " << x.log_tile_leafwise
-      << ", \"log_tree_chunk\": " << x.log_tree_chunk << "}";
+      << ", \"log_tree_chunk\": " << x.log_tree_chunk << ", \"log_leafwise_grid\": " << x.log_leafwise_grid << "}";
     return o.str();
 }
 
@@ -106,7 +106,7 @@ bool same_plan(const ExecPlan &a, const ExecPlan &b)
 {
     return a.log_tile_commit == b.log_tile_commit && a.log_tile_fold == b.log_tile_fold &&
            a.log_leafwise_below == b.log_leafwise_below && a.log_tile_leafwise == b.log_tile_leafwise &&
-           a.log_tree_chunk == b.log_tree_chunk;
+           a.log_tree_chunk == b.log_tree_chunk && a.log_leafwise_grid == b.log_leafwise_grid;
 }
 
 double quantile(std::vector<double> v, double q)
@@ -211,7 +211,8 @@ bool lookup(const azk_tuning_config &c, const azk_circle_prover *p, ExecPlan &pl
         if (read_field(line, "log_tile_commit", x.log_tile_commit) && read_field(line, "log_tile_fold", x.log_tile_fold) &&
             read_field(line, "log_leafwise_below", x.log_leafwise_below) &&
             read_field(line, "log_tile_leafwise", x.log_tile_leafwise) &&
-            read_field(line, "log_tree_chunk", x.log_tree_chunk) && !check_exec_plan(from_exec_plan(x))) {
+            read_field(line, "log_tree_chunk", x.log_tree_chunk) &&
+            read_field(line, "log_leafwise_grid", x.log_leafwise_grid) && !check_exec_plan(from_exec_plan(x))) {
             plan = x;
             median = read_field(line, "median_gpu_ns", med) ? med : 0;
             found = true;
@@ -277,6 +278,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
         {&ExecPlan::log_tile_commit, {9, 10, 11}},
         {&ExecPlan::log_leafwise_below, {17, 18, 19, 20, 21}},
         {&ExecPlan::log_tile_leafwise, {6, 7, 8}},
+        {&ExecPlan::log_leafwise_grid, {0, 8, 10, 11}},
     };
     uint32_t raced = 1;
     for (const Knob &k : knobs) {

@@ -274,8 +274,12 @@ typedef struct azk_circle_exec_plan {
     uint32_t log_tile_commit;     /* final FFT pass + first-layer treelets: 2^6..2^11 leaves per block (11) */
     uint32_t log_tile_fold;       /* committed folds, big-tile form: 2^6..2^11 outputs per block (11) */
     uint32_t log_leafwise_below;  /* committed layers below 2^this fold one leaf per thread (19) */
-    uint32_t log_tile_leafwise;   /* 2^5..2^8 leaves per block in that form (8) */
-    uint32_t log_tree_chunk;      /* upper Merkle levels reduced per block, 3..10 (4) */
+    uint32_t log_tile_leafwise;   /* at most 2^3..2^8 leaves per block in that form (8) */
+    uint32_t log_tree_chunk;      /* upper Merkle levels reduced per block, 3..10 (3) */
+    uint32_t log_leafwise_grid;   /* a leafwise layer of 2^k outputs takes tiles of 2^(k - this)
+                                     leaves (2^(k - this + ceil(log2 J)) in a batch of J jobs),
+                                     clamped to 2^3..2^log_tile_leafwise; 0 = fixed tile, else
+                                     8..20 (9) */
 } azk_circle_exec_plan;
 azk_status azk_circle_prover_set_exec_plan(azk_circle_prover *prover, const azk_circle_exec_plan *plan);
 azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *prover, azk_circle_exec_plan *plan);

@@ -21,6 +21,9 @@ PLANS = [
     dict(log_leafwise_below=21, log_tile_leafwise=6),
     dict(log_leafwise_below=0, log_tile_fold=8),      # every committed layer in the big-tile form
     dict(log_leafwise_below=31, log_tile_leafwise=7),  # every committed layer one leaf per thread
+    dict(log_leafwise_grid=11),                        # leafwise tiles shrink with the layer
+    dict(log_leafwise_grid=8, log_tile_leafwise=3),    # 8-leaf tiles throughout
+    dict(log_leafwise_below=31, log_leafwise_grid=10, log_tree_chunk=3),
 ]
 
 
@@ -41,7 +44,7 @@ def test_every_plan_gives_the_oracle_proof(ctx, log_n, rounds):
     for plan in PLANS:
         print(f"2^{log_n} plan {plan}", flush=True)
         p.set_exec_plan(log_tile_commit=11, log_tile_fold=11, log_leafwise_below=19, log_tile_leafwise=8,
-                        log_tree_chunk=10)
+                        log_tree_chunk=10, log_leafwise_grid=0)
         p.set_exec_plan(**plan)
         assert p.prove_fft_coefficients_resident(buf) == want, plan
         p.set_graph(False)
@@ -51,7 +54,8 @@ def test_every_plan_gives_the_oracle_proof(ctx, log_n, rounds):
 
 def test_invalid_plan_rejected(ctx):
     p = azk.CircleCodewordPCSFRIProverV1(ctx, 10, 3, SEC, DIGEST)
-    for bad in (dict(log_tile_commit=12), dict(log_tile_fold=5), dict(log_tile_leafwise=9), dict(log_tree_chunk=11)):
+    for bad in (dict(log_tile_commit=12), dict(log_tile_fold=5), dict(log_tile_leafwise=9), dict(log_tile_leafwise=2),
+                dict(log_tree_chunk=11), dict(log_leafwise_grid=7), dict(log_leafwise_grid=21)):
         with pytest.raises(azk.AZKError) as e:
             p.set_exec_plan(**bad)
         assert e.value.code == 10  # AZK_ERR_INVALID_KERNEL_CONFIG
