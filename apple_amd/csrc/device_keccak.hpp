@@ -1,16 +1,15 @@
 // Keccak-f[1600] and fixed-shape SHA3-256 sponges for gfx950.
 //
-// Three forms of the same permutation:
+// Two forms of the same permutation:
 //  * kf_v: one state per lane, each 64-bit lane as a VGPR pair.  Rotations are
 //    v_alignbit_b32 pairs, the 5-way column parities and chi are v_bitop3_b32
 //    (truth tables 0x96 = xor3, chi via the compiler's own bitop3 formation):
 //    ~190 VALU ops per round, 4555 per permutation.  The throughput form.
 //  * kf_w: one state per wave, one state word per lane (plus copies); DPP and
 //    permlane swaps for theta, one ds_bpermute exchange per round for pi and chi.
-//    The latency form (2.9 us per dependent permutation): Merkle tree tops and the
-//    Fiat-Shamir transcript.
-//  * kf_s: wave-uniform state; the compiler keeps it in SGPRs and runs 64-bit
-//    SALU ops.  Used by the sum-check transcript.
+//    The latency form (2.9 us per dependent permutation, against 8.8 us for a
+//    wave-uniform state on the scalar unit): Merkle tree tops and the Fiat-Shamir
+//    transcripts of the commit and of the sum-check.
 // Semantics follow the reference's SHA3 oracle (Sources/AppleZKProver/SHA3Oracle.swift:
 // 181-280) and its Metal keccak_f1600 (Resources/HashMerkleKernels.metal:308-348);
 // round constants and rho offsets are the FIPS 202 tables.
@@ -225,43 +224,6 @@ __device__ __forceinline__ void kf_w(uint32_t &lo, uint32_t &hi, const WaveLane 
         lo = ((b0l ^ (~b1l & b2l)) & c.keep) ^ (rl & c.lead);
         hi = ((b0h ^ (~b1h & b2h)) & c.keep) ^ (rh & c.lead);
     }
-}
-
-// ---------------------------------------------------------------- SALU form
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int n) { return n == 0 ? x : ((x << n) | (x >> (64 - n))); }
-
-__device__ __forceinline__ void kround_s(uint64_t (&a)[25], uint64_t rc)
-{
-    uint64_t c[5], d[5], b[25];
-#pragma unroll
-    for (int x = 0; x < 5; ++x) c[x] = a[x] ^ a[x + 5] ^ a[x + 10] ^ a[x + 15] ^ a[x + 20];
-#pragma unroll
-    for (int x = 0; x < 5; ++x) d[x] = c[(x + 4) % 5] ^ rotl64(c[(x + 1) % 5], 1);
-    constexpr int R[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39, 41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
-#pragma unroll
-    for (int x = 0; x < 5; ++x)
-#pragma unroll
-        for (int y = 0; y < 5; ++y) b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl64(a[x + 5 * y] ^ d[x], R[x + 5 * y]);
-#pragma unroll
-    for (int y = 0; y < 25; y += 5)
-#pragma unroll
-        for (int x = 0; x < 5; ++x) a[y + x] = b[y + x] ^ (~b[y + (x + 1) % 5] & b[y + (x + 2) % 5]);
-    a[0] ^= rc;
-}
-
-__device__ __forceinline__ void kf_s(uint64_t (&a)[25])
-{
-#pragma unroll 1
-    for (int r = 0; r < 24; ++r) kround_s(a, (uint64_t)kRcLo[r] | ((uint64_t)kRcHi[r] << 32));
-}
-
-// Wave-uniform 64-bit load: the value is moved to SGPRs so the sponge runs on the
-// scalar unit (measured 8.8 us per permutation on one wave vs 12.3 us on VALU).
-__device__ __forceinline__ uint64_t ld_uniform(const uint64_t *p)
-{
-    uint64_t v = *p;
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // ---------------------------------------------------------------- sponges

@@ -80,71 +80,72 @@ __device__ __forceinline__ void block_sum3(uint64_t &a, uint64_t &b, uint64_t &c
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// XOR of a little-endian u32 at byte offset OFF into a padded block held as lanes.
+// The transcript runs on one wave in the one-state-per-wave layout: every lane that holds
+// state word w (its own lane and any copy) applies the same block word, so a padded frame
+// block is read as one u64 per lane.
+
+// This lane's part of a little-endian u32 at byte offset OFF of a padded block.
 template <int OFF>
-__device__ __forceinline__ void put32(uint64_t (&blk)[kFrameLanes], uint32_t v)
+__device__ __forceinline__ uint64_t put32(const WaveLane &c, uint32_t v)
 {
-    constexpr int lane = OFF / 8, sh = (OFF % 8) * 8;
-    blk[lane] ^= (uint64_t)v << sh;
-    if constexpr (sh > 32) blk[lane + 1] ^= (uint64_t)v >> (64 - sh);
+    constexpr uint32_t lane = OFF / 8, sh = (OFF % 8) * 8;
+    uint64_t x = c.word == lane ? (uint64_t)v << sh : 0;
+    if constexpr (sh > 32) {
+        if (c.word == lane + 1) x ^= (uint64_t)v >> (64 - sh);
+    }
+    return x;
 }
 
-__device__ __forceinline__ void load_block(uint64_t (&blk)[kFrameLanes], const uint64_t *src)
+__device__ __forceinline__ uint64_t block_word(const WaveLane &c, const uint64_t *src)
 {
-#pragma unroll
-    for (int i = 0; i < (int)kFrameLanes; ++i) blk[i] = ld_uniform(src + i);
+    return c.word < kFrameLanes ? src[c.word] : 0;
 }
 
-__device__ __forceinline__ void absorb_block(uint64_t (&a)[25], const uint64_t (&blk)[kFrameLanes])
+__device__ __forceinline__ void absorb_block(uint32_t &lo, uint32_t &hi, const WaveLane &c, uint64_t w)
 {
-#pragma unroll
-    for (int i = 0; i < (int)kFrameLanes; ++i) a[i] ^= blk[i];
-    kf_s(a);
+    lo ^= (uint32_t)w;
+    hi ^= (uint32_t)(w >> 32);
+    kf_w(lo, hi, c);
 }
 
 // First u32 below 2^32 - (2^32 mod p) read from a copy of the state, reduced mod p
 // (TranscriptState.squeezeUInt32 with count 1, SHA3Oracle.swift:41-70).  Only the two
 // words 2^32 - 2 and 2^32 - 1 are rejected; the loop bound (2^16 candidates) is never
 // reached by a permutation output and only keeps the loop finite.
-__device__ __forceinline__ uint32_t squeeze_m31(const uint64_t (&a)[25])
+__device__ __forceinline__ uint32_t squeeze_m31(uint32_t lo, uint32_t hi, const WaveLane &c)
 {
-    uint64_t b[25];
-#pragma unroll
-    for (int i = 0; i < 25; ++i) b[i] = a[i];
     const uint64_t limit = (1ull << 32) - ((1ull << 32) % M31_P);
     for (uint32_t cand = 0; cand < (1u << 16); ++cand) {
-        if (cand > 0 && cand % 34 == 0) kf_s(b);
-        const uint32_t wi = cand % 34;
-        uint64_t lane = 0;
-#pragma unroll
-        for (int i = 0; i < 17; ++i)
-            if ((uint32_t)i == (wi >> 1)) lane = b[i];
-        const uint64_t w = (wi & 1) ? (lane >> 32) : (lane & 0xffffffffull);
+        if (cand > 0 && cand % 34 == 0) kf_w(lo, hi, c);
+        const uint32_t wi = cand % 34, at = wave_lane_of(wi >> 1);
+        // readlane returns a signed int: keep the word unsigned before widening
+        const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((wi & 1) ? hi : lo, at);
         if (w < limit) return (uint32_t)(w % M31_P);
     }
     return 0;
 }
 
-// One round of the transcript on wave-uniform values: header first when r == 0, then
-// the round frame (r, e0, e1) and the challenge frame (r, p); returns the challenge.
-// `tmpl` holds padded blocks: [header][round 0][challenge 0][round 1]...
-__device__ __forceinline__ uint32_t sc_transcript_round(uint64_t (&a)[25], const uint64_t *tmpl, uint32_t r,
-                                                        uint32_t e0, uint32_t e1)
+// One round of the transcript on the wave's state: header first when r == 0, then the
+// round frame (r, e0, e1) and the challenge frame (r, p); returns the challenge.  `tmpl`
+// holds padded blocks: [header][round 0][challenge 0][round 1]...
+__device__ __forceinline__ uint32_t sc_transcript_round(uint32_t &lo, uint32_t &hi, const WaveLane &c,
+                                                        const uint64_t *tmpl, uint32_t r, uint32_t e0, uint32_t e1)
 {
-    uint64_t blk[kFrameLanes];
-    if (r == 0) {
-        load_block(blk, tmpl);
-        put32<kHdrClaimed>(blk, m31_add(e0, e1));
-        absorb_block(a, blk);
-    }
     const uint64_t *rt = tmpl + (size_t)kFrameLanes * (1 + 2 * r);
-    load_block(blk, rt);
-    put32<kRoundE0>(blk, e0);
-    put32<kRoundE1>(blk, e1);
-    absorb_block(a, blk);
-    load_block(blk, rt + kFrameLanes);
-    absorb_block(a, blk);
-    return squeeze_m31(a);
+    const uint64_t round_w = block_word(c, rt), chal_w = block_word(c, rt + kFrameLanes);  // both loads in flight
+    if (r == 0) absorb_block(lo, hi, c, block_word(c, tmpl) ^ put32<kHdrClaimed>(c, m31_add(e0, e1)));
+    absorb_block(lo, hi, c, round_w ^ put32<kRoundE0>(c, e0) ^ put32<kRoundE1>(c, e1));
+    absorb_block(lo, hi, c, chal_w);
+    return squeeze_m31(lo, hi, c);
+}
+
+// The state words of this lane from memory (zero state when r == 0), and back (own lanes).
+__device__ __forceinline__ void load_state(uint32_t &lo, uint32_t &hi, const WaveLane &c, const uint64_t *state,
+                                           bool zero)
+{
+    const uint64_t v = (!zero && c.word < 25) ? state[c.word] : 0;
+    lo = (uint32_t)v;
+    hi = (uint32_t)(v >> 32);
 }
 
 // Per-block partial sums of the input: even-index words, odd-index words, words >= p.
@@ -181,15 +182,10 @@ __global__ __launch_bounds__(64) void k_sc_round(const uint64_t *__restrict__ pa
     o = wave_sum(o);
     bad = wave_sum(bad);
     const uint32_t e0 = uni((uint32_t)(e % M31_P)), e1 = uni((uint32_t)(o % M31_P));
-    uint64_t a[25];
-    if (r == 0) {
-#pragma unroll
-        for (int i = 0; i < 25; ++i) a[i] = 0;
-    } else {
-#pragma unroll
-        for (int i = 0; i < 25; ++i) a[i] = ld_uniform(state + i);
-    }
-    const uint32_t c = sc_transcript_round(a, tmpl, r, e0, e1);
+    const WaveLane wl = wave_lane();
+    uint32_t lo, hi;
+    load_state(lo, hi, wl, state, r == 0);
+    const uint32_t c = sc_transcript_round(lo, hi, wl, tmpl, r, e0, e1);
     if (threadIdx.x == 0) {
         out.w[2 * r] = e0;
         out.w[2 * r + 1] = e1;
@@ -198,9 +194,8 @@ __global__ __launch_bounds__(64) void k_sc_round(const uint64_t *__restrict__ pa
             out.w[3 * out.rounds + 1] = m31_add(e0, e1);
             out.w[3 * out.rounds + 2] = bad > 0xffffffffull ? 0xffffffffu : (uint32_t)bad;
         }
-#pragma unroll
-        for (int i = 0; i < 25; ++i) state[i] = a[i];
     }
+    if (wl.own) state[wl.word] = (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // Fold of round r (challenge read from out) fused with the partial sums of round r + 1.
@@ -241,17 +236,10 @@ __global__ __launch_bounds__(kScThreads) void k_sc_tail(const uint32_t *__restri
         A[i] = x;
     }
     __syncthreads();
-    uint64_t a[25];
-    const bool w0 = threadIdx.x < 64;
-    if (w0) {
-        if (r0 == 0) {
-#pragma unroll
-            for (int i = 0; i < 25; ++i) a[i] = 0;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 25; ++i) a[i] = ld_uniform(state + i);
-        }
-    }
+    const bool w0 = threadIdx.x < 64;  // wave 0 holds the transcript state
+    const WaveLane wl = wave_lane();
+    uint32_t lo = 0, hi = 0;
+    if (w0) load_state(lo, hi, wl, state, r0 == 0);
     uint32_t *cur = A, *nxt = B;
     for (uint32_t r = r0; r < rounds; ++r) {
         uint64_t e = 0, o = 0;
@@ -264,7 +252,7 @@ __global__ __launch_bounds__(kScThreads) void k_sc_tail(const uint32_t *__restri
         __syncthreads();
         if (w0) {
             const uint32_t e0 = uni((uint32_t)(sh_sum[0] % M31_P)), e1 = uni((uint32_t)(sh_sum[1] % M31_P));
-            const uint32_t c = sc_transcript_round(a, tmpl, r, e0, e1);
+            const uint32_t c = sc_transcript_round(lo, hi, wl, tmpl, r, e0, e1);
             if (threadIdx.x == 0) {
                 sh_c = c;
                 out.w[2 * r] = e0;
@@ -289,11 +277,8 @@ __global__ __launch_bounds__(kScThreads) void k_sc_tail(const uint32_t *__restri
         nxt = t;
         m = half;
     }
-    if (threadIdx.x == 0) {
-        out.w[3 * out.rounds] = cur[0];
-#pragma unroll
-        for (int i = 0; i < 25; ++i) state[i] = a[i];
-    }
+    if (threadIdx.x == 0) out.w[3 * out.rounds] = cur[0];
+    if (w0 && wl.own) state[wl.word] = (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // ------------------------------------------------------------------ host side
