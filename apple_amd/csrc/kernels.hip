@@ -155,74 +155,80 @@ __device__ __forceinline__ void ld_node(const uint32_t *p, uint32_t (&d)[8])
     d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
 }
 
-// Parents q < half of the level-l nodes in `src`, one per wave at a time with the
-// one-state-per-wave permutation (lanes holding words < 8 load the 64-byte message, the
-// own lanes of words 0..3 write the digest).  The loop is wave-uniform.
-__device__ __forceinline__ void wave_parents(const uint32_t *src, uint32_t *dst, uint32_t half, int l, uint64_t g,
+// In-place Merkle levels in LDS: the nodes of the current level sit `s` node slots (8s
+// words) apart; parent q of nodes 2q and 2q + 1 overwrites node 2q's slot, so the next
+// level sits 2s slots apart.  Each parent reads and writes only its own children's slots,
+// so one barrier per level suffices and the reduction needs no second LDS region.
+
+// Parents q < half, one per wave at a time with the one-state-per-wave permutation (lanes
+// holding words < 8 load the 64-byte message, the own lanes of words 0..3 write the
+// digest).  The loop is wave-uniform.
+__device__ __forceinline__ void wave_parents(uint32_t *A, uint32_t half, uint32_t s, int l, uint64_t g,
                                              uint32_t *tree, uint64_t m, const WaveLane &c)
 {
     const uint32_t waves = blockDim.x >> 6;
     for (uint32_t q = threadIdx.x >> 6; q < half; q += waves) {
+        uint32_t *left = A + 16 * s * q;
         uint32_t lo = 0, hi = 0;
-        if (c.word < 8) {
-            lo = src[16 * q + 2 * c.word];
-            hi = src[16 * q + 2 * c.word + 1];
+        if (c.word < 8) {  // words 0..3: left child, 4..7: right child (8s words further)
+            const uint32_t at = c.word < 4 ? 2 * c.word : 8 * s + 2 * (c.word - 4);
+            lo = left[at];
+            hi = left[at + 1];
         }
         if (c.word == 8) lo = 0x06u;          // SHA3 padding after 64 bytes
         if (c.word == 16) hi = 0x80000000u;   // final bit of the 136-byte block
-        kf_w(lo, hi, c);
+        kf_w(lo, hi, c);                      // every load of this wave has returned
         if (c.own && c.word < 4) {
-            dst[8 * q + 2 * c.word] = lo;
-            dst[8 * q + 2 * c.word + 1] = hi;
+            left[2 * c.word] = lo;
+            left[2 * c.word + 1] = hi;
             uint32_t *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
             reinterpret_cast<uint2 *>(node)[c.word] = make_uint2(lo, hi);
         }
     }
 }
 
-// Reduces `cnt` level-`l` nodes held in LDS region A (node q at words 8q..8q+7),
-// whose first node has global index `g` in that level, up to one node.  Every
-// computed level is written to the tree in HBM.  Ping-pong A -> B -> A ... with one
-// barrier per level (B holds cnt/2 nodes).  Levels with at most kWaveParents parents per
-// wave (the serial top of the tree) use the one-state-per-wave permutation, 2.9 us per
-// dependent permutation against ~9 us for a level in the one-lane form.
+// Reduces `cnt` level-`l` nodes held contiguously in LDS (node q at words 8q..8q+7),
+// whose first node has global index `g` in that level, up to one node at A[0..7].  Every
+// computed level is written to the tree in HBM.  Levels with at most kWaveParents parents
+// per wave (the serial top of the tree) use the one-state-per-wave permutation, 2.9 us
+// per dependent permutation against ~9 us for a level in the one-lane form.
 #ifndef AZK_WAVE_PARENTS
 #define AZK_WAVE_PARENTS 2
 #endif
 constexpr uint32_t kWaveParents = AZK_WAVE_PARENTS;
-__device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t *B, uint32_t cnt, int l, uint64_t g, uint32_t *tree,
-                                           uint64_t m)
+__device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t cnt, int l, uint64_t g, uint32_t *tree, uint64_t m)
 {
-    uint32_t *src = A, *dst = B;
     const uint32_t wave_max = kWaveParents * (blockDim.x >> 6);
     const WaveLane c = wave_lane();
+    uint32_t s = 1;
     while (cnt > 1) {
         uint32_t half = cnt >> 1;
         if (half <= wave_max) {
-            wave_parents(src, dst, half, l, g, tree, m, c);
+            wave_parents(A, half, s, l, g, tree, m, c);
         } else {
             for (uint32_t q = threadIdx.x; q < half; q += blockDim.x) {
                 uint32_t lft[8], rgt[8], o[8];
-                ld_node(src + 16 * q, lft);
-                ld_node(src + 16 * q + 8, rgt);
+                ld_node(A + 16 * s * q, lft);
+                ld_node(A + 16 * s * q + 8 * s, rgt);
                 sha3_parent(lft, rgt, o);
-                st_node(dst + 8 * q, o);
+                st_node(A + 16 * s * q, o);
                 st_node(node_ptr(tree, m, l + 1, (g >> 1) + q), o);
             }
         }
         __syncthreads();
-        uint32_t *t = src; src = dst; dst = t;
         cnt = half;
+        s *= 2;
         ++l;
         g >>= 1;
     }
 }
 
-// Hashes the T QM31 leaves held in LDS region A (leaf i at words 4i..4i+3, global
-// leaf index g0 + i) and builds the tile's subtree up to its root, writing every
-// node.  Leaf pair p's 32 bytes are overwritten in place by its parent.
-__device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t *B, uint32_t T, uint64_t g0, uint32_t *tree,
-                                               uint64_t m)
+// Hashes the T QM31 leaves held in LDS (leaf i at words 4i..4i+3, global leaf index
+// g0 + i) and builds the tile's subtree up to its root, writing every node.  Leaf pair
+// p's 32 bytes are overwritten in place by its parent.  The first leaf digest is stored
+// before the second leaf is hashed: with both permutations in flight the compiler needs
+// 155 VGPRs for the kernel instead of 114 (four waves per SIMD).
+__device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t T, uint64_t g0, uint32_t *tree, uint64_t m)
 {
     if (T == 1) {
         uint32_t v[4] = {A[0], A[1], A[2], A[3]}, h[8];
@@ -237,15 +243,15 @@ __device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t *B, uint32_
         uint32_t v0[4] = {e0.x, e0.y, e0.z, e0.w}, v1[4] = {e1.x, e1.y, e1.z, e1.w};
         uint32_t h0[8], h1[8], par[8];
         sha3_leaf16(v0, h0);
-        sha3_leaf16(v1, h1);
         st_node(node_ptr(tree, m, 0, g0 + 2 * p), h0);
+        sha3_leaf16(v1, h1);
         st_node(node_ptr(tree, m, 0, g0 + 2 * p + 1), h1);
         sha3_parent(h0, h1, par);
         st_node(node_ptr(tree, m, 1, (g0 >> 1) + p), par);
         st_node(A + 8 * p, par);
     }
     __syncthreads();
-    reduce_lds(A, B, T / 2, 1, g0 >> 1, tree, m);
+    reduce_lds(A, T / 2, 1, g0 >> 1, tree, m);
 }
 
 // Raw-leaf Merkle tile: leaves of `len` <= 136 bytes at `stride`, T leaves per
@@ -261,7 +267,7 @@ __global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *_
     const uint32_t stride = d.stride, len = d.len;
     uint32_t *tree = d.tree;
     const uint64_t m = d.m;
-    uint32_t *A = lds, *B = lds + 4 * T;
+    uint32_t *A = lds;
     const uint64_t g0 = (uint64_t)tile * T;
     if (T == 1) {
         if (threadIdx.x == 0) {
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *_
         st_node(A + 8 * p, par);
     }
     __syncthreads();
-    reduce_lds(A, B, T / 2, 1, g0 >> 1, tree, m);
+    reduce_lds(A, T / 2, 1, g0 >> 1, tree, m);
 }
 
 __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops);
@@ -308,7 +314,7 @@ __global__ __launch_bounds__(AZK_TREE_THREADS) void k_tree_reduce(const TreeDesc
     uint32_t *tree = d.tree;
     const uint64_t m = d.m;
     const int l = (int)d.level;
-    uint32_t *A = lds, *B = lds + 8 * C;
+    uint32_t *A = lds;
     const uint64_t g = (uint64_t)chunk * C;
     for (uint32_t q = threadIdx.x; q < C; q += blockDim.x) {
         uint32_t v[8];
@@ -316,7 +322,7 @@ __global__ __launch_bounds__(AZK_TREE_THREADS) void k_tree_reduce(const TreeDesc
         st_node(A + 8 * q, v);
     }
     __syncthreads();
-    reduce_lds(A, B, C, l, g, tree, m);
+    reduce_lds(A, C, l, g, tree, m);
     if (d.t_ops) {  // uniform per block; the barrier makes the root's HBM words visible to wave 0
         __syncthreads();
         if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
@@ -513,7 +519,7 @@ __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restric
         // every wave has read its elements before the subtree overwrites leaf pairs in place
         __syncthreads();
     }
-    if (tree) tile_tree_qm31(lds, lds + 4 * T, T, base, tree, n);
+    if (tree) tile_tree_qm31(lds, T, base, tree, n);
 }
 
 // ============================================================== FRI fold
@@ -549,7 +555,7 @@ __global__ __launch_bounds__(256) void k_fold(const FoldDesc *__restrict__ tab, 
     }
     if (!tree) return;
     __syncthreads();
-    tile_tree_qm31(lds, lds + 4 * T, T, base, tree, d.n_out);
+    tile_tree_qm31(lds, T, base, tree, d.n_out);
 }
 
 // Streaming form when no commitment follows (the fold chain plan): one output per
@@ -572,7 +578,7 @@ __global__ __launch_bounds__(256) void k_fold_stream(const FoldDesc *__restrict_
 // its leaf hashes in LDS (serial chain: one leaf permutation + log2 T levels).
 __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restrict__ tab, uint32_t ntab)
 {
-    __shared__ uint32_t A[8 * 256], B[4 * 256];
+    __shared__ uint32_t A[8 * 256];
     uint32_t tile;
     const FoldDesc &d = find_desc(tab, ntab, tile);
     if (d.t_ops) {  // block-uniform: a transcript block, no barrier below is reached
@@ -599,7 +605,7 @@ __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restric
     }
     if (!tree) return;
     __syncthreads();
-    reduce_lds(A, B, T, 0, base, tree, n_out);
+    reduce_lds(A, T, 0, base, tree, n_out);
 }
 
 // ============================================================== transcript
@@ -766,7 +772,9 @@ __global__ __launch_bounds__(256) void k_merkle_path(const uint32_t *__restrict_
 // ============================================================== host launchers
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
-static inline size_t tile_lds_bytes(uint32_t log_t) { return (size_t)24 << log_t; }  // 16T + 8T bytes
+// a tile of T QM31 values (16T bytes); the final FFT pass also stages T twiddle words after it
+static inline size_t tile_lds_bytes(uint32_t log_t) { return (size_t)16 << log_t; }
+static inline size_t fft_tile_lds_bytes(uint32_t log_t) { return (size_t)20 << log_t; }  // 40 KB at 2^11
 
 void launch_keccakf_batch(hipStream_t st, const void *in, uint32_t in_stride, void *out, uint32_t out_stride, uint32_t count)
 {
@@ -825,7 +833,7 @@ void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t 
 void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
 {
     if (!blocks) return;
-    hipLaunchKernelGGL(k_fft_final, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
+    hipLaunchKernelGGL(k_fft_final, dim3(blocks), dim3(256), fft_tile_lds_bytes(max_log_t), st, t, n);
 }
 
 void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
@@ -849,7 +857,7 @@ void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_
 void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c)
 {
     if (!blocks) return;
-    size_t lds = (size_t)(8u << max_log_c) * 4 + (size_t)(4u << max_log_c) * 4;
+    const size_t lds = (size_t)32 << max_log_c;  // 2^log_c nodes of 32 bytes, reduced in place
     // 4 waves (one per SIMD): 8-wave blocks with 16-node chunks measured no faster
     hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(AZK_TREE_THREADS), lds, st, t, n);
 }

@@ -77,7 +77,7 @@ inline uint32_t ilog2(uint64_t v)
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr uint32_t kP = 0x7fffffffu;
-constexpr uint32_t kDefaultLogTile = 11;    // 2048-element tiles: 48 KB LDS per block
+constexpr uint32_t kDefaultLogTile = 11;    // 2048-element tiles: 40 KB LDS (final FFT pass), 32 KB (folds)
 constexpr uint32_t kUpperLogChunk = 10;     // largest upper-tree reduction chunk
 // Commit trees: 8-node chunks, so every level of a step runs in the one-state-per-wave
 // form with one parent per wave (a 2^21 commit measured 2% faster than 16-node chunks).
