@@ -544,8 +544,11 @@ extern "C" azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_
                     desc->leaf_length);
     if (desc->leaf_stride < desc->leaf_length) return fail(AZK_ERR_INVALID_LAYOUT, "leaf stride shorter than leaf length");
     auto *p = new azk_merkle_plan{ctx, *desc, ilog2(desc->leaf_count), 0, {}, {}, {}, false, {}, {}};
-    uint32_t lt = desc->subtree_log ? desc->subtree_log : kDefaultLogTile;
-    if (lt > kDefaultLogTile) lt = kDefaultLogTile;  // 24 x 2^11 bytes of LDS per block
+    // default treelet: at least 256 blocks (one per CU) until tiles reach 2^11 leaves; 2^16 x 64-byte
+    // leaves measured 148 us with 2^8-leaf treelets against 243 us with 2^11
+    const uint32_t lm = ilog2(desc->leaf_count);
+    uint32_t lt = desc->subtree_log ? desc->subtree_log : std::min(kDefaultLogTile, std::max(8u, lm > 8 ? lm - 8 : 0u));
+    if (lt > kDefaultLogTile) lt = kDefaultLogTile;  // 16 x 2^11 bytes of LDS per block
     p->log_t = std::min(lt, p->log_m);
     HIP_TRY(hipSetDevice(ctx->device));
     azk_status s = p->tree.ensure(((size_t)2 * desc->leaf_count - 1) * 32);

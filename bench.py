@@ -302,6 +302,49 @@ def keccak_form(ctx, log_count: int = 22, reps: int = 10, warm: int = 30) -> dic
             **res}
 
 
+def merkle_plain_form(ctx, log_count: int = 16, leaf_bytes: int = 64, openings: int = 16, reps: int = 20,
+                      check: bool = True) -> dict:
+    """The plain SHA3-256 Merkle commitment of 2^16 leaves of 64 bytes (seeded uniform bytes):
+    resident commit (root) and 16 opening paths gathered from the resident tree, against the
+    oracle on one host core; root and openings bit-exact."""
+    count = 1 << log_count
+    raw = inputs.random_bytes(16, count * leaf_bytes)
+    plan = azk.SHA3RawLeavesMerkleCommitPlan(ctx, count, leaf_bytes, leaf_bytes)
+    buf = ctx.upload(raw)
+    for _ in range(reps):
+        root = plan.commit_resident(buf)
+    t_gpu, t_wall = [], []
+    for _ in range(reps):
+        plan.commit_resident(buf)
+        t_gpu.append(plan.last_stats["gpu_ns"] * 1e-9)
+        t_wall.append(plan.last_stats["wall_ns"] * 1e-9)
+    idx = [int(x) for x in inputs.splitmix64(count, openings) % np.uint64(count)]
+    t0 = time.perf_counter()
+    paths = [plan.open(i) for i in idx]
+    t_open = time.perf_counter() - t0
+    res = {"workload": f"sha3_merkle_2^{log_count}x{leaf_bytes}B", "leaves": count, "openings": openings,
+           "commit_gpu_ms": 1e3 * float(np.median(t_gpu)), "commit_wall_ms": 1e3 * float(np.median(t_wall)),
+           "commits_per_s": 1.0 / float(np.median(t_wall)), "openings_ms": 1e3 * t_open}
+    perms = 2 * count - 1  # one permutation per 64-byte leaf and per parent
+    achieved = perms * KECCAK_PERM_OPS / float(np.median(t_gpu)) / 1e12
+    res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                       "frac": achieved / VALU_PEAK_TOPS, "permutations": perms}
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O  # checker and CPU baseline only
+        t0 = time.perf_counter()
+        want = O.merkle_root(raw, count, leaf_bytes, leaf_bytes)
+        res["cpu_oracle_ms"] = 1e3 * (time.perf_counter() - t0)
+        ok = want == root and all(O.merkle_opening(raw, count, leaf_bytes, leaf_bytes, i) == pth
+                                  for i, pth in zip(idx, paths))
+        if not ok:
+            raise SystemExit("FATAL: Merkle root or openings differ from the oracle")
+        res["bit_exact_vs_oracle"] = True
+    buf.free()
+    plan.close()
+    return res
+
+
 def calibrate_copy(ctx, log_count: int = 23) -> None:
     """A plain copy of known size for scaling the PMC byte counters: k_copy moves
     2^23 x 32 bytes (256 MiB) in and out (the prehashed commit's first kernel)."""
@@ -435,6 +478,7 @@ def main() -> int:
         line["serving"] = serving_form(ctx)
     if args.keccak:
         line["keccak_batch"] = keccak_form(ctx)
+        line["merkle_plain"] = merkle_plain_form(ctx, check=bool(args.check))
     if args.monomial:
         line["monomial_input"] = monomial_input_form(ctx, prover)
     if args.standalone:

@@ -129,7 +129,7 @@ typedef struct azk_merkle_desc {
     uint32_t leaf_count;     /* power of two */
     uint32_t leaf_stride;
     uint32_t leaf_length;    /* 0..136 */
-    uint32_t subtree_log;    /* leaves per LDS treelet = 2^subtree_log; 0 = default */
+    uint32_t subtree_log;    /* leaves per LDS treelet = 2^subtree_log; 0 = default: clamp(log2 count - 8, 8, 11) */
 } azk_merkle_desc;
 typedef struct azk_merkle_plan azk_merkle_plan;
 azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_desc *desc, azk_merkle_plan **out);
