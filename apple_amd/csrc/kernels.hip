@@ -155,6 +155,14 @@ __device__ __forceinline__ void ld_node(const uint32_t *p, uint32_t (&d)[8])
     d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
 }
 
+// Thread index rotated by (block mod 4) waves, for loops over fewer items than threads: the
+// active waves of the blocks that share a CU then sit on different SIMDs instead of all on
+// the SIMDs of waves 0, 1, ...  A bijection on [0, blockDim.x) (a power of two).
+__device__ __forceinline__ uint32_t rotated_tid()
+{
+    return (threadIdx.x + ((blockIdx.x & 3u) << 6)) & (blockDim.x - 1);
+}
+
 // In-place Merkle levels in LDS: the nodes of the current level sit `s` node slots (8s
 // words) apart; parent q of nodes 2q and 2q + 1 overwrites node 2q's slot, so the next
 // level sits 2s slots apart.  Each parent reads and writes only its own children's slots,
@@ -200,13 +208,14 @@ __device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t cnt, int l, uin
 {
     const uint32_t wave_max = kWaveParents * (blockDim.x >> 6);
     const WaveLane c = wave_lane();
+    const uint32_t tid = rotated_tid();
     uint32_t s = 1;
     while (cnt > 1) {
         uint32_t half = cnt >> 1;
         if (half <= wave_max) {
             wave_parents(A, half, s, l, g, tree, m, c);
         } else {
-            for (uint32_t q = threadIdx.x; q < half; q += blockDim.x) {
+            for (uint32_t q = tid; q < half; q += blockDim.x) {
                 uint32_t lft[8], rgt[8], o[8];
                 ld_node(A + 16 * s * q, lft);
                 ld_node(A + 16 * s * q + 8 * s, rgt);
@@ -591,7 +600,7 @@ __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restric
     const qm31 c = {ch[0], ch[1], ch[2], ch[3]};
     uint32_t *tree = d.tree;
     const uint64_t n_out = d.n_out;
-    const uint32_t L = threadIdx.x;
+    const uint32_t L = rotated_tid();
     if (L < T) {
         const uint64_t g = base + L;
         qm31 o = fri_fold_pair(ld_qm(d.in[2 * g]), ld_qm(d.in[2 * g + 1]), d.tw[g], c);
