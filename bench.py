@@ -78,6 +78,27 @@ def cpu_baseline(coeffs, sample_commits: int) -> dict:
                       f"{dt:.1f} s on one host core"}
 
 
+def cpu_all_cores(coeffs, threads: int | None = None) -> dict:
+    """The same oracle commit on every host core at once: one independent 2^21 commit per
+    thread (the oracle's C calls release the interpreter lock), a reported baseline only."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O  # the oracle is the CPU baseline
+
+    n = threads or min(16, len(os.sched_getaffinity(0)))
+
+    def one(_):
+        cw = O.circle_fft(coeffs, LOG_N)
+        O.fri_prove(cw.reshape(-1, 4), LOG_N, SECURITY, DIGEST, ROUNDS)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(n) as ex:
+        list(ex.map(one, range(n)))
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "commits/s", "cores": n, "kind": "port",
+            "sample": f"{n} full 2^{LOG_N} commits, one per thread, oracle/liboracle.so, {dt:.1f} s"}
+
+
 def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
     """64 independent commits, log sizes uniform in 14..21 (seeded), one batch program vs
     the same jobs proved one after another; every batched proof equals its plain proof."""
@@ -486,6 +507,7 @@ def main() -> int:
         line["standalone"]["witness_basis"] = witness_basis_form(ctx)
     if args.cpu_sample:
         line["cpu_baseline"] = cpu_baseline(coeffs, args.cpu_sample)
+        line["cpu_baseline_all_cores"] = cpu_all_cores(coeffs)
     print(json.dumps(line), flush=True)
     return 0
 
