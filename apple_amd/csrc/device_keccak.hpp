@@ -116,6 +116,78 @@ __device__ __forceinline__ void kf_v(u2 (&a)[25])
     kround_v(a, 0x80008008u, 0x80000000u);
 }
 
+// ---------------------------------------------------------------- two lanes per state
+// Lanes 2k and 2k + 1 hold one state: the even lane the low, the odd lane the high 32-bit
+// halves of all 25 words.  A 64-bit rotation by r is one alignbit of this lane's half with
+// the partner's (DPP quad_perm [1,0,3,2]), the same formula on both lanes: alignbit(mine,
+// partner, 32 - r) for r < 32, alignbit(partner, mine, 64 - r) above.  Theta's parities,
+// chi and iota stay per half: about 125 instructions per round per lane against 190 for
+// one lane per state, for levels of a Merkle tree with 16..128 parents per block.  Both
+// lanes of a pair must be active.
+__device__ __forceinline__ uint32_t pswap(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, false); }
+
+template <int N>
+__device__ __forceinline__ uint32_t hrot(uint32_t v)
+{
+    if constexpr (N == 0) return v;
+    else if constexpr (N < 32) return __builtin_amdgcn_alignbit(v, pswap(v), 32 - N);
+    else return __builtin_amdgcn_alignbit(pswap(v), v, 64 - N);
+}
+
+__device__ __forceinline__ void kround_h(uint32_t (&a)[25], uint32_t rc)
+{
+    const uint32_t c0 = xor3(xor3(a[0], a[5], a[10]), a[15], a[20]);
+    const uint32_t c1 = xor3(xor3(a[1], a[6], a[11]), a[16], a[21]);
+    const uint32_t c2 = xor3(xor3(a[2], a[7], a[12]), a[17], a[22]);
+    const uint32_t c3 = xor3(xor3(a[3], a[8], a[13]), a[18], a[23]);
+    const uint32_t c4 = xor3(xor3(a[4], a[9], a[14]), a[19], a[24]);
+    const uint32_t d0 = c4 ^ hrot<1>(c1), d1 = c0 ^ hrot<1>(c2), d2 = c1 ^ hrot<1>(c3);
+    const uint32_t d3 = c2 ^ hrot<1>(c4), d4 = c3 ^ hrot<1>(c0);
+    uint32_t b[25];  // B[y, 2x+3y] = rot(A[x, y] ^ D[x], r[x, y]), as in kround_v
+    b[0] = a[0] ^ d0;
+    b[10] = hrot<1>(a[1] ^ d1);
+    b[20] = hrot<62>(a[2] ^ d2);
+    b[5] = hrot<28>(a[3] ^ d3);
+    b[15] = hrot<27>(a[4] ^ d4);
+    b[16] = hrot<36>(a[5] ^ d0);
+    b[1] = hrot<44>(a[6] ^ d1);
+    b[11] = hrot<6>(a[7] ^ d2);
+    b[21] = hrot<55>(a[8] ^ d3);
+    b[6] = hrot<20>(a[9] ^ d4);
+    b[7] = hrot<3>(a[10] ^ d0);
+    b[17] = hrot<10>(a[11] ^ d1);
+    b[2] = hrot<43>(a[12] ^ d2);
+    b[12] = hrot<25>(a[13] ^ d3);
+    b[22] = hrot<39>(a[14] ^ d4);
+    b[23] = hrot<41>(a[15] ^ d0);
+    b[8] = hrot<45>(a[16] ^ d1);
+    b[18] = hrot<15>(a[17] ^ d2);
+    b[3] = hrot<21>(a[18] ^ d3);
+    b[13] = hrot<8>(a[19] ^ d4);
+    b[14] = hrot<18>(a[20] ^ d0);
+    b[24] = hrot<2>(a[21] ^ d1);
+    b[9] = hrot<61>(a[22] ^ d2);
+    b[19] = hrot<56>(a[23] ^ d3);
+    b[4] = hrot<14>(a[24] ^ d4);
+#pragma unroll
+    for (int y = 0; y < 25; y += 5) {
+        const uint32_t t0 = b[y], t1 = b[y + 1], t2 = b[y + 2], t3 = b[y + 3], t4 = b[y + 4];
+        a[y] = t0 ^ (~t1 & t2);
+        a[y + 1] = t1 ^ (~t2 & t3);
+        a[y + 2] = t2 ^ (~t3 & t4);
+        a[y + 3] = t3 ^ (~t4 & t0);
+        a[y + 4] = t4 ^ (~t0 & t1);
+    }
+    a[0] ^= rc;
+}
+
+// `hi`: this lane holds the high halves (the odd lane of its pair).
+__device__ __forceinline__ void kf_h(uint32_t (&a)[25], bool hi)
+{
+#pragma unroll 2
+    for (int r = 0; r < 24; ++r) kround_h(a, hi ? kRcHi[r] : kRcLo[r]);
+}
+
 __device__ __forceinline__ uint32_t bperm(int addr, uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);

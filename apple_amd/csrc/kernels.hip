@@ -195,11 +195,36 @@ __device__ __forceinline__ void wave_parents(uint32_t *A, uint32_t half, uint32_
     }
 }
 
+// Parent q by a lane pair with the two-lanes-per-state permutation: lane `hi` of the pair
+// loads and writes the high (odd) 32-bit word of each 64-bit message and digest word.
+__device__ __forceinline__ void pair_parent(uint32_t *A, uint32_t q, uint32_t s, bool hi, int l, uint64_t g,
+                                            uint32_t *tree, uint64_t m)
+{
+    uint32_t *left = A + 16 * s * q, *right = left + 8 * s;
+    uint32_t a[25];
+#pragma unroll
+    for (int w = 0; w < 25; ++w) a[w] = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        a[w] = left[2 * w + hi];
+        a[4 + w] = right[2 * w + hi];
+    }
+    a[8] = hi ? 0u : 0x06u;           // SHA3 padding after 64 bytes
+    a[16] = hi ? 0x80000000u : 0u;    // final bit of the 136-byte block
+    kf_h(a, hi);
+    uint32_t *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        left[2 * w + hi] = a[w];
+        node[2 * w + hi] = a[w];
+    }
+}
+
 // Reduces `cnt` level-`l` nodes held contiguously in LDS (node q at words 8q..8q+7),
 // whose first node has global index `g` in that level, up to one node at A[0..7].  Every
-// computed level is written to the tree in HBM.  Levels with at most kWaveParents parents
-// per wave (the serial top of the tree) use the one-state-per-wave permutation, 2.9 us
-// per dependent permutation against ~9 us for a level in the one-lane form.
+// computed level is written to the tree in HBM.  By the number of parents of a level:
+// more than half the threads, one lane per parent (~10 us a level); down to kWaveParents
+// per wave, two lanes per parent (5.4 us); the serial top, one wave per parent (2.9 us).
 #ifndef AZK_WAVE_PARENTS
 #define AZK_WAVE_PARENTS 2
 #endif
@@ -214,6 +239,8 @@ __device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t cnt, int l, uin
         uint32_t half = cnt >> 1;
         if (half <= wave_max) {
             wave_parents(A, half, s, l, g, tree, m, c);
+        } else if (2 * half <= blockDim.x) {
+            if ((tid >> 1) < half) pair_parent(A, tid >> 1, s, tid & 1u, l, g, tree, m);  // pairs are whole
         } else {
             for (uint32_t q = tid; q < half; q += blockDim.x) {
                 uint32_t lft[8], rgt[8], o[8];
@@ -600,8 +627,33 @@ __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restric
     const qm31 c = {ch[0], ch[1], ch[2], ch[3]};
     uint32_t *tree = d.tree;
     const uint64_t n_out = d.n_out;
-    const uint32_t L = rotated_tid();
-    if (L < T) {
+    const uint32_t tid = rotated_tid();
+    if (tree && 2 * T <= blockDim.x) {
+        // two lanes per leaf: both fold the pair, the even lane stores the value, and the
+        // pair hashes it with the two-lanes-per-state permutation (lo: a, c; hi: b, d)
+        const uint32_t L = tid >> 1;
+        const bool hi = tid & 1u;
+        if (L < T) {
+            const uint64_t g = base + L;
+            const qm31 o = fri_fold_pair(ld_qm(d.in[2 * g]), ld_qm(d.in[2 * g + 1]), d.tw[g], c);
+            if (!hi) d.out[g] = st_qm(o);
+            uint32_t a[25];
+#pragma unroll
+            for (int w = 0; w < 25; ++w) a[w] = 0;
+            a[0] = hi ? o.b : o.a;
+            a[1] = hi ? o.d : o.c;
+            a[2] = hi ? 0u : 0x06u;
+            a[16] = hi ? 0x80000000u : 0u;
+            kf_h(a, hi);
+            uint32_t *node = node_ptr(tree, n_out, 0, g);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                node[2 * w + hi] = a[w];
+                A[8 * L + 2 * w + hi] = a[w];
+            }
+        }
+    } else if (tid < T) {
+        const uint32_t L = tid;
         const uint64_t g = base + L;
         qm31 o = fri_fold_pair(ld_qm(d.in[2 * g]), ld_qm(d.in[2 * g + 1]), d.tw[g], c);
         d.out[g] = st_qm(o);

@@ -75,9 +75,9 @@ azk_status check_exec_plan(const azk_circle_exec_plan &x)
     if (x.log_tile_commit < 6 || x.log_tile_commit > kDefaultLogTile || x.log_tile_fold < 6 ||
         x.log_tile_fold > kDefaultLogTile || x.log_tile_leafwise < kLeafwiseMinLogTile ||
         x.log_tile_leafwise > kLeafwiseLogTile || x.log_leafwise_below > 31 || x.log_tree_chunk < 3 ||
-        x.log_tree_chunk > kUpperLogChunk || (x.log_leafwise_grid && (x.log_leafwise_grid < 8 || x.log_leafwise_grid > 20)))
+        x.log_tree_chunk > kUpperLogChunk || (x.log_leafwise_grid && (x.log_leafwise_grid < 4 || x.log_leafwise_grid > 20)))
         return fail(AZK_ERR_INVALID_KERNEL_CONFIG, "execution plan outside the supported tiles (commit/fold 6..11, "
-                                                   "leafwise 3..8, chunk 3..10, leafwise grid 0 or 8..20)");
+                                                   "leafwise 3..8, chunk 3..10, leafwise grid 0 or 4..20)");
     return AZK_OK;
 }
 

@@ -84,11 +84,11 @@ constexpr uint32_t kUpperLogChunk = 10;     // largest upper-tree reduction chun
 constexpr uint32_t kCommitLogTreeChunk = 3;
 constexpr uint32_t kLeafwiseLogTile = 8;    // latency form: one leaf per thread, 256 per block
 constexpr uint32_t kLeafwiseMinLogTile = 3;
-// Leafwise tiles of 2^(k - 9) leaves for a 2^k layer (at least 8, at most 256): small layers
-// keep only levels of a few parents inside the fold block (the one-state-per-wave form),
-// large ones keep 256-leaf tiles for throughput; 2^21 commits measured 3% faster than
-// fixed 256-leaf tiles.
-constexpr uint32_t kLeafwiseLogGrid = 9;
+// Leafwise tiles of 2^(k - 8) leaves for a 2^k layer (at least 8, at most 256): small layers
+// keep only levels of a few parents inside the fold block, large ones keep 256-leaf tiles for
+// throughput.  Measured on 2^21 commits with the two-lanes-per-state permutation: grid 8
+// 2.085 ms, 7 2.095, 9 2.110, fixed 256-leaf tiles 2.104.
+constexpr uint32_t kLeafwiseLogGrid = 8;
 
 struct DeviceMem {
     void *p = nullptr;

@@ -278,7 +278,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
         {&ExecPlan::log_tile_commit, {9, 10, 11}},
         {&ExecPlan::log_leafwise_below, {17, 18, 19, 20, 21}},
         {&ExecPlan::log_tile_leafwise, {6, 7, 8}},
-        {&ExecPlan::log_leafwise_grid, {0, 8, 10, 11}},
+        {&ExecPlan::log_leafwise_grid, {0, 7, 9, 10}},
     };
     uint32_t raced = 1;
     for (const Knob &k : knobs) {

@@ -279,7 +279,7 @@ typedef struct azk_circle_exec_plan {
     uint32_t log_leafwise_grid;   /* a leafwise layer of 2^k outputs takes tiles of 2^(k - this)
                                      leaves (2^(k - this + ceil(log2 J)) in a batch of J jobs),
                                      clamped to 2^3..2^log_tile_leafwise; 0 = fixed tile, else
-                                     8..20 (9) */
+                                     4..20 (8) */
 } azk_circle_exec_plan;
 azk_status azk_circle_prover_set_exec_plan(azk_circle_prover *prover, const azk_circle_exec_plan *plan);
 azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *prover, azk_circle_exec_plan *plan);

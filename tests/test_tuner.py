@@ -55,7 +55,7 @@ def test_every_plan_gives_the_oracle_proof(ctx, log_n, rounds):
 def test_invalid_plan_rejected(ctx):
     p = azk.CircleCodewordPCSFRIProverV1(ctx, 10, 3, SEC, DIGEST)
     for bad in (dict(log_tile_commit=12), dict(log_tile_fold=5), dict(log_tile_leafwise=9), dict(log_tile_leafwise=2),
-                dict(log_tree_chunk=11), dict(log_leafwise_grid=7), dict(log_leafwise_grid=21)):
+                dict(log_tree_chunk=11), dict(log_leafwise_grid=3), dict(log_leafwise_grid=21)):
         with pytest.raises(azk.AZKError) as e:
             p.set_exec_plan(**bad)
         assert e.value.code == 10  # AZK_ERR_INVALID_KERNEL_CONFIG
