@@ -263,22 +263,21 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
         proof_bytes = align_up(proof_bytes + 4 * p->proof_words, 256);
     }
     R.nonce.assign(jobs.size(), 0);
-    azk_status s = R.tail_dev.ensure(tail_bytes);
-    if (!s) s = R.qidx_dev.ensure(qidx_bytes);
-    if (!s) s = R.proof_dev.ensure(proof_bytes);
-    if (!s) s = R.tail_host.ensure(tail_bytes);
+    // The tail (state + final layer), the query indices and the gathered proofs live in pinned
+    // host memory that the kernels read and write directly: no copy steps around the host step.
+    azk_status s = R.tail_host.ensure(tail_bytes);
     if (!s) s = R.qidx_host.ensure(qidx_bytes);
     if (!s) s = R.proof_host.ensure(proof_bytes);
     if (s) return s;
     for (size_t k = 0; k < jobs.size(); ++k) {
         azk_circle_prover *p = jobs[k].p;
-        uint32_t *tail = (uint32_t *)((uint8_t *)R.tail_dev.p + R.tail_off[k]);
+        uint32_t *tail = (uint32_t *)((uint8_t *)R.tail_host.p + R.tail_off[k]);
         R.tail.add(CopyDesc{(const uint32_t *)p->tstate, tail, 50, 0, 0}, 1);
         const uint64_t fw = 4 * p->nfin;
         R.tail.add(CopyDesc{(const uint32_t *)p->layers[p->d.rounds], tail + 50, fw, 0, 0}, (uint32_t)((fw + 1023) / 1024));
         // proof words start from the constant template; the gather fills the rest
-        uint32_t *proof = (uint32_t *)((uint8_t *)R.proof_dev.p + R.proof_off[k]);
-        HIP_TRY(hipMemcpy(proof, p->proof_template.data(), 4 * p->proof_words, hipMemcpyHostToDevice));
+        uint32_t *proof = (uint32_t *)((uint8_t *)R.proof_host.p + R.proof_off[k]);
+        memcpy(proof, p->proof_template.data(), 4 * p->proof_words);
         EmitParams e{};
         e.proof = proof;
         e.roots_off = p->roots_off;
@@ -295,7 +294,7 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
             e.layers[r] = r == 0 ? jobs[k].layer0() : p->layers[r];
         }
         e.final_layer = (const uint32_t *)p->layers[p->d.rounds];
-        e.query_pairs = (const uint32_t *)((uint8_t *)R.qidx_dev.p + R.qidx_off[k]);
+        e.query_pairs = (const uint32_t *)((uint8_t *)R.qidx_host.p + R.qidx_off[k]);
         e.queries = p->d.query_count;
         R.emit.add(e, 1 + p->d.query_count * p->d.rounds);
     }
@@ -336,8 +335,7 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
         for (auto &l : R.tree_r[r]) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log);
         clk.mark(PH_TREE);
     }
-    launch_copy(st, tab(R, R.tail), R.tail.n(), R.tail.blocks);
-    HIP_TRY(hipMemcpyAsync(R.tail_host.p, R.tail_dev.p, R.tail_dev.n, hipMemcpyDeviceToHost, st));
+    launch_copy(st, tab(R, R.tail), R.tail.n(), R.tail.blocks);  // into pinned host memory
     clk.mark(PH_TAIL);
     return AZK_OK;
 }
@@ -478,9 +476,8 @@ void patch_nonces(Runner &R)
 
 azk_status enqueue_b(Runner &R, hipStream_t st)
 {
-    HIP_TRY(hipMemcpyAsync(R.qidx_dev.p, R.qidx_host.p, R.qidx_dev.n, hipMemcpyHostToDevice, st));
+    // reads the query indices from, and writes the proofs into, pinned host memory
     launch_emit_proof(st, tab(R, R.emit), R.emit.n(), R.emit.blocks);
-    HIP_TRY(hipMemcpyAsync(R.proof_host.p, R.proof_dev.p, R.proof_dev.n, hipMemcpyDeviceToHost, st));
     return AZK_OK;
 }
 

@@ -82,8 +82,7 @@ struct Runner {
     std::vector<std::vector<Launch<TreeDesc>>> tree_r;
     Launch<CopyDesc> tail;
     Launch<EmitParams> emit;
-    DeviceMem tail_dev, qidx_dev, proof_dev;
-    PinnedMem tail_host, qidx_host, proof_host;
+    PinnedMem tail_host, qidx_host, proof_host;  // written / read by the kernels directly
     std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job
     std::vector<uint64_t> nonce;                         // grinding nonce per job (grinding_bits > 0)
     DeviceMem grind_dev;                                 // device nonce search: state + best
