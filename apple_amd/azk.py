@@ -74,7 +74,7 @@ class SumcheckResult(C.Structure):
 class ExecPlan(C.Structure):
     _fields_ = [("log_tile_commit", C.c_uint32), ("log_tile_fold", C.c_uint32), ("log_leafwise_below", C.c_uint32),
                 ("log_tile_leafwise", C.c_uint32), ("log_tree_chunk", C.c_uint32),
-                ("log_leafwise_grid", C.c_uint32)]
+                ("log_leafwise_grid", C.c_uint32), ("tree_waves", C.c_uint32)]
 
 
 class TuningConfig(C.Structure):

@@ -338,10 +338,8 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
 // Upper Merkle levels: each block loads 2^log_c nodes of `level` and reduces them
 // by log_c levels (MerkleCommitter.swift:853-893: parents per level + fused upper tree).
 // A block of a job's last step then runs the job's transcript program for that root.
-#ifndef AZK_TREE_THREADS
-#define AZK_TREE_THREADS 256
-#endif
-__global__ __launch_bounds__(AZK_TREE_THREADS) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
+// Blocks of 4, 8 or 16 waves (the execution plan's tree_waves).
+__global__ __launch_bounds__(1024) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
     uint32_t chunk;
@@ -915,12 +913,12 @@ void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_
     hipLaunchKernelGGL(k_fold_leafwise, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
-void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c)
+void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c, uint32_t waves)
 {
     if (!blocks) return;
     const size_t lds = (size_t)32 << max_log_c;  // 2^log_c nodes of 32 bytes, reduced in place
     // 4 waves (one per SIMD): 8-wave blocks with 16-node chunks measured no faster
-    hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(AZK_TREE_THREADS), lds, st, t, n);
+    hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(64 * waves), lds, st, t, n);
 }
 
 void launch_grind(hipStream_t st, const uint64_t *state, uint32_t bits, uint64_t base, uint32_t count, unsigned long long *best)

@@ -81,6 +81,7 @@ struct Runner {
     std::vector<Launch<FoldDesc>> fold_big, fold_leaf;
     std::vector<std::vector<Launch<TreeDesc>>> tree_r;
     Launch<CopyDesc> tail;
+    uint32_t tree_waves = 4;  // largest tree_waves of the jobs' plans
     Launch<EmitParams> emit;
     PinnedMem tail_host, qidx_host, proof_host;  // written / read by the kernels directly
     std::vector<size_t> tail_off, qidx_off, proof_off;  // bytes, per job

@@ -75,9 +75,10 @@ azk_status check_exec_plan(const azk_circle_exec_plan &x)
     if (x.log_tile_commit < 6 || x.log_tile_commit > kDefaultLogTile || x.log_tile_fold < 6 ||
         x.log_tile_fold > kDefaultLogTile || x.log_tile_leafwise < kLeafwiseMinLogTile ||
         x.log_tile_leafwise > kLeafwiseLogTile || x.log_leafwise_below > 31 || x.log_tree_chunk < 3 ||
-        x.log_tree_chunk > kUpperLogChunk || (x.log_leafwise_grid && (x.log_leafwise_grid < 4 || x.log_leafwise_grid > 20)))
+        x.log_tree_chunk > kUpperLogChunk || (x.log_leafwise_grid && (x.log_leafwise_grid < 4 || x.log_leafwise_grid > 20)) ||
+        (x.tree_waves != 4 && x.tree_waves != 8 && x.tree_waves != 16))
         return fail(AZK_ERR_INVALID_KERNEL_CONFIG, "execution plan outside the supported tiles (commit/fold 6..11, "
-                                                   "leafwise 3..8, chunk 3..10, leafwise grid 0 or 4..20)");
+                                                   "leafwise 3..8, chunk 3..10, leafwise grid 0 or 4..20, tree waves 4/8/16)");
     return AZK_OK;
 }
 
@@ -90,6 +91,7 @@ ExecPlan to_exec_plan(const azk_circle_exec_plan &x)
     e.log_tile_leafwise = x.log_tile_leafwise;
     e.log_tree_chunk = x.log_tree_chunk;
     e.log_leafwise_grid = x.log_leafwise_grid;
+    e.tree_waves = x.tree_waves;
     return e;
 }
 
@@ -102,6 +104,7 @@ azk_circle_exec_plan from_exec_plan(const ExecPlan &e)
     x.log_tile_leafwise = e.log_tile_leafwise;
     x.log_tree_chunk = e.log_tree_chunk;
     x.log_leafwise_grid = e.log_leafwise_grid;
+    x.tree_waves = e.tree_waves;
     return x;
 }
 

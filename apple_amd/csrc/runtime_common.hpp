@@ -245,6 +245,7 @@ struct ExecPlan {
     uint32_t log_tile_leafwise = kLeafwiseLogTile; // latency-form tile
     uint32_t log_tree_chunk = kCommitLogTreeChunk; // upper-tree reduction chunk
     uint32_t log_leafwise_grid = kLeafwiseLogGrid; // leafwise tiles shrink to keep ~2^this blocks
+    uint32_t tree_waves = 4;                       // waves per upper-tree block (4, 8, 16)
 };
 azk_status check_exec_plan(const azk_circle_exec_plan &x);
 ExecPlan to_exec_plan(const azk_circle_exec_plan &x);

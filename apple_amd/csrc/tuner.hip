@@ -69,7 +69,8 @@ std::string plan_json(const ExecPlan &x)
     std::ostringstream o;
     o << "{\"log_tile_commit\": " << x.log_tile_commit << ", \"log_tile_fold\": " << x.log_tile_fold
       << ", \"log_leafwise_below\": " << x.log_leafwise_below << ", \"log_tile_leafwise\": " << x.log_tile_leafwise
-      << ", \"log_tree_chunk\": " << x.log_tree_chunk << ", \"log_leafwise_grid\": " << x.log_leafwise_grid << "}";
+      << ", \"log_tree_chunk\": " << x.log_tree_chunk << ", \"log_leafwise_grid\": " << x.log_leafwise_grid
+      << ", \"tree_waves\": " << x.tree_waves << "}";
     return o.str();
 }
 
@@ -106,7 +107,8 @@ bool same_plan(const ExecPlan &a, const ExecPlan &b)
 {
     return a.log_tile_commit == b.log_tile_commit && a.log_tile_fold == b.log_tile_fold &&
            a.log_leafwise_below == b.log_leafwise_below && a.log_tile_leafwise == b.log_tile_leafwise &&
-           a.log_tree_chunk == b.log_tree_chunk && a.log_leafwise_grid == b.log_leafwise_grid;
+           a.log_tree_chunk == b.log_tree_chunk && a.log_leafwise_grid == b.log_leafwise_grid &&
+           a.tree_waves == b.tree_waves;
 }
 
 double quantile(std::vector<double> v, double q)
@@ -212,7 +214,8 @@ bool lookup(const azk_tuning_config &c, const azk_circle_prover *p, ExecPlan &pl
             read_field(line, "log_leafwise_below", x.log_leafwise_below) &&
             read_field(line, "log_tile_leafwise", x.log_tile_leafwise) &&
             read_field(line, "log_tree_chunk", x.log_tree_chunk) &&
-            read_field(line, "log_leafwise_grid", x.log_leafwise_grid) && !check_exec_plan(from_exec_plan(x))) {
+            read_field(line, "log_leafwise_grid", x.log_leafwise_grid) && read_field(line, "tree_waves", x.tree_waves) &&
+            !check_exec_plan(from_exec_plan(x))) {
             plan = x;
             median = read_field(line, "median_gpu_ns", med) ? med : 0;
             found = true;
@@ -231,7 +234,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
     if (config) c = *config;
     if (!c.warmup_runs) c.warmup_runs = 2;
     if (!c.measured_runs) c.measured_runs = 5;
-    if (!c.max_candidates) c.max_candidates = 24;
+    if (!c.max_candidates) c.max_candidates = 28;
     if (!c.validation_runs) c.validation_runs = 3;
     std::lock_guard<std::mutex> g(p->mu);
     azk_status s = check_range(coeffs, off, (size_t)16 << p->d.log_n);
@@ -274,6 +277,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
     struct Knob { uint32_t ExecPlan::*field; std::vector<uint32_t> values; };
     const std::vector<Knob> knobs = {
         {&ExecPlan::log_tree_chunk, {3, 4, 5, 6, 8, 10}},
+        {&ExecPlan::tree_waves, {8, 16}},
         {&ExecPlan::log_tile_fold, {9, 10, 11}},
         {&ExecPlan::log_tile_commit, {9, 10, 11}},
         {&ExecPlan::log_leafwise_below, {17, 18, 19, 20, 21}},

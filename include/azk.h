@@ -280,6 +280,7 @@ typedef struct azk_circle_exec_plan {
                                      leaves (2^(k - this + ceil(log2 J)) in a batch of J jobs),
                                      clamped to 2^3..2^log_tile_leafwise; 0 = fixed tile, else
                                      4..20 (8) */
+    uint32_t tree_waves;          /* waves per upper-tree block: 4, 8 or 16 (4) */
 } azk_circle_exec_plan;
 azk_status azk_circle_prover_set_exec_plan(azk_circle_prover *prover, const azk_circle_exec_plan *plan);
 azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *prover, azk_circle_exec_plan *plan);
@@ -287,7 +288,7 @@ azk_status azk_circle_prover_get_exec_plan(const azk_circle_prover *prover, azk_
 typedef struct azk_tuning_config {
     uint32_t warmup_runs;       /* 2 */
     uint32_t measured_runs;     /* 5 */
-    uint32_t max_candidates;    /* 24 plans raced (coordinate descent from the default) */
+    uint32_t max_candidates;    /* 28 plans raced (coordinate descent from the default) */
     uint32_t validation_runs;   /* 3 random inputs on which the winner must equal the default plan */
     const char *database_path;  /* JSON-lines plan records; NULL = no persistence */
     int force;                  /* 1 = race even when the database holds a winner */

@@ -24,6 +24,8 @@ PLANS = [
     dict(log_leafwise_grid=11),                        # leafwise tiles shrink with the layer
     dict(log_leafwise_grid=8, log_tile_leafwise=3),    # 8-leaf tiles throughout
     dict(log_leafwise_below=31, log_leafwise_grid=10, log_tree_chunk=3),
+    dict(tree_waves=8, log_tree_chunk=4),               # 8-wave upper-tree blocks
+    dict(tree_waves=16, log_tree_chunk=5, log_leafwise_grid=6),
 ]
 
 
@@ -44,7 +46,7 @@ def test_every_plan_gives_the_oracle_proof(ctx, log_n, rounds):
     for plan in PLANS:
         print(f"2^{log_n} plan {plan}", flush=True)
         p.set_exec_plan(log_tile_commit=11, log_tile_fold=11, log_leafwise_below=19, log_tile_leafwise=8,
-                        log_tree_chunk=10, log_leafwise_grid=0)
+                        log_tree_chunk=10, log_leafwise_grid=0, tree_waves=4)
         p.set_exec_plan(**plan)
         assert p.prove_fft_coefficients_resident(buf) == want, plan
         p.set_graph(False)
@@ -55,7 +57,7 @@ def test_every_plan_gives_the_oracle_proof(ctx, log_n, rounds):
 def test_invalid_plan_rejected(ctx):
     p = azk.CircleCodewordPCSFRIProverV1(ctx, 10, 3, SEC, DIGEST)
     for bad in (dict(log_tile_commit=12), dict(log_tile_fold=5), dict(log_tile_leafwise=9), dict(log_tile_leafwise=2),
-                dict(log_tree_chunk=11), dict(log_leafwise_grid=3), dict(log_leafwise_grid=21)):
+                dict(log_tree_chunk=11), dict(log_leafwise_grid=3), dict(log_leafwise_grid=21), dict(tree_waves=5)):
         with pytest.raises(azk.AZKError) as e:
             p.set_exec_plan(**bad)
         assert e.value.code == 10  # AZK_ERR_INVALID_KERNEL_CONFIG
