@@ -10,7 +10,8 @@
 // Device program for n = 2^k words (k rounds):
 //   k_sc_reduce0   even/odd sums of the input (+ count of words >= p), per block
 //   k_sc_round     one wave: sums the partials, absorbs the header (round 0), the round
-//                  frame and the challenge frame, squeezes the challenge (SALU Keccak)
+//                  frame and the challenge frame, squeezes the challenge (one-state-per-wave
+//                  Keccak)
 //   k_sc_fold      v'[i] = v[2i] + c (v[2i+1] - v[2i]) fused with the next round's sums
 //   k_sc_tail      the last rounds (<= 2^kTailLog words) in one block: LDS-resident
 //                  vector, block reductions, wave-0 transcript, LDS folds
