@@ -23,14 +23,15 @@ __device__ __forceinline__ uint32_t m31_sub(uint32_t a, uint32_t b)
 }
 __device__ __forceinline__ uint32_t m31_neg(uint32_t a) { return a ? M31_P - a : 0; }
 
-// a*b < 2^62: fold the 62-bit product twice at bit 31, then one conditional subtract.
-// The 64-bit product is one v_mad_u64_u32 (measured 15% more products per second on
-// gfx950 than the v_mul_lo_u32 / v_mul_hi_u32 pair).
+// For a, b <= p the product t <= p^2 < 2^62, so one fold at bit 31 gives
+// r = (t mod 2^31) + (t >> 31) <= (2^31 - 1) + (2^31 - 2) = 2p - 1, and one conditional
+// subtract (v_add + v_min_u32) makes it canonical: 6 VALU ops per product.  The 64-bit
+// product is one v_mad_u64_u32 (measured 15% more products per second on gfx950 than
+// the v_mul_lo_u32 / v_mul_hi_u32 pair).
 __device__ __forceinline__ uint32_t m31_mul(uint32_t a, uint32_t b)
 {
     const uint64_t t = (uint64_t)a * b;
-    uint32_t r = ((uint32_t)t & M31_P) + (uint32_t)(t >> 31); // < 2^32
-    r = (r & M31_P) + (r >> 31);                              // <= p
+    const uint32_t r = ((uint32_t)t & M31_P) + (uint32_t)(t >> 31); // <= 2p - 1
     return r >= M31_P ? r - M31_P : r;
 }
 

@@ -176,7 +176,7 @@ def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 300) -> d
     }
 
 
-FFT_BUTTERFLY_OPS = 70  # VALU instructions of one QM31 butterfly in this build's tile loop (gfx950 assembly)
+FFT_BUTTERFLY_OPS = 58  # VALU instructions of one QM31 butterfly in this build's tile loop (gfx950 assembly)
 
 
 def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
