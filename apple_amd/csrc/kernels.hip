@@ -465,12 +465,42 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *tws, const uint32_t *__
 // its twiddle is tws[2^(k-1-t) + (b >> (t + m))].  Reference stage loop:
 // CircleCodewordOracle.evaluateWithCircleFFT (CircleCodeword.swift:72-98), Metal
 // circle_codeword_fft_stage :1881-1908.
+//
+// Stages run in pairs (t, t - 1): group g of a pair holds the four elements i0 + j 2^sh
+// (sh = t - 1 + m, j < 4, i0 = g with two zero bits inserted at sh), so stage t's two
+// butterflies (j = 0/2 and 1/3, twiddle index i0 >> (sh + 2)) and stage t - 1's two
+// (0/1 and 2/3, twiddle indices 2 hp and 2 hp + 1) run in registers between one LDS load
+// and one store: half the LDS traffic and barriers of one stage per round trip.  An odd
+// stage count leaves stage 0 as a single radix-2 stage.
 __device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *tws, uint32_t k, uint32_t m)
 {
-    const uint32_t half = 1u << (k + m - 1);
-    for (int t = (int)k - 1; t >= 0; --t) {
-        const uint32_t sh = (uint32_t)t + m;
-        const uint32_t *tw_t = tws + (1u << (k - 1 - t));
+    int t = (int)k - 1;
+    for (; t >= 1; t -= 2) {
+        const uint32_t quarter = 1u << (k + m - 2), sh = (uint32_t)t - 1 + m, d = 1u << sh;
+        const uint32_t *tw_hi = tws + (1u << (k - 1 - t)), *tw_lo = tws + (1u << (k - t));
+        for (uint32_t g = threadIdx.x; g < quarter; g += blockDim.x) {
+            const uint32_t hp = g >> sh;
+            const uint32_t i0 = (hp << (sh + 2)) | (g & (d - 1));
+            qm31 a0 = ld_qm(s[i0]), a1 = ld_qm(s[i0 + d]), a2 = ld_qm(s[i0 + 2 * d]), a3 = ld_qm(s[i0 + 3 * d]);
+            const uint32_t w = tw_hi[hp], wa = tw_lo[2 * hp], wb = tw_lo[2 * hp + 1];
+            qm31 r = qm_scale(a2, w);
+            a2 = qm_sub(a0, r);
+            a0 = qm_add(a0, r);
+            r = qm_scale(a3, w);
+            a3 = qm_sub(a1, r);
+            a1 = qm_add(a1, r);
+            r = qm_scale(a1, wa);
+            s[i0 + d] = st_qm(qm_sub(a0, r));
+            s[i0] = st_qm(qm_add(a0, r));
+            r = qm_scale(a3, wb);
+            s[i0 + 3 * d] = st_qm(qm_sub(a2, r));
+            s[i0 + 2 * d] = st_qm(qm_add(a2, r));
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const uint32_t half = 1u << (k + m - 1), sh = m;
+        const uint32_t *tw_t = tws + (1u << (k - 1));
         for (uint32_t b = threadIdx.x; b < half; b += blockDim.x) {
             const uint32_t hp = b >> sh;
             const uint32_t i1 = (hp << (sh + 1)) | (b & ((1u << sh) - 1));

@@ -176,7 +176,27 @@ def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 300) -> d
     }
 
 
-FFT_BUTTERFLY_OPS = 58  # VALU instructions of one QM31 butterfly in this build's tile loop (gfx950 assembly)
+# VALU instructions per QM31 butterfly in this build's tile loops (gfx950 assembly): stages run in
+# radix-4 pairs (205 instructions per group of four butterflies); an odd stage count per pass leaves
+# one radix-2 stage (58 per butterfly in the column passes, 53 in the final contiguous pass).
+FFT_PAIR_BUTTERFLY_OPS = 205 / 4
+FFT_SINGLE_BUTTERFLY_OPS = (58, 53)
+
+
+def fft_stage_split(log_n: int, log_t_final: int = 11, log_t_pass: int = 11) -> list:
+    """Stage counts of the passes the codeword plan runs: passes of at most log_t_pass - 3
+    stages, balanced, then the final pass of min(log_t_final, log_n) stages."""
+    lt = min(log_t_final, log_n)
+    s, kmax = log_n - lt, log_t_pass - 3
+    p = (s + kmax - 1) // kmax if s else 0
+    return [s // p + (1 if i < s % p else 0) for i in range(p)] + [lt]
+
+
+def fft_valu_ops(log_n: int) -> float:
+    ks = fft_stage_split(log_n)
+    per_element_half = sum((k // 2) * 2 * FFT_PAIR_BUTTERFLY_OPS + (k % 2) * FFT_SINGLE_BUTTERFLY_OPS[i == len(ks) - 1]
+                           for i, k in enumerate(ks))
+    return (1 << (log_n - 1)) * per_element_half
 
 
 def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
@@ -198,7 +218,7 @@ def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
     fold.execute_resident(cout, ch, fout)
     t_fold = float(np.median([fold.execute_resident(cout, ch, fout)["gpu_ns"] for _ in range(reps)])) * 1e-9
     fold_bytes = 16 * n + 4 * (n // 2) + 16 * (n // 2)  # pairs in, M31 twiddles in, outputs
-    fft_ops = log_n * (n // 2) * FFT_BUTTERFLY_OPS
+    fft_ops = fft_valu_ops(log_n)
     for b in (cin, cout, fout):
         b.free()
     return {
@@ -210,7 +230,8 @@ def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
                 "algorithmic_gbps": 32 * n / t_fft / 1e9,
                 "roofline": {"bound": "valu", "achieved": fft_ops / t_fft / 1e12, "peak": VALU_PEAK_TOPS,
                              "unit": "TOP/s", "frac": fft_ops / t_fft / 1e12 / VALU_PEAK_TOPS,
-                             "ops": fft_ops, "ops_per_butterfly": FFT_BUTTERFLY_OPS}},
+                             "ops": fft_ops, "ops_per_butterfly": fft_ops / (log_n * (n // 2)),
+                             "stages_per_pass": fft_stage_split(log_n)}},
     }
 
 
