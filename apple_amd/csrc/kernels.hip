@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "device_field.hpp"
 #include "device_keccak.hpp"
 #include "kernels.hpp"
@@ -444,16 +446,16 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *tws, const uint32_t *__
         const uint32_t level = 31 - __builtin_clz(i), sg = lo + k - 1 - level;
         return (sg ? n - (n >> sg) : 0) + (high << level) + (i - (1u << level));
     };
-    const uint32_t S = 1u << k;
-    if (S >= 256) {
-        const uint32_t J = S >> 8;
+    const uint32_t S = 1u << k, B = blockDim.x;  // S <= 8B
+    if (S >= B) {
+        const uint32_t J = S / B;
         uint32_t v[8];
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j)
-            if (j < J) v[j] = g[src(threadIdx.x + 256u * j)];
+            if (j < J) v[j] = g[src(threadIdx.x + B * j)];
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j)
-            if (j < J) tws[threadIdx.x + 256u * j] = v[j];
+            if (j < J) tws[threadIdx.x + B * j] = v[j];
     } else if (threadIdx.x < S) {
         tws[threadIdx.x] = g[src(threadIdx.x)];
     }
@@ -510,30 +512,31 @@ __device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *tws, u
     }
 }
 
-// Loads T <= 2048 elements of a tile into LDS (index(L) maps tile slot L to the global
-// element).  For T >= 256 the trip count T / 256 is uniform, so a thread's loads are
+// Loads T <= 8 x blockDim elements of a tile into LDS (index(L) maps tile slot L to the
+// global element).  For T >= blockDim the trip count is uniform, so a thread's loads are
 // all in flight (global, not flat) before the first LDS store.
 template <class F>
 __device__ __forceinline__ void load_tile(uint4 *s, const uint4 *__restrict__ src, uint32_t T, F &&index)
 {
-    if (T >= 256) {
+    const uint32_t B = blockDim.x;
+    if (T >= B) {
         const global_u32x4 *g = (const global_u32x4 *)src;
-        const uint32_t J = T >> 8;
+        const uint32_t J = T / B;
         u32x4 v[8];
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j)
-            if (j < J) v[j] = g[index(threadIdx.x + 256u * j)];
+            if (j < J) v[j] = g[index(threadIdx.x + B * j)];
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j)
-            if (j < J) s[threadIdx.x + 256u * j] = make_uint4(v[j].x, v[j].y, v[j].z, v[j].w);
+            if (j < J) s[threadIdx.x + B * j] = make_uint4(v[j].x, v[j].y, v[j].z, v[j].w);
     } else if (threadIdx.x < T) {
         s[threadIdx.x] = src[index(threadIdx.x)];
     }
 }
 
 // Non-final pass (lo >= 1): tile index -> (high bits, column chunk).  LDS: T elements,
-// then the tile's 2^k twiddle slots.
-__global__ __launch_bounds__(256) void k_fft_pass(const FftPassDesc *__restrict__ tab, uint32_t ntab)
+// then the tile's 2^k <= 2^10 twiddle slots.  T / 8 threads per block, 256 to 1024.
+__global__ __launch_bounds__(1024) void k_fft_pass(const FftPassDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint4 s[];
     uint32_t tile;
@@ -919,8 +922,9 @@ void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint3
 void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
 {
     if (!blocks) return;
-    // T elements + 2^k <= T twiddle words
-    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(256), (size_t)20 << max_log_t, st, t, n);
+    // T elements + 2^k <= 2^10 twiddle words (passes have at most 10 stages)
+    const uint32_t threads = std::min(1024u, std::max(256u, (1u << max_log_t) / 8));
+    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(threads), ((size_t)16 << max_log_t) + 4096, st, t, n);
 }
 
 void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)

@@ -59,12 +59,14 @@ std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t l
     uint32_t lt = std::min(log_t_final, log_n);
     uint32_t S = log_n - lt;
     if (!S) return v;
-    uint32_t kmax = log_t_pass - 3;
+    const uint32_t kmax = log_t_pass - 3;
     uint32_t P = (S + kmax - 1) / kmax, hi = log_n;
     for (uint32_t p = 0; p < P; ++p) {
         uint32_t k = S / P + (p < S % P ? 1 : 0);
         uint32_t lo = hi - k;
-        v.push_back({lo, k, std::min(log_t_pass - k, lo)});  // columns come from the lo low bits
+        // 2^11-element tiles unless the pass needs more rows; columns come from the lo low bits
+        const uint32_t log_t = std::max(kDefaultLogTile, k + 3);
+        v.push_back({lo, k, std::min(log_t - k, lo)});
         hi = lo;
     }
     return v;

@@ -181,10 +181,13 @@ azk_status run_timed(azk_context *ctx, azk_stats *stats, F &&enqueue)
 
 struct FftPass { uint32_t lo, k, m; };
 
-// Stages log_n-1 .. lt (lt = min(log_t_final, log_n)) split into passes of at most
-// log_t_pass - 3 stages, each on tiles of up to 2^log_t_pass elements with >= 8
-// contiguous columns (fewer when the pass starts below stage log_t_pass - k).
-std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t log_t_pass = kDefaultLogTile);
+// Stages log_n-1 .. lt (lt = min(log_t_final, log_n)) split into the fewest balanced
+// passes of at most log_t_pass - 3 stages.  A pass of k stages runs on tiles of
+// 2^max(11, k + 3) elements: 2^k strided rows x >= 8 contiguous columns (128 bytes), so the
+// 10 stages above a 2^21 codeword's final pass take one 2^13-element pass (132 KB of LDS,
+// one block per CU) instead of two passes over HBM.
+constexpr uint32_t kMaxPassLogTile = 13;
+std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t log_t_pass = kMaxPassLogTile);
 
 // FFT twiddles (and, with `fold_tw`, the first `rounds` stages inverted = FRI fold twiddles).
 azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw);

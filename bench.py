@@ -183,7 +183,7 @@ FFT_PAIR_BUTTERFLY_OPS = 205 / 4
 FFT_SINGLE_BUTTERFLY_OPS = (58, 53)
 
 
-def fft_stage_split(log_n: int, log_t_final: int = 11, log_t_pass: int = 11) -> list:
+def fft_stage_split(log_n: int, log_t_final: int = 11, log_t_pass: int = 13) -> list:
     """Stage counts of the passes the codeword plan runs: passes of at most log_t_pass - 3
     stages, balanced, then the final pass of min(log_t_final, log_n) stages."""
     lt = min(log_t_final, log_n)
