@@ -341,11 +341,9 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
 // by log_c levels (MerkleCommitter.swift:853-893: parents per level + fused upper tree).
 // A block of a job's last step then runs the job's transcript program for that root.
 // Blocks of 4, 8 or 16 waves (the execution plan's tree_waves).
-__global__ __launch_bounds__(1024) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
+__device__ __forceinline__ void tree_reduce(const TreeDesc &d, uint32_t chunk)
 {
     extern __shared__ uint32_t lds[];
-    uint32_t chunk;
-    const TreeDesc &d = find_desc(tab, ntab, chunk);
     const uint32_t C = 1u << d.log_c;
     uint32_t *tree = d.tree;
     const uint64_t m = d.m;
@@ -363,6 +361,20 @@ __global__ __launch_bounds__(1024) void k_tree_reduce(const TreeDesc *__restrict
         __syncthreads();
         if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
     }
+}
+
+__global__ __launch_bounds__(1024) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
+{
+    uint32_t chunk;
+    const TreeDesc &d = find_desc(tab, ntab, chunk);
+    tree_reduce(d, chunk);
+}
+
+// A launch of one tree step takes its descriptor as a kernel argument: the node loads
+// then wait on no table read (one dependent HBM round trip less per launch).
+__global__ __launch_bounds__(1024) void k_tree_reduce1(const TreeDesc d)
+{
+    tree_reduce(d, blockIdx.x);
 }
 
 // Word copies, 1024 words per block.
@@ -951,12 +963,16 @@ void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_
     hipLaunchKernelGGL(k_fold_leafwise, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
-void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c, uint32_t waves)
+void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c, uint32_t waves,
+                 const TreeDesc *single)
 {
     if (!blocks) return;
     const size_t lds = (size_t)32 << max_log_c;  // 2^log_c nodes of 32 bytes, reduced in place
     // 4 waves (one per SIMD): 8-wave blocks with 16-node chunks measured no faster
-    hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(64 * waves), lds, st, t, n);
+    if (single && n == 1)
+        hipLaunchKernelGGL(k_tree_reduce1, dim3(blocks), dim3(64 * waves), lds, st, *single);
+    else
+        hipLaunchKernelGGL(k_tree_reduce, dim3(blocks), dim3(64 * waves), lds, st, t, n);
 }
 
 void launch_grind(hipStream_t st, const uint64_t *state, uint32_t bits, uint64_t base, uint32_t count, unsigned long long *best)

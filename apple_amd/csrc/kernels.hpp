@@ -125,7 +125,9 @@ void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_
 // No commitment: one output per thread, 256 outputs per block (log_t ignored).
 void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
 // waves per block: 4, 8 or 16
-void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c, uint32_t waves = 4);
+// `single`: host copy of the table's only entry (n == 1), passed by value to the kernel.
+void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c, uint32_t waves = 4,
+                 const TreeDesc *single = nullptr);
 void launch_transcript(hipStream_t st, const TranscriptDesc *t, uint32_t n);
 // Nonces base .. base + count - 1 (count a multiple of 256) against the grinding state.
 void launch_grind(hipStream_t st, const uint64_t *state, uint32_t bits, uint64_t base, uint32_t count,

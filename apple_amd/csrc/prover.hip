@@ -325,7 +325,7 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
     clk.mark(PH_FFT);
     launch_fft_final(st, tab(R, R.fft_final), R.fft_final.n(), R.fft_final.blocks, R.fft_final.max_log);
     clk.mark(PH_COMMIT0);
-    for (auto &l : R.tree0) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves);
+    for (auto &l : R.tree0) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves, l.v.data());
     clk.mark(PH_TREE);
     for (uint32_t r = 0; r < R.rounds; ++r) {
         launch_transcript(st, tab(R, R.tr_a[r]), R.tr_a[r].n());
@@ -334,7 +334,7 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
         launch_fold(st, tab(R, fb), fb.n(), fb.blocks, fb.max_log);
         launch_fold_leafwise(st, tab(R, fl), fl.n(), fl.blocks);
         clk.mark(PH_FOLD);
-        for (auto &l : R.tree_r[r]) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves);
+        for (auto &l : R.tree_r[r]) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves, l.v.data());
         clk.mark(PH_TREE);
     }
     launch_copy(st, tab(R, R.tail), R.tail.n(), R.tail.blocks);  // into pinned host memory

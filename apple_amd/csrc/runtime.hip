@@ -581,7 +581,7 @@ static azk_status merkle_enqueue(azk_merkle_plan *p, const void *leaves)
     const bool aligned = ((p->d.leaf_stride & 3u) == 0) && (((uintptr_t)leaves & 3u) == 0);
     hipStream_t st = p->ctx->stream;
     launch_merkle_raw(st, o.arena.at<MerkleRawDesc>(o.raw.off), o.raw.n(), o.raw.blocks, aligned, o.raw.max_log);
-    for (auto &t : o.trees) launch_tree(st, o.arena.at<TreeDesc>(t.off), t.n(), t.blocks, t.max_log);
+    for (auto &t : o.trees) launch_tree(st, o.arena.at<TreeDesc>(t.off), t.n(), t.blocks, t.max_log, 4, t.v.data());
     return AZK_OK;
 }
 
@@ -718,7 +718,7 @@ extern "C" azk_status azk_prehashed_commit_resident(azk_prehashed_plan *p, const
     s = run_timed(p->ctx, stats, [&]() {
         hipStream_t st = p->ctx->stream;
         launch_copy(st, o.arena.at<CopyDesc>(o.copy.off), o.copy.n(), o.copy.blocks);
-        for (auto &t : o.trees) launch_tree(st, o.arena.at<TreeDesc>(t.off), t.n(), t.blocks, t.max_log);
+        for (auto &t : o.trees) launch_tree(st, o.arena.at<TreeDesc>(t.off), t.n(), t.blocks, t.max_log, 4, t.v.data());
         return AZK_OK;
     });
     if (s) return s;
