@@ -611,11 +611,9 @@ __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restric
 // and two waves per SIMD hide latency that one wave leaves exposed.  Measured on MI355X: 175 us
 // against 193 us at 256 threads and 188 us at 1024; the fold to 2^20 outputs 302 / 307 / 368 us.
 constexpr uint32_t kFoldThreads = 512;
-__global__ __launch_bounds__(kFoldThreads) void k_fold(const FoldDesc *__restrict__ tab, uint32_t ntab)
+__device__ __forceinline__ void fold_tile(const FoldDesc &d, uint32_t tile)
 {
     extern __shared__ uint32_t lds[];
-    uint32_t tile;
-    const FoldDesc &d = find_desc(tab, ntab, tile);
     if (d.t_ops) {  // block-uniform: a transcript block, no barrier below is reached
         if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
         return;
@@ -641,6 +639,29 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(const FoldDesc *__restric
     tile_tree_qm31(lds, T, base, tree, d.n_out);
 }
 
+__global__ __launch_bounds__(kFoldThreads) void k_fold(const FoldDesc *__restrict__ tab, uint32_t ntab)
+{
+    uint32_t tile;
+    const FoldDesc &d = find_desc(tab, ntab, tile);
+    fold_tile(d, tile);
+}
+
+// A fold launch of one or two entries (a job's transcript block and its fold) takes them as
+// kernel arguments, so the first loads wait on no table read.
+__device__ __forceinline__ FoldDesc pick(const FoldPair &p, uint32_t &local)
+{
+    const FoldDesc d = (p.n > 1 && blockIdx.x >= p.d[1].first_block) ? p.d[1] : p.d[0];
+    local = blockIdx.x - d.first_block;
+    return d;
+}
+
+__global__ __launch_bounds__(kFoldThreads) void k_fold_pair(const FoldPair p)
+{
+    uint32_t tile;
+    const FoldDesc d = pick(p, tile);
+    fold_tile(d, tile);
+}
+
 // Streaming form when no commitment follows (the fold chain plan): one output per
 // thread over a full grid of 256-output blocks, measured fastest on MI355X (5.9 TB/s
 // at 2^26 -> 2^25 against 4.5-5.1 TB/s for grid-stride loops).
@@ -659,11 +680,9 @@ __global__ __launch_bounds__(256) void k_fold_stream(const FoldDesc *__restrict_
 // Latency form for small layers: one output element per thread, T <= 256 outputs
 // per block; each thread folds its pair, hashes the new leaf and the block reduces
 // its leaf hashes in LDS (serial chain: one leaf permutation + log2 T levels).
-__global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restrict__ tab, uint32_t ntab)
+__device__ __forceinline__ void fold_leafwise_tile(const FoldDesc &d, uint32_t tile)
 {
     __shared__ uint32_t A[8 * 256];
-    uint32_t tile;
-    const FoldDesc &d = find_desc(tab, ntab, tile);
     if (d.t_ops) {  // block-uniform: a transcript block, no barrier below is reached
         if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
         return;
@@ -714,6 +733,20 @@ __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restric
     if (!tree) return;
     __syncthreads();
     reduce_lds(A, T, 0, base, tree, n_out);
+}
+
+__global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restrict__ tab, uint32_t ntab)
+{
+    uint32_t tile;
+    const FoldDesc &d = find_desc(tab, ntab, tile);
+    fold_leafwise_tile(d, tile);
+}
+
+__global__ __launch_bounds__(256) void k_fold_leafwise_pair(const FoldPair p)
+{
+    uint32_t tile;
+    const FoldDesc d = pick(p, tile);
+    fold_leafwise_tile(d, tile);
 }
 
 // ============================================================== transcript
@@ -945,10 +978,24 @@ void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_
     hipLaunchKernelGGL(k_fft_final, dim3(blocks), dim3(256), fft_tile_lds_bytes(max_log_t), st, t, n);
 }
 
-void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
+static bool fold_pair(const FoldDesc *host, uint32_t n, FoldPair &p)
+{
+    if (!host || n == 0 || n > 2) return false;
+    p = FoldPair{};
+    p.d[0] = host[0];
+    if (n > 1) p.d[1] = host[1];
+    p.n = n;
+    return true;
+}
+
+void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, const FoldDesc *host)
 {
     if (!blocks) return;
-    hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(kFoldThreads), tile_lds_bytes(max_log_t), st, t, n);
+    FoldPair p;
+    if (fold_pair(host, n, p))
+        hipLaunchKernelGGL(k_fold_pair, dim3(blocks), dim3(kFoldThreads), tile_lds_bytes(max_log_t), st, p);
+    else
+        hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(kFoldThreads), tile_lds_bytes(max_log_t), st, t, n);
 }
 
 void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks)
@@ -957,10 +1004,14 @@ void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t 
     hipLaunchKernelGGL(k_fold_stream, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
-void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks)
+void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, const FoldDesc *host)
 {
     if (!blocks) return;
-    hipLaunchKernelGGL(k_fold_leafwise, dim3(blocks), dim3(256), 0, st, t, n);
+    FoldPair p;
+    if (fold_pair(host, n, p))
+        hipLaunchKernelGGL(k_fold_leafwise_pair, dim3(blocks), dim3(256), 0, st, p);
+    else
+        hipLaunchKernelGGL(k_fold_leafwise, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
 void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c, uint32_t waves,

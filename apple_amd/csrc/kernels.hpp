@@ -62,6 +62,12 @@ struct FoldDesc {
     uint32_t t_nops, pad;
 };
 
+// One or two fold entries passed by value (n of them).
+struct FoldPair {
+    FoldDesc d[2];
+    uint32_t n, pad;
+};
+
 // Reduction of chunks of 2^log_c nodes of `level` up by log_c levels.
 struct TreeDesc {
     uint32_t *tree;
@@ -120,8 +126,10 @@ void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t 
 void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t);
 void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
 void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
-void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
-void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
+// `host`: host copy of the table; a launch of at most two entries passes them by value.
+void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t,
+                 const FoldDesc *host = nullptr);
+void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, const FoldDesc *host = nullptr);
 // No commitment: one output per thread, 256 outputs per block (log_t ignored).
 void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
 // waves per block: 4, 8 or 16

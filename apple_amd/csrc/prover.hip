@@ -331,8 +331,8 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
         launch_transcript(st, tab(R, R.tr_a[r]), R.tr_a[r].n());
         clk.mark(PH_TRANSCRIPT);
         auto &fb = R.fold_big[r], &fl = R.fold_leaf[r];
-        launch_fold(st, tab(R, fb), fb.n(), fb.blocks, fb.max_log);
-        launch_fold_leafwise(st, tab(R, fl), fl.n(), fl.blocks);
+        launch_fold(st, tab(R, fb), fb.n(), fb.blocks, fb.max_log, fb.v.data());
+        launch_fold_leafwise(st, tab(R, fl), fl.n(), fl.blocks, fl.v.data());
         clk.mark(PH_FOLD);
         for (auto &l : R.tree_r[r]) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves, l.v.data());
         clk.mark(PH_TREE);
