@@ -501,6 +501,11 @@ def main() -> int:
                 "traffic": pmc_traffic("k_fft_final"),
                 "traffic_source": "profiles/pmc_bytes_per_dispatch_2^21.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
                 "algorithmic_ops_per_launch": perms0 * KECCAK_OPS, "avg_launch_ms": t0_s * 1e3}
+    # the same launch against HBM: 2^21 x 16 B codeword read and written, 2^19 x 16 B twiddles,
+    # every tree node below the tile roots written (32 B each) -- far below the HBM roof
+    hbm_bytes = 2 * n * 16 + (n >> 2) * 16 + perms0 * 32
+    roofline["hbm_view"] = {"bound": "hbm", "achieved": hbm_bytes / t0_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": hbm_bytes / t0_s / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": hbm_bytes}
 
     line = {
         "metric": METRIC, "value": commits_per_s, "unit": "commits/s", "n_gpus": 1, "steps": args.steps,
