@@ -107,6 +107,17 @@ class JobInfo(C.Structure):
                 ("submit_ns", C.c_uint64), ("admit_ns", C.c_uint64), ("start_ns", C.c_uint64), ("done_ns", C.c_uint64)]
 
 
+class PolynomialClaim(C.Structure):
+    _fields_ = [("log_n", C.c_uint32), ("x_count", C.c_uint32), ("y_count", C.c_uint32), ("claim_count", C.c_uint32),
+                ("x_coeffs", C.c_void_p), ("y_coeffs", C.c_void_p), ("storage_indices", C.c_void_p),
+                ("points", C.c_void_p), ("values", C.c_void_p)]
+
+
+class ParameterSet(C.Structure):
+    _fields_ = [("log_blowup", C.c_uint32), ("queries", C.c_uint32), ("grinding_bits", C.c_uint32),
+                ("target_soundness_bits", C.c_uint32)]
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [("fft_ns", C.c_uint64), ("commit0_ns", C.c_uint64), ("fold_ns", C.c_uint64),
                 ("tree_ns", C.c_uint64), ("transcript_ns", C.c_uint64), ("emit_ns", C.c_uint64),
@@ -206,6 +217,12 @@ def load() -> C.CDLL:
         "azk_fold_execute_verified": (st, [vp, vp, sz, vp, vp, sz, sp]),
         "azk_witness_basis_execute_verified": (st, [vp, vp, u32, vp, u32, vp, sz, sp]),
         "azk_host_circle_fft": (st, [vp, u32, vp]),
+        "azk_circle_grinding_digest_meets_target": (C.c_int, [vp, sz, u32]),
+        "azk_circle_polynomial_claim_make": (st, [C.POINTER(PolynomialClaim), vp, vp]),
+        "azk_circle_polynomial_claim_digest": (st, [C.POINTER(PolynomialClaim), vp]),
+        "azk_circle_verify_polynomial": (st, [vp, sz, C.POINTER(PolynomialClaim), C.POINTER(C.c_int)]),
+        "azk_circle_verify_contract": (st, [vp, sz, C.POINTER(ParameterSet), C.POINTER(PolynomialClaim),
+                                            C.POINTER(C.c_int)]),
         "azk_chain_plan_create": (st, [vp, C.POINTER(ProverDesc), C.POINTER(vp)]),
         "azk_chain_plan_destroy": (None, [vp]),
         "azk_chain_execute_resident": (st, [vp, vp, sz, vp, vp, vp, sz, sp]),
@@ -850,6 +867,67 @@ def circle_verify_with_coefficients(proof: bytes, public_input_digest: bytes, co
     c = np.ascontiguousarray(coeffs, dtype=np.uint32).reshape(-1)
     ok = C.c_int(0)
     _ck(lib.azk_circle_verify_with_coefficients(b.ctypes.data, len(proof), d.ctypes.data, _ptr(c), c.size, C.byref(ok)))
+    return bool(ok.value)
+
+
+def grinding_digest_meets_target(digest: bytes, bits: int) -> bool:
+    """CircleFRIGrindingV1.digestMeetsTarget."""
+    d = np.frombuffer(digest, np.uint8) if len(digest) else np.zeros(1, np.uint8)
+    return bool(load().azk_circle_grinding_digest_meets_target(d.ctypes.data, len(digest), bits))
+
+
+class CirclePCSFRIPolynomialClaimV1:
+    """A polynomial (monomial x / y QM31 coefficients) on the canonical 2^log_n domain with
+    evaluation claims at storage indices; `points` / `values` default to what
+    CirclePCSFRIPolynomialClaimV1.make computes."""
+
+    def __init__(self, log_n: int, x_coeffs, y_coeffs, storage_indices, points=None, values=None):
+        self.log_n = log_n
+        self.x = np.ascontiguousarray(np.asarray(x_coeffs, dtype=np.uint32).reshape(-1, 4))
+        self.y = np.ascontiguousarray(np.asarray(y_coeffs, dtype=np.uint32).reshape(-1, 4))
+        self.indices = np.ascontiguousarray(np.asarray(storage_indices, dtype=np.uint64).reshape(-1))
+        k = self.indices.size
+        if points is None or values is None:
+            self.points = np.zeros((k, 2), np.uint32)
+            self.values = np.zeros((k, 4), np.uint32)
+            _ck(load().azk_circle_polynomial_claim_make(C.byref(self._struct()), _ptr(self.points), _ptr(self.values)))
+        if points is not None:
+            self.points = np.ascontiguousarray(np.asarray(points, dtype=np.uint32).reshape(k, 2))
+        if values is not None:
+            self.values = np.ascontiguousarray(np.asarray(values, dtype=np.uint32).reshape(k, 4))
+
+    def _struct(self) -> PolynomialClaim:
+        def p(a):
+            return a.ctypes.data if a.size else None
+        pts = getattr(self, "points", None)
+        vals = getattr(self, "values", None)
+        return PolynomialClaim(self.log_n, len(self.x), len(self.y), self.indices.size, p(self.x), p(self.y),
+                               p(self.indices), p(pts) if pts is not None else None,
+                               p(vals) if vals is not None else None)
+
+    def digest(self) -> bytes:
+        """CirclePCSFRIPolynomialClaimDigestV1.digest: the public-input digest of the claim."""
+        out = np.zeros(32, np.uint8)
+        _ck(load().azk_circle_polynomial_claim_digest(C.byref(self._struct()), _ptr(out)))
+        return out.tobytes()
+
+
+def circle_verify_polynomial(proof: bytes, claim: CirclePCSFRIPolynomialClaimV1) -> bool:
+    """CirclePCSFRIPolynomialVerifierV1.verify."""
+    b = np.frombuffer(proof, np.uint8) if len(proof) else np.zeros(1, np.uint8)
+    ok = C.c_int(0)
+    _ck(load().azk_circle_verify_polynomial(b.ctypes.data, len(proof), C.byref(claim._struct()), C.byref(ok)))
+    return bool(ok.value)
+
+
+def circle_verify_contract(proof: bytes, claim: CirclePCSFRIPolynomialClaimV1, log_blowup: int = 4, queries: int = 36,
+                           grinding_bits: int = 0, target_soundness_bits: int = 128) -> bool:
+    """CirclePCSFRIContractVerifierV1.verify(encodedProof:statement:); the defaults are the
+    conservative-128 profile (CircleProofFormat.swift:238-250)."""
+    b = np.frombuffer(proof, np.uint8) if len(proof) else np.zeros(1, np.uint8)
+    ps = ParameterSet(log_blowup, queries, grinding_bits, target_soundness_bits)
+    ok = C.c_int(0)
+    _ck(load().azk_circle_verify_contract(b.ctypes.data, len(proof), C.byref(ps), C.byref(claim._struct()), C.byref(ok)))
     return bool(ok.value)
 
 

@@ -8,6 +8,11 @@
 // CirclePCSFRIProofVerifierV1.verify (:1397-1546).  Malformed bytes are a layout error
 // (the reference's decode throws); a well-formed proof that does not verify is
 // reported as not accepted (the reference returns false).
+//
+// Beside it, the statement-level checks a consumer of a polynomial commitment runs:
+// CirclePCSFRIPolynomialClaimDigestV1, CirclePCSFRIPolynomialVerifierV1 and
+// CirclePCSFRIContractVerifierV1 (:1095-1151,1548-1752).
+#include <new>
 #include <vector>
 
 #include "host_field.hpp"
@@ -105,6 +110,9 @@ azk_status decode(const uint8_t *data, size_t len, Proof &pf)
     for (uint32_t k = 0; k < R; ++k) pf.roots.push_back(r.bytes(32));
     const uint64_t nfin = r.u64();
     if (!r.ok || nfin != (1ull << pf.dom.log_n) >> R) return bad("final layer size");
+    // sizes come from untrusted counts: the bytes must be there before anything is allocated
+    // (CanonicalByteReader.readBytes throws on short input, CircleProofFormat.swift:550-561)
+    if (16 * nfin > r.n - r.at) return bad("final layer truncated");
     pf.final_layer.resize(nfin);
     for (auto &q : pf.final_layer) {
         q = r.qm();
@@ -113,6 +121,10 @@ azk_status decode(const uint8_t *data, size_t len, Proof &pf)
     const uint32_t nq = r.u32();
     if (!r.ok || nq != pf.queries) return bad("query count");
     const uint64_t n = 1ull << pf.dom.log_n;
+    // smallest encoded query: pair + layer count, then per layer index + pair + two openings
+    // of at least one sibling each (leaf index, value, count, 32 bytes)
+    const uint64_t min_query = 12 + (uint64_t)R * (12 + 2 * (28 + 32));
+    if ((uint64_t)nq * min_query > r.n - r.at) return bad("queries truncated");
     pf.qs.resize(nq);
     for (Query &q : pf.qs) {
         q.pair = r.u64();
@@ -218,10 +230,7 @@ bool derive(const Proof &pf, std::vector<Q> &challenges, std::vector<uint32_t> &
         t.absorb(nb, 8);
         uint8_t dg[32];
         memcpy(dg, t.s, 32);
-        uint32_t bits = pf.grinding, i = 0;
-        for (; bits >= 8; bits -= 8, ++i)
-            if (dg[i]) return false;
-        if (bits && (dg[i] & (uint8_t)(0xff << (8 - bits)))) return false;
+        if (!azk_circle_grinding_digest_meets_target(dg, 32, pf.grinding)) return false;
     }
     f = frame(6);
     put32(f, pf.queries);
@@ -293,23 +302,27 @@ extern "C" azk_status azk_circle_verify_with_coefficients(const uint8_t *proof, 
 {
     CHECK_ARG(proof && public_input_digest && coeffs && accepted);
     *accepted = 0;
-    Proof pf;
-    azk_status s = decode(proof, len, pf);
-    if (s) return s;
-    const size_t n = (size_t)1 << pf.dom.log_n;
-    if (pf.dom.log_n > 26 || coeff_words != 4 * n)
-        return fail(AZK_ERR_INVALID_LAYOUT, "need %zu coefficient words for the proof's 2^%u domain", 4 * n, pf.dom.log_n);
-    for (size_t i = 0; i < coeff_words; ++i)
-        if (coeffs[i] >= kP) return fail(AZK_ERR_INVALID_LAYOUT, "non-canonical coefficient");
-    if (!verify(pf, public_input_digest)) return AZK_OK;
-    std::vector<uint32_t> cw(4 * n);
-    host::circle_fft(coeffs, pf.dom.log_n, cw.data());
-    auto same = [&](const Opening &o) { return memcmp(&o.value, cw.data() + 4 * o.leaf, 16) == 0; };
-    for (const Query &q : pf.qs)
-        if (!same(q.layers[0].left) || !same(q.layers[0].right)) return AZK_OK;
-    for (const Opening &o : pf.claimed)
-        if (!same(o)) return AZK_OK;
-    *accepted = 1;
+    try {
+        Proof pf;
+        azk_status s = decode(proof, len, pf);
+        if (s) return s;
+        const size_t n = (size_t)1 << pf.dom.log_n;
+        if (pf.dom.log_n > 26 || coeff_words != 4 * n)
+            return fail(AZK_ERR_INVALID_LAYOUT, "need %zu coefficient words for the proof's 2^%u domain", 4 * n, pf.dom.log_n);
+        for (size_t i = 0; i < coeff_words; ++i)
+            if (coeffs[i] >= kP) return fail(AZK_ERR_INVALID_LAYOUT, "non-canonical coefficient");
+        if (!verify(pf, public_input_digest)) return AZK_OK;
+        std::vector<uint32_t> cw(4 * n);
+        host::circle_fft(coeffs, pf.dom.log_n, cw.data());
+        auto same = [&](const Opening &o) { return memcmp(&o.value, cw.data() + 4 * o.leaf, 16) == 0; };
+        for (const Query &q : pf.qs)
+            if (!same(q.layers[0].left) || !same(q.layers[0].right)) return AZK_OK;
+        for (const Opening &o : pf.claimed)
+            if (!same(o)) return AZK_OK;
+        *accepted = 1;
+    } catch (const std::bad_alloc &) {
+        return fail(AZK_ERR_ALLOCATION, "Circle PCS proof: out of host memory");
+    }
     return AZK_OK;
 }
 
@@ -318,9 +331,222 @@ extern "C" azk_status azk_circle_verify(const uint8_t *proof, size_t len, const 
 {
     CHECK_ARG(proof && public_input_digest && accepted);
     *accepted = 0;
-    Proof pf;
-    azk_status s = decode(proof, len, pf);
+    try {
+        Proof pf;
+        azk_status s = decode(proof, len, pf);
+        if (s) return s;
+        *accepted = verify(pf, public_input_digest) ? 1 : 0;
+    } catch (const std::bad_alloc &) {
+        return fail(AZK_ERR_ALLOCATION, "Circle PCS proof: out of host memory");
+    }
+    return AZK_OK;
+}
+
+// ====================================================================== grinding target
+
+extern "C" int azk_circle_grinding_digest_meets_target(const uint8_t *digest, size_t len, uint32_t bits)
+{
+    if ((uint64_t)bits > 8 * (uint64_t)len || (!digest && bits)) return 0;
+    size_t i = 0;
+    for (; bits >= 8; bits -= 8, ++i)
+        if (digest[i]) return 0;
+    return !bits || (digest[i] & (uint8_t)(0xffu << (8 - bits))) == 0;
+}
+
+// ====================================================================== polynomial claims
+
+namespace {
+
+// CirclePCSFRIPolynomialClaimV1.init, CircleCodewordPolynomial.init and
+// CirclePCSFRIEvaluationClaimV1.init (CircleProofFormat.swift:1000-1046, CircleCodeword.swift:7-20):
+// with `with_values`, the claimed points must lie on the circle and the values be canonical.
+azk_status check_claim(const azk_circle_polynomial_claim *c, bool with_values)
+{
+    auto bad = [](const char *what) { return fail(AZK_ERR_INVALID_LAYOUT, "polynomial claim: %s", what); };
+    if (!c) return fail(AZK_ERR_INVALID_ARGUMENT, "null claim");
+    if (c->log_n < 1 || c->log_n > 30) return bad("domain log size outside 1..30");
+    if (!c->x_count && !c->y_count) return bad("no coefficients");
+    if ((c->x_count && !c->x_coeffs) || (c->y_count && !c->y_coeffs)) return fail(AZK_ERR_INVALID_ARGUMENT, "null coefficients");
+    for (size_t i = 0; i < 4ull * c->x_count; ++i)
+        if (c->x_coeffs[i] >= kP) return bad("non-canonical x coefficient");
+    for (size_t i = 0; i < 4ull * c->y_count; ++i)
+        if (c->y_coeffs[i] >= kP) return bad("non-canonical y coefficient");
+    if (!c->claim_count) return bad("no evaluation claims");
+    if (!c->storage_indices) return fail(AZK_ERR_INVALID_ARGUMENT, "null storage indices");
+    const uint64_t n = 1ull << c->log_n;
+    for (uint32_t k = 0; k < c->claim_count; ++k)
+        if (c->storage_indices[k] >= n || (k && c->storage_indices[k] <= c->storage_indices[k - 1]))
+            return bad("storage indices not strictly increasing inside the domain");
+    if (with_values) {
+        if (!c->points || !c->values) return fail(AZK_ERR_INVALID_ARGUMENT, "null claim points or values");
+        for (uint32_t k = 0; k < c->claim_count; ++k) {
+            const uint32_t x = c->points[2 * k], y = c->points[2 * k + 1];
+            if (x >= kP || y >= kP || fadd(fmul(x, x), fmul(y, y)) != 1) return bad("claimed point off the circle");
+            for (int w = 0; w < 4; ++w)
+                if (c->values[4 * k + w] >= kP) return bad("non-canonical claimed value");
+        }
+    }
+    return AZK_OK;
+}
+
+// CircleCodewordOracle.evaluate(polynomial:at:) (CircleCodeword.swift:53-70,173-185):
+// X(x) + y Y(x), Horner in x over QM31 coefficients.
+Q evaluate_at(const azk_circle_polynomial_claim *c, Pt p)
+{
+    auto horner = [&](const uint32_t *co, uint32_t count) {
+        Q acc{0, 0, 0, 0};
+        for (uint32_t i = count; i-- > 0;) acc = qadd(qscale(acc, p.x), Q{co[4 * i], co[4 * i + 1], co[4 * i + 2], co[4 * i + 3]});
+        return acc;
+    };
+    return qadd(horner(c->x_coeffs, c->x_count), qscale(horner(c->y_coeffs, c->y_count), p.y));
+}
+
+// CirclePCSFRIPolynomialClaimDigestV1 frames (:1118-1150): u32 len || domain string || u8 type.
+std::vector<uint8_t> claim_frame(uint8_t type)
+{
+    static const char dom[] = "AppleZKProver.CirclePCS.PolynomialClaim.V1";
+    std::vector<uint8_t> f;
+    put32(f, (uint32_t)(sizeof dom - 1));
+    f.insert(f.end(), dom, dom + sizeof dom - 1);
+    f.push_back(type);
+    return f;
+}
+
+void claim_digest(const azk_circle_polynomial_claim *c, uint8_t out[32])
+{
+    static const char dom[] = "AppleZKProver.CirclePCS.PolynomialClaim.V1";
+    std::vector<uint8_t> m = claim_frame(0);  // domain frame: the string again, proof and transcript versions
+    put32(m, (uint32_t)(sizeof dom - 1));
+    m.insert(m.end(), dom, dom + sizeof dom - 1);
+    put32(m, 1);
+    put32(m, 1);
+    m.insert(m.end(), kDescMagic, kDescMagic + 8);  // CircleDomainDescriptorCodecV1.encode
+    for (uint32_t w : {1u, kP, 31u, 2u, 1268011823u, c->log_n, 1u << (30 - c->log_n), c->log_n - 1, 1u}) put32(m, w);
+    std::vector<uint8_t> f = claim_frame(1);  // polynomial frame: counts and the 16-byte element size
+    put32(f, c->x_count);
+    put32(f, c->y_count);
+    put32(f, 16);
+    m.insert(m.end(), f.begin(), f.end());
+    for (size_t i = 0; i < 4ull * c->x_count; ++i) put32(m, c->x_coeffs[i]);
+    for (size_t i = 0; i < 4ull * c->y_count; ++i) put32(m, c->y_coeffs[i]);
+    f = claim_frame(2);  // evaluation-claim frame
+    put32(f, c->claim_count);
+    m.insert(m.end(), f.begin(), f.end());
+    for (uint32_t k = 0; k < c->claim_count; ++k) {
+        put64(m, c->storage_indices[k]);
+        put32(m, c->points[2 * k]);
+        put32(m, c->points[2 * k + 1]);
+        for (int w = 0; w < 4; ++w) put32(m, c->values[4 * k + w]);
+    }
+    host::sha3_256(m.data(), m.size(), out);
+}
+
+// CirclePCSFRIPolynomialVerifierV1.verify (:1548-1659) on a decoded proof.
+bool verify_polynomial(const Proof &pf, const azk_circle_polynomial_claim *c)
+{
+    uint8_t digest[32];
+    claim_digest(c, digest);
+    if (memcmp(pf.digest, digest, 32) || pf.dom.log_n != c->log_n || pf.claimed.size() != c->claim_count) return false;
+    // coefficientCountsFitDomainAndSecurity (:1585-1599)
+    if (pf.log_blowup > pf.dom.log_n) return false;
+    const uint64_t cap = std::max<uint64_t>(1, (1ull << pf.dom.log_n) >> pf.log_blowup);
+    if (c->x_count > cap || c->y_count > cap) return false;
+    if (!verify(pf, digest)) return false;
+    const Domain dom = Domain::canonical_of(c->log_n);
+    for (uint32_t k = 0; k < c->claim_count; ++k) {
+        const Opening &o = pf.claimed[k];
+        const Pt want = dom.storage_point(c->storage_indices[k]);  // claimMatchesDomainAndPolynomial (:1601-1625)
+        const Q value{c->values[4 * k], c->values[4 * k + 1], c->values[4 * k + 2], c->values[4 * k + 3]};
+        if (want.x != c->points[2 * k] || want.y != c->points[2 * k + 1] || !(evaluate_at(c, want) == value)) return false;
+        if (o.leaf != c->storage_indices[k] || !(o.value == value)) return false;
+        if (o.count != c->log_n || !verify_opening(o, pf.roots[0])) return false;  // verifyClaimOpening (:1627-1647)
+    }
+    return true;
+}
+
+} // namespace
+
+extern "C" azk_status azk_circle_polynomial_claim_make(const azk_circle_polynomial_claim *claim, uint32_t *points,
+                                                       uint32_t *values)
+{
+    azk_status s = check_claim(claim, false);
     if (s) return s;
-    *accepted = verify(pf, public_input_digest) ? 1 : 0;
+    CHECK_ARG(points && values);
+    const Domain dom = Domain::canonical_of(claim->log_n);
+    for (uint32_t k = 0; k < claim->claim_count; ++k) {
+        const Pt p = dom.storage_point(claim->storage_indices[k]);
+        const Q v = evaluate_at(claim, p);
+        points[2 * k] = p.x;
+        points[2 * k + 1] = p.y;
+        memcpy(values + 4 * k, &v, 16);
+    }
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_circle_polynomial_claim_digest(const azk_circle_polynomial_claim *claim, uint8_t digest[32])
+{
+    azk_status s = check_claim(claim, true);
+    if (s) return s;
+    CHECK_ARG(digest);
+    try {
+        claim_digest(claim, digest);
+    } catch (const std::bad_alloc &) {
+        return fail(AZK_ERR_ALLOCATION, "polynomial claim digest: out of host memory");
+    }
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_circle_verify_polynomial(const uint8_t *proof, size_t len, const azk_circle_polynomial_claim *claim,
+                                                   int *accepted)
+{
+    CHECK_ARG(proof && accepted);
+    *accepted = 0;
+    azk_status s = check_claim(claim, true);
+    if (s) return s;
+    try {
+        Proof pf;
+        s = decode(proof, len, pf);
+        if (s) return s;
+        *accepted = verify_polynomial(pf, claim) ? 1 : 0;
+    } catch (const std::bad_alloc &) {
+        return fail(AZK_ERR_ALLOCATION, "Circle PCS proof: out of host memory");
+    }
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_circle_verify_contract(const uint8_t *proof, size_t len, const azk_circle_parameter_set *params,
+                                                 const azk_circle_polynomial_claim *claim, int *accepted)
+{
+    CHECK_ARG(proof && params && accepted);
+    *accepted = 0;
+    azk_status s = check_claim(claim, true);
+    if (s) return s;
+    // CirclePCSFRIParameterSetV1.init / CircleFRISecurityParametersV1.init (:135-153,216-236)
+    const uint64_t nominal = (uint64_t)params->log_blowup * params->queries + params->grinding_bits;
+    if (!params->log_blowup || !params->queries || params->grinding_bits > 64 || nominal > 0xffffffffull ||
+        !params->target_soundness_bits || nominal < params->target_soundness_bits)
+        return fail(AZK_ERR_INVALID_LAYOUT, "Circle PCS parameter set: invalid security parameters");
+    // CirclePCSFRIStatementV1.init: validateDomain and the coefficient budget (:1664-1700)
+    if (claim->log_n <= params->log_blowup)
+        return fail(AZK_ERR_INVALID_LAYOUT, "Circle PCS statement: domain 2^%u not above the blowup 2^%u", claim->log_n,
+                    params->log_blowup);
+    const uint64_t used = (uint64_t)claim->x_count + claim->y_count;
+    if (used > ((1ull << claim->log_n) >> params->log_blowup))
+        return fail(AZK_ERR_INVALID_LAYOUT, "Circle PCS statement: %llu coefficients over the committed capacity",
+                    (unsigned long long)used);
+    try {
+        Proof pf;
+        s = decode(proof, len, pf);
+        if (s) return s;
+        // acceptsProofShape (:270-285)
+        const uint32_t rounds = claim->log_n - params->log_blowup;
+        bool ok = pf.dom.log_n == claim->log_n && pf.log_blowup == params->log_blowup && pf.queries == params->queries &&
+                  pf.folding == 1 && pf.grinding == params->grinding_bits && pf.roots.size() == rounds &&
+                  pf.final_layer.size() == (1ull << params->log_blowup);
+        for (size_t k = 1; ok && k < pf.final_layer.size(); ++k) ok = pf.final_layer[k] == pf.final_layer[0];
+        *accepted = ok && verify_polynomial(pf, claim) ? 1 : 0;
+    } catch (const std::bad_alloc &) {
+        return fail(AZK_ERR_ALLOCATION, "Circle PCS proof: out of host memory");
+    }
     return AZK_OK;
 }

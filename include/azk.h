@@ -325,6 +325,50 @@ azk_status azk_circle_verify(const uint8_t *proof, size_t len, const uint8_t *pu
  * the host Circle FFT of `coeffs` (4 x 2^logN words, the proof's domain) at its leaf. */
 azk_status azk_circle_verify_with_coefficients(const uint8_t *proof, size_t len, const uint8_t *public_input_digest,
                                                const uint32_t *coeffs, size_t coeff_words, int *accepted);
+/* CircleFRIGrindingV1.digestMeetsTarget (CircleProofFormat.swift:663-681): 1 when the first
+ * `bits` bits of the digest (most significant bit of byte 0 first) are zero; 0 when they are
+ * not or when bits exceeds 8 x len. */
+int azk_circle_grinding_digest_meets_target(const uint8_t *digest, size_t len, uint32_t bits);
+
+/* CirclePCSFRIPolynomialClaimV1 (CircleProofFormat.swift:1018-1093) as plain arrays: the
+ * canonical bit-reversed domain of 2^log_n points, a polynomial in monomial x / y
+ * coefficients (QM31, 4 LE u32 each; p(x, y) = X(x) + y Y(x), CircleCodeword.swift:53-70)
+ * and claim_count evaluation claims at strictly increasing storage indices, each with its
+ * circle point (x, y) and QM31 value. */
+typedef struct azk_circle_polynomial_claim {
+    uint32_t log_n;
+    uint32_t x_count, y_count, claim_count;
+    const uint32_t *x_coeffs;        /* x_count x 4 words */
+    const uint32_t *y_coeffs;        /* y_count x 4 words */
+    const uint64_t *storage_indices; /* claim_count */
+    const uint32_t *points;          /* claim_count x 2 words (x, y) */
+    const uint32_t *values;          /* claim_count x 4 words */
+} azk_circle_polynomial_claim;
+/* CirclePCSFRIPolynomialClaimV1.make (:1048-1073): fills claim->points and claim->values
+ * (caller-owned arrays passed as `points` / `values`) for the claim's storage indices. */
+azk_status azk_circle_polynomial_claim_make(const azk_circle_polynomial_claim *claim, uint32_t *points,
+                                            uint32_t *values);
+/* CirclePCSFRIPolynomialClaimDigestV1.digest (:1095-1151): the public-input digest a prover
+ * of this claim commits to. */
+azk_status azk_circle_polynomial_claim_digest(const azk_circle_polynomial_claim *claim, uint8_t digest[32]);
+/* CirclePCSFRIPolynomialVerifierV1.verify (:1548-1659): the proof verifies against the
+ * claim's digest, its domain is the claim's, the coefficient counts fit the blowup, and
+ * every claimed opening matches its claim (index, point, polynomial value) and opens against
+ * the first commitment. */
+azk_status azk_circle_verify_polynomial(const uint8_t *proof, size_t len, const azk_circle_polynomial_claim *claim,
+                                        int *accepted);
+/* CirclePCSFRIParameterSetV1 (:207-286): security profile of a statement. */
+typedef struct azk_circle_parameter_set {
+    uint32_t log_blowup, queries, grinding_bits, target_soundness_bits;
+} azk_circle_parameter_set;
+/* CirclePCSFRIContractVerifierV1.verify(encodedProof:statement:) (:1726-1752) with the
+ * statement CirclePCSFRIStatementV1(parameterSet:polynomialClaim:) (:1660-1700): the proof's
+ * security parameters, round count (log_n - log_blowup) and constant final layer of
+ * 2^log_blowup values match the profile, then azk_circle_verify_polynomial.  An invalid
+ * statement (domain not above the blowup, coefficients over capacity, target above the
+ * nominal security) gives AZK_ERR_INVALID_LAYOUT, as its initializer throws. */
+azk_status azk_circle_verify_contract(const uint8_t *proof, size_t len, const azk_circle_parameter_set *params,
+                                      const azk_circle_polynomial_claim *claim, int *accepted);
 
 /* ---- CPU gates (host code; what the *_verified entry points compare against)
  * CircleCodewordOracle.evaluateWithCircleFFT (CircleCodeword.swift:72-98): N = 2^log_n

@@ -823,12 +823,22 @@ static void grinding_digest(ora_transcript *t, uint32_t bits, uint64_t nonce, ui
     ora_transcript_absorb(t, nb, 8, 0x06);
     ora_transcript_squeeze_bytes(t, 32, d);
 }
-static int digest_meets(const uint8_t *d, uint32_t bits)     /* :659-681 */
+/* digestMeetsTarget: the first `bits` bits of the digest are zero, most significant bit of
+   byte 0 first; more bits than the digest holds never meet the target.       :663-681 */
+int ora_digest_meets_target(const uint8_t *d, size_t len, uint32_t bits)
 {
+    if ((uint64_t)bits > 8 * (uint64_t)len) return 0;
     uint32_t k = 0;
     while (bits >= 8) { if (d[k]) return 0; ++k; bits -= 8; }
     if (!bits) return 1;
     return (d[k] & (uint8_t)(0xffu << (8 - bits))) == 0;
+}
+static int digest_meets(const uint8_t *d, uint32_t bits) { return ora_digest_meets_target(d, 32, bits); }
+void ora_circle_grinding_digest(const uint64_t state[25], uint32_t bits, uint64_t nonce, uint8_t out[32])
+{
+    ora_transcript t;
+    memcpy(t.s, state, sizeof t.s);
+    grinding_digest(&t, bits, nonce, out);
 }
 
 static void transcript_queries(ora_transcript *t, uint32_t q, uint64_t pairs, uint32_t *out)   /* :708-717 */
@@ -919,6 +929,7 @@ int ora_circle_fri_prove(const uint32_t *evals, uint32_t log_n, const ora_securi
     const uint32_t *fin = layer[rounds];
     transcript_final_layer(&t, fin, cur);
 
+    if (info) memcpy(info->pre_grinding, t.s, sizeof t.s);
     int have_nonce = 0;
     uint64_t nonce = 0;
     if (sec->grinding_bits > 0) {                               /* :753-794 */
@@ -979,6 +990,7 @@ int ora_circle_fri_prove(const uint32_t *evals, uint32_t log_n, const ora_securi
         memcpy(info->commitments, roots, 32 * (size_t)rounds);
         memcpy(info->challenges, ch, 16 * (size_t)rounds);
         info->n_queries = sec->query_count;
+        info->grinding_nonce = nonce;
         memcpy(info->query_pairs, qp, 4 * (size_t)sec->query_count);
     }
     for (uint32_t r = 0; r <= rounds; ++r) free(layer[r]);

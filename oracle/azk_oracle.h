@@ -89,12 +89,18 @@ typedef struct {
     uint32_t challenges[32 * 4];
     uint32_t query_pairs[4096];
     uint32_t n_queries;
+    uint64_t grinding_nonce;        /* 0 without grinding */
+    uint64_t pre_grinding[25];      /* transcript state after the final layer, before the grinding frame */
 } ora_prove_info;
 
 int ora_circle_fri_prove(const uint32_t *evals, uint32_t log_n, const ora_security *sec,
                          const uint8_t digest[32], uint32_t rounds, const uint32_t *claimed, uint32_t n_claimed,
                          uint8_t *out, size_t cap, size_t *out_len, ora_prove_info *info);
 int ora_circle_fri_verify(const uint8_t *proof, size_t len, const uint8_t digest[32]);
+/* digestMeetsTarget over a digest of any length (CircleProofFormat.swift:663-681). */
+int ora_digest_meets_target(const uint8_t *digest, size_t len, uint32_t bits);
+/* Grinding digest of `nonce` from a pre-grinding transcript state (:885-896,956-963). */
+void ora_circle_grinding_digest(const uint64_t state[25], uint32_t bits, uint64_t nonce, uint8_t out[32]);
 int ora_circle_transcript_challenges(uint32_t log_n, const ora_security *sec, const uint8_t digest[32],
                                      const uint8_t *commitments, uint32_t rounds, uint32_t *out);
 

@@ -44,7 +44,8 @@ class Security(C.Structure):
 
 class ProveInfo(C.Structure):
     _fields_ = [("commitments", C.c_uint8 * (32 * 32)), ("challenges", C.c_uint32 * (32 * 4)),
-                ("query_pairs", C.c_uint32 * 4096), ("n_queries", C.c_uint32)]
+                ("query_pairs", C.c_uint32 * 4096), ("n_queries", C.c_uint32),
+                ("grinding_nonce", C.c_uint64), ("pre_grinding", C.c_uint64 * 25)]
 
 
 class Transcript(C.Structure):
@@ -88,6 +89,8 @@ def _declare(l: C.CDLL) -> None:
         "ora_circle_fri_prove": (C.c_int, [u32p, C.c_uint32, C.POINTER(Security), u8p, C.c_uint32, u32p,
                                            C.c_uint32, u8p, sz, C.POINTER(sz), C.POINTER(ProveInfo)]),
         "ora_circle_fri_verify": (C.c_int, [u8p, sz, u8p]),
+        "ora_digest_meets_target": (C.c_int, [u8p, sz, C.c_uint32]),
+        "ora_circle_grinding_digest": (None, [u64p, C.c_uint32, C.c_uint64, u8p]),
         "ora_circle_transcript_challenges": (C.c_int, [C.c_uint32, C.POINTER(Security), u8p, u8p, C.c_uint32, u32p]),
         "ora_sumcheck_chunk": (C.c_int, [u32p, sz, C.c_uint32, u32p, u32p, u32p]),
         "ora_sumcheck_multilinear": (C.c_int, [u32p, sz, u32p, u32p, u32p, u32p, u8p]),
@@ -354,6 +357,8 @@ def fri_prove(evals: np.ndarray, log_n: int, security, digest: bytes, rounds: in
         "commitments": [bytes(info.commitments[32 * r:32 * r + 32]) for r in range(rounds)],
         "challenges": [tuple(info.challenges[4 * r:4 * r + 4]) for r in range(rounds)],
         "query_pairs": list(info.query_pairs[: info.n_queries]),
+        "grinding_nonce": int(info.grinding_nonce),
+        "pre_grinding": np.array(info.pre_grinding[:], np.uint64),
     }
     return out[: n.value].tobytes(), meta
 
@@ -373,6 +378,18 @@ def transcript_challenges(log_n: int, security, digest: bytes, commitments: list
 
 
 # ---- sum-check -----------------------------------------------------------
+def digest_meets_target(digest: bytes, bits: int) -> bool:
+    d = _bytes(digest) if digest else np.zeros(1, np.uint8)
+    return bool(lib().ora_digest_meets_target(_p8(d), len(digest), bits))
+
+
+def grinding_digest(pre_grinding: np.ndarray, bits: int, nonce: int) -> bytes:
+    st = np.ascontiguousarray(pre_grinding, dtype=np.uint64)
+    out = np.zeros(32, np.uint8)
+    lib().ora_circle_grinding_digest(st.ctypes.data_as(u64p), bits, nonce, _p8(out))
+    return out.tobytes()
+
+
 def sumcheck_chunk(evals, rounds: int):
     e = np.ascontiguousarray(evals, dtype=np.uint32)
     n = e.size

@@ -362,6 +362,22 @@ def test_grinding_roundtrip():
     assert O.fri_verify(bad, digest) != 1
 
 
+@pytest.mark.parametrize("bits", [3, 7, 10])
+def test_grinding_nonce_is_the_smallest_hit(bits):
+    """findGrindingNonce (CircleProofFormat.swift:753-794) counts up from 0: every nonce below
+    the proof's misses the target and the proof's meets it.  The target test here is a
+    leading-zero count of the digest as a big-endian integer, not the oracle's predicate."""
+    digest = bytes(range(32))
+    proof, meta = O.fri_prove(_stable(32), 5, (2, 3, 1, bits), digest, 2)
+    nonce = meta["grinding_nonce"]
+    assert int.from_bytes(proof[-8:], "little") == nonce
+    for k in range(nonce + 1):
+        d = O.grinding_digest(meta["pre_grinding"], bits, k)
+        lead = 256 - int.from_bytes(d, "big").bit_length()
+        assert (lead >= bits) == (k == nonce), k
+        assert O.digest_meets_target(d, bits) == (k == nonce)
+
+
 # ---------------------------------------------------------------- sum-check
 def test_sumcheck_chunk_reference_vector():
     sc = V["sumcheck_chunk"]
