@@ -296,11 +296,9 @@ __device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t T, uint64_t
 // block.  Reference: SHA3RawLeavesMerkleCommitPlan.encodeCommitmentRoot
 // (MerkleCommitter.swift:801-894) with the treelet kernel :1167-1218.
 template <bool kAligned>
-__global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *__restrict__ tab, uint32_t ntab)
+__device__ __forceinline__ void merkle_raw_tile(const MerkleRawDesc &d, uint32_t tile)
 {
     extern __shared__ uint32_t lds[];
-    uint32_t tile;
-    const MerkleRawDesc &d = find_desc(tab, ntab, tile);
     const uint32_t T = 1u << d.log_t;
     const uint32_t stride = d.stride, len = d.len;
     uint32_t *tree = d.tree;
@@ -333,6 +331,21 @@ __global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *_
     }
     __syncthreads();
     reduce_lds(A, T / 2, 1, g0 >> 1, tree, m);
+}
+
+template <bool kAligned>
+__global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *__restrict__ tab, uint32_t ntab)
+{
+    uint32_t tile;
+    const MerkleRawDesc &d = find_desc(tab, ntab, tile);
+    merkle_raw_tile<kAligned>(d, tile);
+}
+
+// One job's tiles with the descriptor as a kernel argument (no table in device memory).
+template <bool kAligned>
+__global__ __launch_bounds__(256) void k_merkle_raw_tiles1(const MerkleRawDesc d)
+{
+    merkle_raw_tile<kAligned>(d, blockIdx.x);
 }
 
 __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops);
@@ -955,13 +968,19 @@ void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t 
     hipLaunchKernelGGL(k_inverse, dim3(blocks_for(n, 256)), dim3(256), 0, st, in, out, n);
 }
 
-void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t)
+void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t,
+                       const MerkleRawDesc *single)
 {
     if (!blocks) return;
-    if (aligned)
-        hipLaunchKernelGGL(k_merkle_raw_tiles<true>, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
-    else
-        hipLaunchKernelGGL(k_merkle_raw_tiles<false>, dim3(blocks), dim3(256), tile_lds_bytes(max_log_t), st, t, n);
+    const size_t lds = tile_lds_bytes(max_log_t);
+    if (single && n == 1) {
+        if (aligned) hipLaunchKernelGGL(k_merkle_raw_tiles1<true>, dim3(blocks), dim3(256), lds, st, *single);
+        else hipLaunchKernelGGL(k_merkle_raw_tiles1<false>, dim3(blocks), dim3(256), lds, st, *single);
+    } else if (aligned) {
+        hipLaunchKernelGGL(k_merkle_raw_tiles<true>, dim3(blocks), dim3(256), lds, st, t, n);
+    } else {
+        hipLaunchKernelGGL(k_merkle_raw_tiles<false>, dim3(blocks), dim3(256), lds, st, t, n);
+    }
 }
 
 void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)

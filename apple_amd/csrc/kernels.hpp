@@ -123,7 +123,9 @@ void launch_fft_twiddles(hipStream_t st, uint32_t *out, uint32_t log_n);
 void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n);
 
 // Table-driven launches: `blocks` = total blocks over all descriptors.
-void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t);
+// `single`: host copy of the table's only entry (n == 1), passed by value (`t` may then be null).
+void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t,
+                       const MerkleRawDesc *single = nullptr);
 void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
 void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
 // `host`: host copy of the table; a launch of at most two entries passes them by value.

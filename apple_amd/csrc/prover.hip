@@ -20,6 +20,7 @@
 //   graph B: one H2D copy of all indices -> proof gather for every job -> one D2H copy
 // The transcript prefix (header, domain, security, public-input frames, digest,
 // commitment frame 0) is constant per prover and absorbed once on the host.
+#include <functional>
 #include <memory>
 #include <thread>
 
@@ -880,10 +881,10 @@ extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers,
         if (((uintptr_t)coeffs[j]->ptr + offsets[j]) % 16) return fail(AZK_ERR_INVALID_LAYOUT, "job %u misaligned", j);
         list.push_back(Job{provers[j], (const uint8_t *)coeffs[j]->ptr + offsets[j], 0});
     }
+    HIP_TRY(hipSetDevice(ctx->device));
     auto *b = new azk_circle_batch();
     b->ctx = ctx;
     b->provers.assign(provers, provers + jobs);
-    HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> gc(ctx->mu);
     azk_status s = runner_build(b->runner, ctx, list);
     if (!s) s = ensure_graphs(b->runner, ctx->stream);
@@ -912,8 +913,12 @@ extern "C" azk_status azk_circle_batch_prove(azk_circle_batch *b, uint8_t *const
         CHECK_ARG(proofs[j]);
         if (caps[j] < 4 * b->provers[j]->proof_words) return fail(AZK_ERR_INVALID_LAYOUT, "proof capacity of job %zu", j);
     }
+    // Prover locks in one global order (by address): two batches that share provers, listed
+    // in different orders, then cannot each hold one lock while waiting for the other.
+    std::vector<azk_circle_prover *> order(b->provers);
+    std::sort(order.begin(), order.end(), std::less<azk_circle_prover *>());
     std::vector<std::unique_lock<std::mutex>> locks;
-    for (auto *p : b->provers) locks.emplace_back(p->mu);
+    for (auto *p : order) locks.emplace_back(p->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));
     std::lock_guard<std::mutex> gc(b->ctx->mu);
     azk_status s = run(b->runner, true, nullptr, stats);

@@ -243,10 +243,14 @@ azk_status azk_context_create(int device, azk_context **out)
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(AZK_ERR_NO_DEVICE, "no HIP device");
     if (device < 0 || device >= count) return fail(AZK_ERR_NO_DEVICE, "device %d of %d", device, count);
+    HIP_TRY(hipSetDevice(device));
     auto *c = new azk_context();
     c->device = device;
-    HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipGetDeviceProperties(&c->props, device));
+    const hipError_t e = hipGetDeviceProperties(&c->props, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(AZK_ERR_COMMAND_FAILED, "device %d set-up: %s", device, hipGetErrorString(e));
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(AZK_ERR_ALLOCATION, "hipStreamCreate failed");
@@ -264,7 +268,11 @@ azk_status azk_context_capabilities(azk_context *ctx, azk_capabilities *caps)
 {
     CHECK_ARG(ctx && caps);
     memset(caps, 0, sizeof *caps);
-    snprintf(caps->name, sizeof caps->name, "%s", ctx->props.name);
+    // props.name can be empty on gfx950 parts: fall back to the driver's name, then to the
+    // architecture and CU count
+    if (ctx->props.name[0]) snprintf(caps->name, sizeof caps->name, "%s", ctx->props.name);
+    else if (hipDeviceGetName(caps->name, (int)sizeof caps->name, ctx->device) != hipSuccess || !caps->name[0])
+        snprintf(caps->name, sizeof caps->name, "AMD %s (%d CUs)", ctx->props.gcnArchName, ctx->props.multiProcessorCount);
     snprintf(caps->arch, sizeof caps->arch, "%s", ctx->props.gcnArchName);
     caps->compute_units = ctx->props.multiProcessorCount;
     caps->wavefront_size = ctx->props.warpSize;
@@ -535,9 +543,31 @@ struct azk_merkle_plan {
     uint32_t log_m, log_t;
     DeviceMem tree, staging, path;
     bool committed = false;
-    OneShot shot;
     std::mutex mu;
 };
+
+// Default treelet: at least 256 blocks (one per CU) until tiles reach 2^11 leaves; 2^16 x
+// 64-byte leaves measured 148 us with 2^8-leaf treelets against 243 us with 2^11.
+uint32_t azk::rt::merkle_default_log_tile(uint32_t log_m)
+{
+    return std::min(std::min(kDefaultLogTile, std::max(8u, log_m > 8 ? log_m - 8 : 0u)), log_m);
+}
+
+// Descriptors by value: one dependent table read less per launch, and no upload.
+void azk::rt::merkle_launch(hipStream_t st, const void *leaves, uint32_t *tree, uint64_t m, uint32_t stride,
+                            uint32_t len, uint32_t log_t)
+{
+    const MerkleRawDesc raw{(const uint8_t *)leaves, tree, m, stride, len, log_t, 0};
+    const bool aligned = ((stride & 3u) == 0) && (((uintptr_t)leaves & 3u) == 0);
+    launch_merkle_raw(st, nullptr, 1, (uint32_t)(m >> log_t), aligned, log_t, &raw);
+    const uint32_t log_m = ilog2(m);
+    for (uint32_t level = log_t; level < log_m;) {  // tree_schedule, without a host vector
+        const uint32_t log_c = std::min(kUpperLogChunk, log_m - level);
+        const TreeDesc d{tree, m, nullptr, nullptr, level, log_c, 0, 0};
+        launch_tree(st, nullptr, 1, (uint32_t)((m >> level) >> log_c), log_c, 4, &d);
+        level += log_c;
+    }
+}
 
 extern "C" azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_desc *desc, azk_merkle_plan **out)
 {
@@ -548,11 +578,8 @@ extern "C" azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_
         return fail(AZK_ERR_UNSUPPORTED_ONE_BLOCK_LENGTH, "leaves are hashed as one block (<= 136 bytes), got %u",
                     desc->leaf_length);
     if (desc->leaf_stride < desc->leaf_length) return fail(AZK_ERR_INVALID_LAYOUT, "leaf stride shorter than leaf length");
-    auto *p = new azk_merkle_plan{ctx, *desc, ilog2(desc->leaf_count), 0, {}, {}, {}, false, {}, {}};
-    // default treelet: at least 256 blocks (one per CU) until tiles reach 2^11 leaves; 2^16 x 64-byte
-    // leaves measured 148 us with 2^8-leaf treelets against 243 us with 2^11
-    const uint32_t lm = ilog2(desc->leaf_count);
-    uint32_t lt = desc->subtree_log ? desc->subtree_log : std::min(kDefaultLogTile, std::max(8u, lm > 8 ? lm - 8 : 0u));
+    auto *p = new azk_merkle_plan{ctx, *desc, ilog2(desc->leaf_count), 0, {}, {}, {}, false, {}};
+    uint32_t lt = desc->subtree_log ? desc->subtree_log : merkle_default_log_tile(p->log_m);
     if (lt > kDefaultLogTile) lt = kDefaultLogTile;  // 16 x 2^11 bytes of LDS per block
     p->log_t = std::min(lt, p->log_m);
     HIP_TRY(hipSetDevice(ctx->device));
@@ -568,20 +595,8 @@ extern "C" void azk_merkle_plan_destroy(azk_merkle_plan *p) { destroy_on_ctx(p);
 
 static azk_status merkle_enqueue(azk_merkle_plan *p, const void *leaves)
 {
-    OneShot &o = p->shot;
-    o.reset();
-    const uint64_t m = p->d.leaf_count;
-    o.raw.add(MerkleRawDesc{(const uint8_t *)leaves, (uint32_t *)p->tree.p, m, p->d.leaf_stride, p->d.leaf_length, p->log_t, 0},
-              (uint32_t)(m >> p->log_t), p->log_t);
-    plan_tree(o.trees, (uint32_t *)p->tree.p, m, p->log_t);
-    o.raw.off = o.arena.put(o.raw.v);
-    for (auto &t : o.trees) t.off = o.arena.put(t.v);
-    azk_status s = o.arena.upload();
-    if (s) return s;
-    const bool aligned = ((p->d.leaf_stride & 3u) == 0) && (((uintptr_t)leaves & 3u) == 0);
-    hipStream_t st = p->ctx->stream;
-    launch_merkle_raw(st, o.arena.at<MerkleRawDesc>(o.raw.off), o.raw.n(), o.raw.blocks, aligned, o.raw.max_log);
-    for (auto &t : o.trees) launch_tree(st, o.arena.at<TreeDesc>(t.off), t.n(), t.blocks, t.max_log, 4, t.v.data());
+    merkle_launch(p->ctx->stream, leaves, (uint32_t *)p->tree.p, p->d.leaf_count, p->d.leaf_stride, p->d.leaf_length,
+                  p->log_t);
     return AZK_OK;
 }
 

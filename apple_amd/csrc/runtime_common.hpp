@@ -271,4 +271,13 @@ uint32_t plan_fold(Launch<FoldDesc> &big, Launch<FoldDesc> &leaf, const void *in
 bool plan_tree(std::vector<Launch<TreeDesc>> &steps, uint32_t *tree, uint64_t m, uint32_t level,
                uint32_t log_chunk = kUpperLogChunk, const TranscriptDesc *after = nullptr);
 
+// Default raw-leaf treelet of a 2^log_m-leaf commitment.
+uint32_t merkle_default_log_tile(uint32_t log_m);
+
+// Leaf tiles, then the upper tree, of one raw-leaf commitment into `tree` (all 2m - 1 nodes,
+// level by level).  Every launch takes its descriptor by value: nothing is uploaded,
+// allocated or waited for.
+void merkle_launch(hipStream_t st, const void *leaves, uint32_t *tree, uint64_t m, uint32_t stride, uint32_t len,
+                   uint32_t log_t);
+
 } // namespace azk::rt
