@@ -172,6 +172,7 @@ def load() -> C.CDLL:
         "azk_merkle_commit_verified": (st, [vp, vp, sz, vp, sp]),
         "azk_merkle_open": (st, [vp, u32, vp, sz, vp]),
         "azk_merkle_plan_clear": (st, [vp]),
+        "azk_merkle_commit_noalloc": (st, [vp, vp, sz, u32, u32, u32, vp, sz, vp]),
         "azk_prehashed_plan_create": (st, [vp, u32, C.POINTER(vp)]),
         "azk_prehashed_plan_destroy": (None, [vp]),
         "azk_prehashed_commit_resident": (st, [vp, vp, sz, vp, sp]),
@@ -503,6 +504,14 @@ class SHA3RawLeavesMerkleCommitPlan(_Plan):
 
     def clear(self) -> None:
         _ck(self.ctx._lib.azk_merkle_plan_clear(self.h))
+
+
+def merkle_commit_noalloc(ctx: "Context", leaves: "Buffer", leaf_count: int, leaf_stride: int, leaf_length: int,
+                          root: "Buffer", leaves_offset: int = 0, root_offset: int = 0) -> None:
+    """Allocation-free raw-leaf Merkle root into device memory, enqueued on the context stream
+    (synchronize before reading `root`)."""
+    _ck(load().azk_merkle_commit_noalloc(ctx.h, leaves.h, leaves_offset, leaf_count, leaf_stride, leaf_length, root.h,
+                                         root_offset, None))
 
 
 class SHA3PrehashedLeavesMerkleCommitPlan(_Plan):

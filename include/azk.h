@@ -144,6 +144,17 @@ azk_status azk_merkle_commit_verified(azk_merkle_plan *plan, const void *leaves,
 azk_status azk_merkle_open(azk_merkle_plan *plan, uint32_t index, uint8_t *siblings, size_t siblings_bytes,
                            uint8_t root[32]);
 azk_status azk_merkle_plan_clear(azk_merkle_plan *plan);
+/* No counterpart in the reference: an allocation-free variant of azk_merkle_commit_resident
+ * for callers that cannot allocate device memory (code under hipGraph capture, serving loops
+ * that forbid hipMalloc).  Root of leaf_count = 2^k raw leaves (leaf_count <= 65536, leaf
+ * length <= 136 bytes, stride >= length) at (leaves, leaves_offset), written as 32 bytes to
+ * (root, root_offset).  The tree lives in a 4 MiB library-owned array of each GPU, held until
+ * the process ends; nothing is allocated, copied from the host or waited for: the work is
+ * enqueued on `stream` (a hipStream_t; NULL = the context's stream).  Calls that overlap on
+ * one GPU race on that array.  Larger leaf counts give AZK_ERR_INVALID_LEAF_COUNT. */
+azk_status azk_merkle_commit_noalloc(azk_context *ctx, const azk_buffer *leaves, size_t leaves_offset,
+                                     uint32_t leaf_count, uint32_t leaf_stride, uint32_t leaf_length, azk_buffer *root,
+                                     size_t root_offset, void *stream);
 
 /* ---- Merkle root over prehashed 32-byte leaves
  * SHA3PrehashedLeavesMerkleCommitPlan(context:leafCount:) (MerkleCommitter.swift:314-524). */
