@@ -118,6 +118,15 @@ class ParameterSet(C.Structure):
                 ("target_soundness_bits", C.c_uint32)]
 
 
+class ValuCalibration(C.Structure):
+    _fields_ = [("keccak_perms_per_s", C.c_double), ("keccak_clock_mhz", C.c_double),
+                ("keccak_cycles_per_wave_perm", C.c_double), ("cycles_per_instr", C.c_double * 6),
+                ("clock_mhz", C.c_double * 6)]
+
+
+VALU_CALIBRATION_INSTRUCTIONS = ("v_xor_b32", "v_bitop3_b32", "v_alignbit_b32", "v_add_u32", "v_min_u32", "v_mad_u64_u32")
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [("fft_ns", C.c_uint64), ("commit0_ns", C.c_uint64), ("fold_ns", C.c_uint64),
                 ("tree_ns", C.c_uint64), ("transcript_ns", C.c_uint64), ("emit_ns", C.c_uint64),
@@ -244,6 +253,23 @@ def load() -> C.CDLL:
         "azk_server_take": (st, [vp, C.c_uint64, vp, sz, C.POINTER(sz)]),
         "azk_now_ns": (C.c_uint64, []),
         "azk_host_alloc": (st, [sz, C.POINTER(vp)]),
+        "azk_calibrate_valu": (st, [vp, C.POINTER(ValuCalibration)]),
+        "azk_sumcheck_chunk_plan_create": (st, [vp, C.c_uint64, u32, C.POINTER(vp)]),
+        "azk_sumcheck_chunk_plan_destroy": (None, [vp]),
+        "azk_sumcheck_chunk_coefficient_words": (sz, [C.c_uint64, u32]),
+        "azk_sumcheck_chunk_execute": (st, [vp, vp, sz, vp, vp, vp, sp]),
+        "azk_sumcheck_chunk_execute_resident": (st, [vp, vp, sz, vp, vp, vp, sp]),
+        "azk_sumcheck_chunk_execute_verified": (st, [vp, vp, sz, vp, vp, vp, sp]),
+        "azk_sumcheck_chunk_plan_clear": (st, [vp]),
+        "azk_sumcheck_chunk_proof_size": (sz, [C.c_uint64, u32]),
+        "azk_sumcheck_chunk_prove": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_sumcheck_chunk_verify": (st, [vp, sz, C.POINTER(C.c_int)]),
+        "azk_domain_plan_create": (st, [vp, u32, C.c_int, C.c_int, u32, C.POINTER(vp)]),
+        "azk_domain_plan_destroy": (None, [vp]),
+        "azk_domain_plan_stats": (st, [vp, sp]),
+        "azk_domain_inverse_layout": (st, [vp, vp, vp, u32]),
+        "azk_domain_read": (st, [vp, C.c_int, vp, sz]),
+        "azk_domain_copy_resident": (st, [vp, C.c_int, vp, sz]),
         "azk_host_free": (None, [vp]),
     }
     for name, (res, args) in sigs.items():
@@ -598,6 +624,56 @@ class CircleWitnessToFFTBasisPlanV1(_Plan):
         return _stats_dict(s)
 
 
+class CircleDomainMaterializationPlan(_Plan):
+    """CircleDomainMaterializationPlan: points, FFT twiddles and QM31 inverse-domain layers of
+    the canonical 2^log_n domain, materialized on the device at creation."""
+    POINTS, FFT_TWIDDLES, INVERSE_LAYERS = 0, 1, 2
+    _destroy = "azk_domain_plan_destroy"
+
+    def __init__(self, ctx: Context, log_n: int, materialize_domain_points: bool = True,
+                 materialize_codeword_twiddles: bool = False, inverse_domain_round_count: int = 0):
+        self.ctx, self.log_n, self.rounds = ctx, log_n, inverse_domain_round_count
+        h = vp()
+        _ck(ctx._lib.azk_domain_plan_create(ctx.h, log_n, int(materialize_domain_points),
+                                            int(materialize_codeword_twiddles), inverse_domain_round_count, C.byref(h)))
+        self.h = h
+        _track(self)
+
+    def materialization_stats(self) -> dict:
+        st = Stats()
+        _ck(self.ctx._lib.azk_domain_plan_stats(self.h, C.byref(st)))
+        return _stats_dict(st)
+
+    def inverse_domain_layout(self) -> tuple[list, list]:
+        counts = np.zeros(max(self.rounds, 1), np.uint64)
+        offsets = np.zeros(max(self.rounds, 1), np.uint64)
+        _ck(self.ctx._lib.azk_domain_inverse_layout(self.h, _ptr(counts), _ptr(offsets), self.rounds))
+        return counts[: self.rounds].tolist(), offsets[: self.rounds].tolist()
+
+    def _read(self, which: int, words: int) -> np.ndarray:
+        out = np.zeros(max(words, 1), np.uint32)
+        _ck(self.ctx._lib.azk_domain_read(self.h, which, _ptr(out), 4 * words))
+        return out[:words]
+
+    def read_domain_points(self) -> np.ndarray:
+        return self._read(self.POINTS, 2 << self.log_n).reshape(-1, 2)
+
+    def read_codeword_twiddles(self) -> np.ndarray:
+        return self._read(self.FFT_TWIDDLES, (1 << self.log_n) - 1)
+
+    def read_flat_inverse_domain(self) -> np.ndarray:
+        n = 1 << self.log_n
+        return self._read(self.INVERSE_LAYERS, 4 * (n - (n >> self.rounds))).reshape(-1, 4)
+
+    def read_inverse_domain_layers(self) -> list:
+        flat = self.read_flat_inverse_domain()
+        counts, offsets = self.inverse_domain_layout()
+        return [flat[o:o + c] for c, o in zip(counts, offsets)]
+
+    def copy_resident(self, which: int, dst: Buffer, offset: int = 0) -> None:
+        _ck(self.ctx._lib.azk_domain_copy_resident(self.h, which, dst.h, offset))
+
+
 class CircleFRIFoldChainPlan(_Plan):
     """CircleFRIFoldPlan.swift:5-280 (explicit challenges)."""
     _destroy = "azk_fold_plan_destroy"
@@ -853,6 +929,65 @@ class M31MultilinearSumcheckPlan(_Plan):
                 "initial_digest": bytes(r.initial_digest), "round_polys": polys, "challenges": ch}
 
 
+class MetalSumcheckChunkPlan(_Plan):
+    """The M31 sum-check chunk (MetalSumcheckChunkPlan): per round the whole vector is absorbed,
+    one challenge squeezed, and the vector folded v[2i] + r v[2i + 1]."""
+    _destroy = "azk_sumcheck_chunk_plan_destroy"
+
+    def __init__(self, ctx: Context, lane_count: int, rounds_per_superstep: int):
+        self.ctx, self.lane_count, self.rounds = ctx, lane_count, rounds_per_superstep
+        h = vp()
+        _ck(ctx._lib.azk_sumcheck_chunk_plan_create(ctx.h, lane_count, rounds_per_superstep, C.byref(h)))
+        self.h = h
+        _track(self)
+        self.last_stats = {}
+
+    def _outputs(self):
+        nc = self.ctx._lib.azk_sumcheck_chunk_coefficient_words(self.lane_count, self.rounds)
+        return (np.zeros(self.lane_count >> self.rounds, np.uint32), np.zeros(nc, np.uint32),
+                np.zeros(self.rounds, np.uint32))
+
+    def execute(self, evaluations, verified: bool = False) -> dict:
+        e = np.ascontiguousarray(evaluations, dtype=np.uint32).reshape(-1)
+        fin, co, ch = self._outputs()
+        s = Stats()
+        fn = (self.ctx._lib.azk_sumcheck_chunk_execute_verified if verified
+              else self.ctx._lib.azk_sumcheck_chunk_execute)
+        _ck(fn(self.h, _ptr(e), e.size, _ptr(fin), _ptr(co), _ptr(ch), C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return {"final_vector": fin, "coefficients": co, "challenges": ch}
+
+    def execute_uploaded_vector(self, buf: Buffer, offset: int = 0) -> dict:
+        fin, co, ch = self._outputs()
+        s = Stats()
+        _ck(self.ctx._lib.azk_sumcheck_chunk_execute_resident(self.h, buf.h, offset, _ptr(fin), _ptr(co), _ptr(ch),
+                                                              C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return {"final_vector": fin, "coefficients": co, "challenges": ch}
+
+    def prove(self, evaluations) -> bytes:
+        """M31SumcheckProofBuilderV1.prove, codec bytes ("AZKSCV1")."""
+        e = np.ascontiguousarray(evaluations, dtype=np.uint32).reshape(-1)
+        cap = self.ctx._lib.azk_sumcheck_chunk_proof_size(self.lane_count, self.rounds)
+        out = np.zeros(max(cap, 1), np.uint8)
+        n = C.c_size_t(0)
+        s = Stats()
+        _ck(self.ctx._lib.azk_sumcheck_chunk_prove(self.h, _ptr(e), e.size, _ptr(out), cap, C.byref(n), C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return out[: n.value].tobytes()
+
+    def clear(self) -> None:
+        _ck(self.ctx._lib.azk_sumcheck_chunk_plan_clear(self.h))
+
+
+def sumcheck_chunk_verify(proof: bytes) -> bool:
+    """M31SumcheckProofCodecV1.decode + M31SumcheckVerifierV1.verify (revealed folding trace)."""
+    b = np.frombuffer(proof, np.uint8) if len(proof) else np.zeros(1, np.uint8)
+    ok = C.c_int(0)
+    _ck(load().azk_sumcheck_chunk_verify(b.ctypes.data, len(proof), C.byref(ok)))
+    return bool(ok.value)
+
+
 def circle_verify(proof: bytes, public_input_digest: bytes) -> bool:
     """CirclePCSFRIProofCodecV1.decode + CirclePCSFRIProofVerifierV1.verify on the host."""
     lib = load()
@@ -877,6 +1012,17 @@ def circle_verify_with_coefficients(proof: bytes, public_input_digest: bytes, co
     ok = C.c_int(0)
     _ck(lib.azk_circle_verify_with_coefficients(b.ctypes.data, len(proof), d.ctypes.data, _ptr(c), c.size, C.byref(ok)))
     return bool(ok.value)
+
+
+def calibrate_valu(ctx: "Context") -> dict:
+    """VALU issue rates and the register-resident Keccak-f rate of this chip (measurement aid)."""
+    c = ValuCalibration()
+    _ck(load().azk_calibrate_valu(ctx.h, C.byref(c)))
+    return {"keccak_perms_per_s": c.keccak_perms_per_s, "keccak_clock_mhz": c.keccak_clock_mhz,
+            "keccak_cycles_per_wave_perm": c.keccak_cycles_per_wave_perm,
+            "cycles_per_wave64_instruction": {n: round(c.cycles_per_instr[k], 3)
+                                              for k, n in enumerate(VALU_CALIBRATION_INSTRUCTIONS)},
+            "clock_mhz": {n: round(c.clock_mhz[k], 1) for k, n in enumerate(VALU_CALIBRATION_INSTRUCTIONS)}}
 
 
 def grinding_digest_meets_target(digest: bytes, bits: int) -> bool:

@@ -123,4 +123,8 @@ inline void sha3_256(const uint8_t *p, size_t len, uint8_t out[32], uint8_t suff
     memcpy(out, t.s, 32);
 }
 
+// M31SumcheckEncodingV1.digestWords (SumcheckOracle.swift:1288-1299): SHA3-256 of u32 len ||
+// "AppleZKProver.M31Sumcheck.Words.V1" || u32 1 || u64 n || the words LE (sumcheck.hip).
+void sumcheck_words_digest(const uint32_t *words, uint64_t n, uint8_t out[32]);
+
 } // namespace azk::host

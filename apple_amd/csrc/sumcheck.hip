@@ -378,6 +378,8 @@ uint32_t hmul(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) % kP
 
 } // namespace
 
+void azk::host::sumcheck_words_digest(const uint32_t *words, uint64_t n, uint8_t out[32]) { words_digest(words, n, out); }
+
 struct azk_sumcheck_plan {
     azk_context *ctx;
     uint32_t log_n;

@@ -30,9 +30,32 @@ from apple_amd import azk, inputs  # noqa: E402
 METRIC = "Merkle+FRI commits/sec and GB/s (bit-exact) batched vs plain; p99 latency"
 LOG_N, ROUNDS, SECURITY = 21, 10, (2, 32, 1, 0)  # log blowup 2, 32 queries, folding 1, no grinding
 DIGEST = bytes(0xA0 + i for i in range(32))
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12    # 256 CUs x 4 SIMD16 x 2.4 GHz int32 lane-ops
-KECCAK_OPS = 4555                               # VALU ops per Keccak-f[1600] in this build (v_bitop3/v_alignbit)
 HBM_PEAK_GBS = 8000.0
+# VALU roof of the hashing kernels, in Keccak-f[1600] permutations per second.  gfx950 issues a
+# wave64 v_xor_b32 / v_bitop3_b32 in 2 SIMD cycles and v_alignbit_b32 in 4 (measured live by
+# azk_calibrate_valu).  The one-lane permutation (gfx950 assembly of kf_v) is, per round,
+# 70 v_bitop3_b32 + 62 v_xor_b32 + 58 v_alignbit_b32: 24 x (132 x 2 + 58 x 4) = 11,904 issue
+# cycles per wave of 64 permutations.  At the 2.4 GHz peak clock on 256 CUs x 4 SIMDs:
+SIMDS, PEAK_CLOCK_HZ = 256 * 4, 2.4e9
+KECCAK_ROUND_DOUBLE_RATE, KECCAK_ROUND_ALIGNBIT, KECCAK_ROUNDS = 132, 58, 24
+KECCAK_WAVE_CYCLES = KECCAK_ROUNDS * (2 * KECCAK_ROUND_DOUBLE_RATE + 4 * KECCAK_ROUND_ALIGNBIT)
+KECCAK_PEAK_GPERMS = SIMDS * PEAK_CLOCK_HZ * 64 / KECCAK_WAVE_CYCLES / 1e9   # 13.2 G permutations/s
+CALIBRATION: dict = {}
+
+
+def keccak_roofline(perms: float, seconds: float, **extra) -> dict:
+    """Permutations per second against the issue roof, and against the rate the same
+    permutation reaches on register-resident states on this chip (no memory traffic)."""
+    achieved = perms / seconds / 1e9
+    r = {"bound": "valu", "achieved": achieved, "peak": KECCAK_PEAK_GPERMS, "unit": "Gperm/s",
+         "frac": achieved / KECCAK_PEAK_GPERMS,
+         "peak_model": f"{KECCAK_WAVE_CYCLES} issue cycles per wave64 Keccak-f at 2.4 GHz on 1024 SIMDs"}
+    if CALIBRATION:
+        ceil = CALIBRATION["keccak_perms_per_s"] / 1e9
+        r["measured_ceiling"] = ceil
+        r["frac_of_measured_ceiling"] = achieved / ceil
+    r.update(extra)
+    return r
 
 
 def commit_bytes(log_n: int, rounds: int) -> int:
@@ -227,11 +250,14 @@ def standalone_form(ctx, log_n: int = 26, reps: int = 5) -> dict:
                               "unit": "GB/s", "frac": fold_bytes / t_fold / 1e9 / HBM_PEAK_GBS,
                               "algorithmic_bytes": fold_bytes}},
         "fft": {"shape": f"2^{log_n} QM31 codeword", "ms": t_fft * 1e3,
-                "algorithmic_gbps": 32 * n / t_fft / 1e9,
-                "roofline": {"bound": "valu", "achieved": fft_ops / t_fft / 1e12, "peak": VALU_PEAK_TOPS,
-                             "unit": "TOP/s", "frac": fft_ops / t_fft / 1e12 / VALU_PEAK_TOPS,
-                             "ops": fft_ops, "ops_per_butterfly": fft_ops / (log_n * (n // 2)),
-                             "stages_per_pass": fft_stage_split(log_n)}},
+                # algorithmic bytes: the codeword read once and written once (32 B per element)
+                "roofline": {"bound": "hbm", "achieved": 32 * n / t_fft / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": 32 * n / t_fft / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": 32 * n,
+                             "stages_per_pass": fft_stage_split(log_n),
+                             # VALU issue at 4 SIMD cycles per wave64 instruction (an upper bound: some
+                             # of the butterfly's adds and moves issue in 2)
+                             "valu_instructions_per_butterfly": fft_ops / (log_n * (n // 2)),
+                             "valu_issue_frac_4_cycle_bound": fft_ops / 64 * 4 / (SIMDS * PEAK_CLOCK_HZ * t_fft)}},
     }
 
 
@@ -304,9 +330,6 @@ def monomial_input_form(ctx, prover, steps: int = 20) -> dict:
             "conversion_ms": float(np.median(conv[1:])) * 1e-6, "deterministic": last == first}
 
 
-KECCAK_PERM_OPS = 4555  # VALU instructions of one one-lane Keccak-f[1600] (gfx950 assembly)
-
-
 def keccak_form(ctx, log_count: int = 22, reps: int = 10, warm: int = 30) -> dict:
     """Keccak-f[1600] on 2^22 independent states (200 B each) and SHA3-256 / Keccak-256 of 2^22
     one-block messages of 136 bytes (padding makes two permutations per message at the full
@@ -331,10 +354,9 @@ def keccak_form(ctx, log_count: int = 22, reps: int = 10, warm: int = 30) -> dic
         for _ in range(warm):
             hp.hash_resident(msgs, dout)
         t = float(np.median([hp.hash_resident(msgs, dout)["gpu_ns"] for _ in range(reps)])) * 1e-9
-        hops = 2 * KECCAK_PERM_OPS * count  # a 136-byte message is a full rate block plus a padding block
+        # a 136-byte message is a full rate block plus a padding block: two permutations
         res[name] = {"ms": t * 1e3, "messages_per_s": count / t, "gbps_in": 136 * count / t / 1e9,
-                     "roofline": {"bound": "valu", "achieved": hops / t / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                                  "frac": hops / t / 1e12 / VALU_PEAK_TOPS, "ops_per_message": 2 * KECCAK_PERM_OPS}}
+                     "roofline": keccak_roofline(2 * count, t, permutations_per_message=2)}
         # every digest of the timed launch: against the product's host gate of the same plan
         # (hash_execute_verified recomputes all 2^22 on the host), then 1024 against hashlib
         got = dout.download().reshape(count, 32)
@@ -346,11 +368,8 @@ def keccak_form(ctx, log_count: int = 22, reps: int = 10, warm: int = 30) -> dic
                 for i in range(1024))
     msgs.free()
     dout.free()
-    ops = KECCAK_PERM_OPS * count
     return {"states": count, "permute_ms": t_perm * 1e3, "perms_per_s": count / t_perm,
-            "roofline": {"bound": "valu", "achieved": ops / t_perm / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                         "frac": ops / t_perm / 1e12 / VALU_PEAK_TOPS, "ops_per_perm": KECCAK_PERM_OPS},
-            **res}
+            "roofline": keccak_roofline(count, t_perm), **res}
 
 
 def merkle_plain_form(ctx, log_count: int = 16, leaf_bytes: int = 64, openings: int = 16, reps: int = 20,
@@ -377,9 +396,7 @@ def merkle_plain_form(ctx, log_count: int = 16, leaf_bytes: int = 64, openings: 
            "commit_gpu_ms": 1e3 * float(np.median(t_gpu)), "commit_wall_ms": 1e3 * float(np.median(t_wall)),
            "commits_per_s": 1.0 / float(np.median(t_wall)), "openings_ms": 1e3 * t_open}
     perms = 2 * count - 1  # one permutation per 64-byte leaf and per parent
-    achieved = perms * KECCAK_PERM_OPS / float(np.median(t_gpu)) / 1e12
-    res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                       "frac": achieved / VALU_PEAK_TOPS, "permutations": perms}
+    res["roofline"] = keccak_roofline(perms, float(np.median(t_gpu)), permutations=perms)
     if check:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O  # checker and CPU baseline only
@@ -431,8 +448,9 @@ def main() -> int:
 
     ctx = azk.Context(0)
     caps = ctx.capabilities()
+    CALIBRATION.update(azk.calibrate_valu(ctx))
     if args.keccak_only:
-        print(json.dumps({"keccak_batch": keccak_form(ctx)}), flush=True)
+        print(json.dumps({"keccak_batch": keccak_form(ctx), "valu_calibration": CALIBRATION}), flush=True)
         return 0
     if args.basis_only:
         print(json.dumps({"witness_basis": witness_basis_form(ctx)}), flush=True)
@@ -502,12 +520,12 @@ def main() -> int:
     n = 1 << LOG_N
     perms0 = n + (n - (n >> 11))
     t0_s = phases["commit0_ms"] / 1e3
-    achieved = perms0 * KECCAK_OPS / t0_s / 1e12
-    roofline = {"bound": "valu", "kernel": "k_fft_final (fused leaf hash + treelet)", "achieved": achieved,
-                "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": achieved / VALU_PEAK_TOPS,
-                "traffic": pmc_traffic("k_fft_final"),
-                "traffic_source": "profiles/pmc_bytes_per_dispatch_2^21.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
-                "algorithmic_ops_per_launch": perms0 * KECCAK_OPS, "avg_launch_ms": t0_s * 1e3}
+    # the launch's 11 FFT stages are not counted: a lower bound on its VALU use
+    roofline = keccak_roofline(perms0, t0_s, kernel="k_fft_final (final FFT stages + leaf hashes + tile subtrees)",
+                               traffic=pmc_traffic("k_fft_final"),
+                               traffic_source="profiles/pmc_bytes_per_dispatch_2^21.txt (rocprofv3 --pmc FETCH_SIZE, "
+                                              "WRITE_SIZE)",
+                               algorithmic_permutations_per_launch=perms0, avg_launch_ms=t0_s * 1e3)
     # the same launch against HBM: 2^21 x 16 B codeword read and written, 2^19 x 16 B twiddles,
     # every tree node below the tile roots written (32 B each) -- far below the HBM roof
     hbm_bytes = 2 * n * 16 + (n >> 2) * 16 + perms0 * 32
@@ -527,6 +545,7 @@ def main() -> int:
         "host_input_commits_per_s": host_input_per_s,
         "phases_ms": phases, "bit_exact_vs_oracle": checked, "device": caps["name"] or caps["arch"],
         "roofline": roofline,
+        "valu_calibration": CALIBRATION,
     }
     if args.batched:
         line["batched"] = batched_form(ctx)

@@ -205,6 +205,31 @@ azk_status azk_witness_basis_execute_verified(azk_basis_plan *plan, const uint32
                                               const uint32_t *y, uint32_t y_count, uint32_t *out, size_t out_words,
                                               azk_stats *stats);
 
+/* ---- Circle domain materialization
+ * CircleDomainMaterializationPlan(context:domain:materializeDomainPoints:materializeCodewordTwiddles:
+ * inverseDomainRoundCount:) (CircleDomainMaterializationPlan.swift:54-133): the canonical 2^log_n
+ * domain (bit-reversed storage) materialized on the device at creation: points (N x (x, y) u32),
+ * Circle FFT twiddles (N - 1 u32, stage-major) and the FRI inverse-domain layers (layer r:
+ * N >> (r + 1) QM31 values (t, 0, 0, 0), concatenated).  At least one output; inverse_rounds <= log_n. */
+typedef struct azk_domain_plan azk_domain_plan;
+typedef enum azk_domain_output {
+    AZK_DOMAIN_POINTS = 0,          /* readDomainPoints (:150-168) */
+    AZK_DOMAIN_FFT_TWIDDLES = 1,    /* readCodewordTwiddles (:170-183) */
+    AZK_DOMAIN_INVERSE_LAYERS = 2   /* readFlatInverseDomain (:202-216) */
+} azk_domain_output;
+azk_status azk_domain_plan_create(azk_context *ctx, uint32_t log_n, int points, int fft_twiddles, uint32_t inverse_rounds,
+                                  azk_domain_plan **out);
+void azk_domain_plan_destroy(azk_domain_plan *plan);
+/* materializationStats (:41) */
+azk_status azk_domain_plan_stats(const azk_domain_plan *plan, azk_stats *stats);
+/* inverseDomainLayerCounts / inverseDomainLayerElementOffsets (:37-38), `rounds` entries each. */
+azk_status azk_domain_inverse_layout(const azk_domain_plan *plan, uint64_t *counts, uint64_t *offsets, uint32_t rounds);
+/* Host readback of one output; an output the plan does not hold gives AZK_ERR_INVALID_LAYOUT. */
+azk_status azk_domain_read(azk_domain_plan *plan, azk_domain_output which, void *host, size_t bytes);
+/* Resident use (requireDomainPointBuffer and siblings, :134-148): copies the output into a
+ * caller buffer at `offset`. */
+azk_status azk_domain_copy_resident(azk_domain_plan *plan, azk_domain_output which, azk_buffer *dst, size_t offset);
+
 /* ---- Circle FRI fold chain with explicit challenges
  * CircleFRIFoldPlan / CircleFRIFoldChainPlan (CircleFRIFoldPlan.swift:5-280). */
 typedef struct azk_fold_plan azk_fold_plan;
@@ -456,6 +481,42 @@ azk_status azk_sumcheck_prove(azk_sumcheck_plan *plan, const uint32_t *evals, si
 azk_status azk_sumcheck_prove_resident(azk_sumcheck_plan *plan, const azk_buffer *evals, size_t offset,
                                        const uint8_t *initial_digest, const uint32_t *claimed_sum,
                                        azk_sumcheck_result *result, azk_stats *stats);
+/* ---- M31 sum-check chunk (the reference's GPU sum-check)
+ * MetalSumcheckChunkPlan(context:descriptor:) with SumcheckChunkDescriptor(laneCount:roundsPerSuperstep:)
+ * (Planner/MetalSumcheckChunkPlan.swift:20-28,72-210): lane_count a power of two in 2..2^30,
+ * 1 <= rounds <= log2(lane_count).  Each round absorbs the whole active vector, squeezes one
+ * challenge and folds v[2i] + r v[2i + 1] (SumcheckOracle.m31Chunk, SumcheckOracle.swift:212-276).
+ * Outputs (SumcheckChunkOracleResult): final vector (lane_count >> rounds words), the coefficient
+ * log (every round's vector, azk_sumcheck_chunk_coefficient_words in all) and the challenges. */
+typedef struct azk_sumcheck_chunk_plan azk_sumcheck_chunk_plan;
+azk_status azk_sumcheck_chunk_plan_create(azk_context *ctx, uint64_t lane_count, uint32_t rounds,
+                                          azk_sumcheck_chunk_plan **out);
+void azk_sumcheck_chunk_plan_destroy(azk_sumcheck_chunk_plan *plan);
+size_t azk_sumcheck_chunk_coefficient_words(uint64_t lane_count, uint32_t rounds);
+/* execute(evaluations:) (:212-221): canonical host words. */
+azk_status azk_sumcheck_chunk_execute(azk_sumcheck_chunk_plan *plan, const uint32_t *evals, size_t count,
+                                      uint32_t *final_vector, uint32_t *coefficients, uint32_t *challenges,
+                                      azk_stats *stats);
+/* executeUploadedVector(buffer:offset:) (:235-243): words already in device memory (8-byte aligned). */
+azk_status azk_sumcheck_chunk_execute_resident(azk_sumcheck_chunk_plan *plan, const azk_buffer *evals, size_t offset,
+                                               uint32_t *final_vector, uint32_t *coefficients, uint32_t *challenges,
+                                               azk_stats *stats);
+/* executeVerified (:223-233): AZK_ERR_CORRECTNESS unless a host recomputation agrees. */
+azk_status azk_sumcheck_chunk_execute_verified(azk_sumcheck_chunk_plan *plan, const uint32_t *evals, size_t count,
+                                               uint32_t *final_vector, uint32_t *coefficients, uint32_t *challenges,
+                                               azk_stats *stats);
+/* clearReusableBuffers (:245-258). */
+azk_status azk_sumcheck_chunk_plan_clear(azk_sumcheck_chunk_plan *plan);
+/* M31SumcheckProofBuilderV1.prove + M31SumcheckProofCodecV1.encode (SumcheckOracle.swift:540-556,
+ * 1108-1124): the "AZKSCV1" proof bytes (0 size for an invalid shape). */
+size_t azk_sumcheck_chunk_proof_size(uint64_t lane_count, uint32_t rounds);
+azk_status azk_sumcheck_chunk_prove(azk_sumcheck_chunk_plan *plan, const uint32_t *evals, size_t count, uint8_t *proof,
+                                    size_t capacity, size_t *len, azk_stats *stats);
+/* M31SumcheckProofCodecV1.decode + M31SumcheckVerifierV1.verify(proof:statement:) against the
+ * proof's own statement, scope revealedEvaluationVectorFoldingTrace (:558-700): *accepted = 1 when
+ * both digests, every challenge and every fold agree; malformed bytes give AZK_ERR_INVALID_LAYOUT. */
+azk_status azk_sumcheck_chunk_verify(const uint8_t *proof, size_t len, int *accepted);
+
 /* M31MultilinearSumcheckProofCodecV1.decode + M31MultilinearSumcheckVerifierV1.verify
  * (:941-1064,1196-1233) against the statement the proof carries; *accepted = 1 or 0.
  * Malformed bytes give AZK_ERR_INVALID_LAYOUT (the reference's decode throws). */
@@ -502,6 +563,17 @@ uint64_t azk_now_ns(void);
 /* Pinned host memory for inputs (the buffer kind "pinned_host"). */
 azk_status azk_host_alloc(size_t bytes, void **out);
 void azk_host_free(void *ptr);
+
+/* ---- Measurement support (no reference counterpart): the roofline's VALU figures measured on
+ * this chip.  cycles_per_instr: SIMD cycles per wave64 instruction of v_xor_b32, v_bitop3_b32,
+ * v_alignbit_b32, v_add_u32, v_min_u32, v_mad_u64_u32 (8 independent chains per lane,
+ * registers only), at clock_mhz (s_memtime against the 100 MHz s_memrealtime); the Keccak
+ * figures are the one-lane Keccak-f[1600] on register-resident states (no memory traffic). */
+typedef struct azk_valu_calibration {
+    double keccak_perms_per_s, keccak_clock_mhz, keccak_cycles_per_wave_perm;
+    double cycles_per_instr[6], clock_mhz[6];
+} azk_valu_calibration;
+azk_status azk_calibrate_valu(azk_context *ctx, azk_valu_calibration *out);
 
 #ifdef __cplusplus
 }
