@@ -248,6 +248,9 @@ def load() -> C.CDLL:
         "azk_server_destroy": (None, [vp]),
         "azk_server_submit_commit": (st, [vp, C.POINTER(ProverDesc), vp, sz, C.POINTER(C.c_uint64)]),
         "azk_server_submit_sumcheck": (st, [vp, vp, sz, C.POINTER(C.c_uint64)]),
+        "azk_server_submit_sumcheck_digest": (st, [vp, vp, sz, vp, C.POINTER(C.c_uint64)]),
+        "azk_server_submit_sumcheck_chunk": (st, [vp, vp, sz, u32, C.POINTER(C.c_uint64)]),
+        "azk_sumcheck_prove_with_digest": (st, [vp, vp, sz, vp, vp, vp, sz, C.POINTER(sz), sp]),
         "azk_server_submit_fold": (st, [vp, u32, u32, vp, sz, vp, C.POINTER(C.c_uint64)]),
         "azk_server_wait": (st, [vp, C.c_uint64, C.POINTER(JobInfo)]),
         "azk_server_take": (st, [vp, C.c_uint64, vp, sz, C.POINTER(sz)]),
@@ -897,13 +900,16 @@ class M31MultilinearSumcheckPlan(_Plan):
         self._out = np.zeros(self.proof_size, np.uint8)
         self.last_stats: dict = {}
 
-    def prove(self, evaluations, claimed_sum: int | None = None) -> bytes:
-        """M31MultilinearSumcheckProofCodecV1 bytes of the proof."""
+    def prove(self, evaluations, claimed_sum: int | None = None, initial_digest: bytes | None = None) -> bytes:
+        """M31MultilinearSumcheckProofCodecV1 bytes of the proof (`initial_digest`: the
+        words' digestWords when the caller already holds it)."""
         a = np.ascontiguousarray(evaluations, dtype=np.uint32)
         cs = C.c_uint32(claimed_sum) if claimed_sum is not None else None
+        d = np.frombuffer(initial_digest, np.uint8) if initial_digest is not None else None
         n, s = C.c_size_t(0), Stats()
-        _ck(self.ctx._lib.azk_sumcheck_prove(self.h, _ptr(a), a.size, C.byref(cs) if cs is not None else None,
-                                             _ptr(self._out), self.proof_size, C.byref(n), C.byref(s)))
+        _ck(self.ctx._lib.azk_sumcheck_prove_with_digest(self.h, _ptr(a), a.size, _ptr(d) if d is not None else None,
+                                                         C.byref(cs) if cs is not None else None,
+                                                         _ptr(self._out), self.proof_size, C.byref(n), C.byref(s)))
         self.last_stats = _stats_dict(s)
         return self._out[: n.value].tobytes()
 
@@ -1192,10 +1198,21 @@ class Server:
         self._keep[t.value] = a  # read in place until the job is done
         return t.value
 
-    def submit_sumcheck(self, evals: np.ndarray) -> int:
+    def submit_sumcheck(self, evals: np.ndarray, initial_digest: bytes | None = None) -> int:
         a = np.ascontiguousarray(evals, dtype=np.uint32)
         t = C.c_uint64(0)
-        _ck(self.lib.azk_server_submit_sumcheck(self.h, a.ctypes.data, a.size, C.byref(t)))
+        if initial_digest is None:
+            _ck(self.lib.azk_server_submit_sumcheck(self.h, a.ctypes.data, a.size, C.byref(t)))
+        else:
+            d = np.frombuffer(initial_digest, np.uint8)
+            _ck(self.lib.azk_server_submit_sumcheck_digest(self.h, a.ctypes.data, a.size, d.ctypes.data, C.byref(t)))
+        self._keep[t.value] = a
+        return t.value
+
+    def submit_sumcheck_chunk(self, evals: np.ndarray, rounds: int) -> int:
+        a = np.ascontiguousarray(evals, dtype=np.uint32)
+        t = C.c_uint64(0)
+        _ck(self.lib.azk_server_submit_sumcheck_chunk(self.h, a.ctypes.data, a.size, rounds, C.byref(t)))
         self._keep[t.value] = a
         return t.value
 

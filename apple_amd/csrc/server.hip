@@ -12,8 +12,8 @@
 //    so the next iteration starts on resident inputs.  Provers (their layers, trees,
 //    twiddles and input staging) stay cached per shape and are reused.
 //  * auxiliary lanes (threads, one stream each) run sum-check jobs (the words are hashed
-//    on the host, the serial initial digest, while the device rounds run) and fold-chain
-//    jobs.  An idle lane keeps its plan and buffers for the next job of the same shape
+//    on the host, the serial initial digest, while the device rounds run, unless the caller
+//    supplied that digest), sum-check chunk jobs and fold-chain jobs.  An idle lane keeps its plan and buffers for the next job of the same shape
 //    unless another job waits for memory.
 //  * admission: a job is admitted when the bytes of live device state (cached provers,
 //    sum-check plans) plus its own need fit the budget; idle cached state is evicted
@@ -33,7 +33,7 @@ using namespace azk::prv;
 
 namespace {
 
-enum { KIND_COMMIT = 0, KIND_SUMCHECK = 1, KIND_FOLD = 2 };
+enum { KIND_COMMIT = 0, KIND_SUMCHECK = 1, KIND_FOLD = 2, KIND_CHUNK = 3 };
 
 struct SJob {
     uint64_t ticket = 0;
@@ -43,6 +43,8 @@ struct SJob {
     size_t count = 0;  // sum-check words
     uint32_t log_n = 0, rounds = 0;   // sum-check and fold shapes
     std::vector<uint32_t> challenges;  // fold: 4 words per round, copied at submit
+    bool have_digest = false;          // sum-check: initial digest supplied by the caller
+    uint8_t digest[32] = {};
     size_t need = 0;   // device bytes of the state it needs
     azk_status status = AZK_OK;
     std::string err;
@@ -60,6 +62,8 @@ struct ShapeKey {
 };
 
 size_t sumcheck_need(uint32_t log_n) { return ((size_t)4 << log_n) * 7 / 4 + 64 * 1024; }
+// sum-check chunk plan: the ping-pong vectors and the input staging
+size_t chunk_need(uint32_t log_n) { return ((size_t)8 << log_n) + 64 * 1024; }
 // fold chain plan: FFT + fold twiddles, two ping-pong layers, input and output buffers
 size_t fold_need(uint32_t log_n, uint32_t rounds)
 {
@@ -72,6 +76,7 @@ struct AuxCache {
     uint32_t kind = ~0u, log_n = 0, rounds = 0;
     size_t bytes = 0;
     azk_sumcheck_plan *sc = nullptr;
+    azk_sumcheck_chunk_plan *chunk = nullptr;
     azk_fold_plan *fold = nullptr;
     azk_buffer *in = nullptr, *out = nullptr;
     bool holds(const SJob *j) const { return kind == j->kind && log_n == j->log_n && rounds == j->rounds; }
@@ -79,6 +84,7 @@ struct AuxCache {
     void release()
     {
         if (sc) azk_sumcheck_plan_destroy(sc);
+        if (chunk) azk_sumcheck_chunk_plan_destroy(chunk);
         if (fold) azk_fold_plan_destroy(fold);
         if (in) azk_buffer_free(in);
         if (out) azk_buffer_free(out);
@@ -311,6 +317,8 @@ azk_status run_aux(azk_server *s, size_t lane, AuxCache &c, SJob *j)
         c.bytes = j->need;
         if (j->kind == KIND_SUMCHECK) {
             st = azk_sumcheck_plan_create(ctx, j->log_n, &c.sc);
+        } else if (j->kind == KIND_CHUNK) {
+            st = azk_sumcheck_chunk_plan_create(ctx, (uint64_t)1 << j->log_n, j->rounds, &c.chunk);
         } else {
             st = azk_fold_plan_create(ctx, j->log_n, j->rounds, &c.fold);
             if (!st) st = azk_buffer_alloc(ctx, (size_t)16 << j->log_n, &c.in);
@@ -326,8 +334,16 @@ azk_status run_aux(azk_server *s, size_t lane, AuxCache &c, SJob *j)
     if (j->kind == KIND_SUMCHECK) {
         j->result.resize(azk_sumcheck_proof_size(j->log_n));
         size_t len = 0;
-        st = azk_sumcheck_prove(c.sc, (const uint32_t *)j->input, j->count, nullptr, j->result.data(), j->result.size(),
-                                &len, nullptr);
+        st = azk_sumcheck_prove_with_digest(c.sc, (const uint32_t *)j->input, j->count, j->have_digest ? j->digest : nullptr,
+                                            nullptr, j->result.data(), j->result.size(), &len, nullptr);
+        j->result.resize(st ? 0 : len);
+        return st;
+    }
+    if (j->kind == KIND_CHUNK) {
+        j->result.resize(azk_sumcheck_chunk_proof_size(j->count, j->rounds));
+        size_t len = 0;
+        st = azk_sumcheck_chunk_prove(c.chunk, (const uint32_t *)j->input, j->count, j->result.data(), j->result.size(),
+                                      &len, nullptr);
         j->result.resize(st ? 0 : len);
         return st;
     }
@@ -403,6 +419,40 @@ void aux_lane(azk_server *s, size_t lane)
         std::lock_guard<std::mutex> g(s->mu);
         s->held -= b;
     }
+}
+
+} // namespace
+
+namespace {
+
+azk_status submit_aux(azk_server *s, std::unique_ptr<SJob> j, uint64_t *ticket)
+{
+    j->t_submit = now_ns();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
+        j->ticket = s->next_ticket++;
+        *ticket = j->ticket;
+        s->pend_aux.push_back(j.get());
+        s->jobs.emplace(j->ticket, std::move(j));
+    }
+    s->cv_aux.notify_one();
+    return AZK_OK;
+}
+
+std::unique_ptr<SJob> sumcheck_job(const uint32_t *evals, size_t count, const uint8_t *digest)
+{
+    auto j = std::make_unique<SJob>();
+    j->kind = KIND_SUMCHECK;
+    j->input = evals;
+    j->count = count;
+    j->log_n = ilog2(count);
+    j->need = sumcheck_need(j->log_n);
+    if (digest) {
+        j->have_digest = true;
+        memcpy(j->digest, digest, 32);
+    }
+    return j;
 }
 
 } // namespace
@@ -509,28 +559,37 @@ azk_status azk_server_submit_commit(azk_server *s, const azk_circle_prover_desc 
     return AZK_OK;
 }
 
+azk_status azk_server_submit_sumcheck_digest(azk_server *s, const uint32_t *evals, size_t count, const uint8_t digest[32],
+                                             uint64_t *ticket)
+{
+    CHECK_ARG(s && evals && digest && ticket);
+    if (count < 2 || (count & (count - 1)) || count > ((size_t)1 << 30))
+        return fail(AZK_ERR_INVALID_LAYOUT, "sum-check lane count %zu is not a power of two in [2, 2^30]", count);
+    return submit_aux(s, sumcheck_job(evals, count, digest), ticket);
+}
+
+azk_status azk_server_submit_sumcheck_chunk(azk_server *s, const uint32_t *evals, size_t count, uint32_t rounds,
+                                            uint64_t *ticket)
+{
+    CHECK_ARG(s && evals && ticket);
+    if (count < 2 || (count & (count - 1)) || count > ((size_t)1 << 30) || rounds < 1 || rounds > ilog2(count))
+        return fail(AZK_ERR_INVALID_LAYOUT, "sum-check chunk: %zu lanes (power of two, 2..2^30), %u rounds", count, rounds);
+    auto j = std::make_unique<SJob>();
+    j->kind = KIND_CHUNK;
+    j->input = evals;
+    j->count = count;
+    j->log_n = ilog2(count);
+    j->rounds = rounds;
+    j->need = chunk_need(j->log_n);
+    return submit_aux(s, std::move(j), ticket);
+}
+
 azk_status azk_server_submit_sumcheck(azk_server *s, const uint32_t *evals, size_t count, uint64_t *ticket)
 {
     CHECK_ARG(s && evals && ticket);
     if (count < 2 || (count & (count - 1)) || count > ((size_t)1 << 30))
         return fail(AZK_ERR_INVALID_LAYOUT, "sum-check lane count %zu is not a power of two in [2, 2^30]", count);
-    auto j = std::make_unique<SJob>();
-    j->kind = KIND_SUMCHECK;
-    j->input = evals;
-    j->count = count;
-    j->log_n = ilog2(count);
-    j->need = sumcheck_need(j->log_n);
-    j->t_submit = now_ns();
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
-        j->ticket = s->next_ticket++;
-        *ticket = j->ticket;
-        s->pend_aux.push_back(j.get());
-        s->jobs.emplace(j->ticket, std::move(j));
-    }
-    s->cv_aux.notify_one();
-    return AZK_OK;
+    return submit_aux(s, sumcheck_job(evals, count, nullptr), ticket);
 }
 
 azk_status azk_server_submit_fold(azk_server *s, uint32_t log_n, uint32_t rounds, const void *evals, size_t bytes,
@@ -549,17 +608,7 @@ azk_status azk_server_submit_fold(azk_server *s, uint32_t log_n, uint32_t rounds
     j->rounds = rounds;
     j->challenges.assign(challenges, challenges + 4 * rounds);
     j->need = fold_need(log_n, rounds);
-    j->t_submit = now_ns();
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
-        j->ticket = s->next_ticket++;
-        *ticket = j->ticket;
-        s->pend_aux.push_back(j.get());
-        s->jobs.emplace(j->ticket, std::move(j));
-    }
-    s->cv_aux.notify_one();
-    return AZK_OK;
+    return submit_aux(s, std::move(j), ticket);
 }
 
 azk_status azk_server_wait(azk_server *s, uint64_t ticket, azk_job_info *info)

@@ -576,6 +576,13 @@ azk_status azk_sumcheck_prove_resident(azk_sumcheck_plan *p, const azk_buffer *e
 azk_status azk_sumcheck_prove(azk_sumcheck_plan *p, const uint32_t *evals, size_t count, const uint32_t *claimed_sum,
                               uint8_t *proof, size_t capacity, size_t *proof_len, azk_stats *stats)
 {
+    return azk_sumcheck_prove_with_digest(p, evals, count, nullptr, claimed_sum, proof, capacity, proof_len, stats);
+}
+
+azk_status azk_sumcheck_prove_with_digest(azk_sumcheck_plan *p, const uint32_t *evals, size_t count,
+                                          const uint8_t *initial_digest, const uint32_t *claimed_sum, uint8_t *proof,
+                                          size_t capacity, size_t *proof_len, azk_stats *stats)
+{
     CHECK_ARG(p && evals && proof && proof_len);
     std::lock_guard<std::mutex> g(p->mu);
     const uint32_t R = p->log_n;
@@ -590,8 +597,9 @@ azk_status azk_sumcheck_prove(azk_sumcheck_plan *p, const uint32_t *evals, size_
     // this one uploads and enqueues the input reduction.
     uint8_t digest[32];
     std::thread hasher;
-    const bool threaded = R >= 14;
-    if (threaded) hasher = std::thread([&] { words_digest(evals, count, digest); });
+    const bool threaded = !initial_digest && R >= 14;
+    if (initial_digest) memcpy(digest, initial_digest, 32);
+    else if (threaded) hasher = std::thread([&] { words_digest(evals, count, digest); });
     else words_digest(evals, count, digest);
     hipError_t e = hipMemcpyAsync(p->staging.p, evals, count * 4, hipMemcpyHostToDevice, p->ctx->stream);
     std::vector<uint32_t> polys((size_t)2 * R), ch(R);

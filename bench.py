@@ -160,43 +160,44 @@ def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
 
 
 def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 300) -> dict:
-    """Poisson stream of commits (log N 12..22, 40%), multilinear sum-checks (2^16..2^22,
-    40%) and fold chains (2^16..2^22 QM31, 10 rounds, 20%) through the C ABI server;
-    rate = 0.8 x the server's saturated throughput.  Compared with the same stream run as
-    plain sequential calls (one job at a time)."""
+    """Poisson stream of Circle PCS commits (log N uniform in 12..22) and M31 multilinear
+    sum-checks (2^16..2^22 words), half each, through the C ABI server at 0.8 x its saturated
+    throughput, against the same stream run as plain calls one job at a time on a real
+    timeline (same inputs, same host arrays, provers and plans warm).  Every served result is
+    compared byte for byte with the plain call's.  Two variants: sum-check digests hashed on
+    the host inside each job, and digests supplied by the caller (device work only)."""
     from apple_amd import serving
-    shapes = ([(0.0, "commit", lg) for lg in range(12, 23)] + [(0.0, "sumcheck", lg) for lg in range(16, 23)] +
-              [(0.0, "fold", lg) for lg in range(16, 23)])
+    shapes = [(0.0, "commit", lg) for lg in range(12, 23)] + [(0.0, "sumcheck", lg) for lg in range(16, 23)]
     data = serving.Inputs(shapes, seed=seed)
-    # 12 sum-check lanes: each hashes its job's words on one host core (the box gives 16);
-    # saturated 300-job mixes measured 996-1087 jobs/s with 8 lanes, 1270-1282 with 12
+    plain = serving.PlainRunner(ctx, shapes, data)
+    # 12 sum-check lanes: each hashes its job's words on one host core (the box gives 16)
     srv = azk.Server(0, max_batch=64, aux_lanes=12)
     serving.run_server(srv, shapes + shapes, data)  # warm: provers and plans of every shape
-    sat = serving.run_server(srv, serving.make_trace(seed, sat_jobs, rate=0), data)
-    rate = 0.8 * sat["jobs_per_s"]
-    trace = serving.make_trace(seed + 1, jobs, rate=rate)
-    run = serving.run_server(srv, trace, data, collect=True)
-    # determinism: every result equals the same shape's result from the saturated mix
-    ref = {}
-    for kind, lg, info, proof in serving.run_server(srv, shapes, data, collect=True)["results"]:
-        ref[(kind, lg)] = hashlib.sha3_256(proof).digest()
-    same = all(info["status"] == 0 and hashlib.sha3_256(proof).digest() == ref[(kind, lg)]
-               for kind, lg, info, proof in run["results"])
+    out = {"workload": f"poisson mix: 50% Circle PCS commits log N U{{12..22}} (10 rounds, 32 queries), 50% multilinear "
+                       f"sum-checks 2^U{{16..22}} words; seed {seed}; rate 0.8 x saturated",
+           "jobs": jobs}
+    for variant, supply in (("host_digests", False), ("supplied_digests", True)):
+        sat_trace = serving.make_trace(seed, sat_jobs, rate=0)
+        sat = serving.run_server(srv, sat_trace, data, supply_digests=supply)
+        seq_sat = plain.run(sat_trace, supply_digests=supply)
+        rate = 0.8 * sat["jobs_per_s"]
+        trace = serving.make_trace(seed + 1, jobs, rate=rate)
+        run = serving.run_server(srv, trace, data, collect=True, supply_digests=supply)
+        seq = plain.run(trace, supply_digests=supply)
+        same = run["failed"] == 0 and all(info["status"] == 0 and proof == ref for (_, _, info, proof), (_, _, ref)
+                                          in zip(run["results"], seq["results"]))
+        out[variant] = {
+            "rate_jobs_per_s": rate, "saturated_jobs_per_s": sat["jobs_per_s"],
+            "jobs_per_s": run["jobs_per_s"], "p50_ms": run["p50_ms"], "p99_ms": run["p99_ms"],
+            "mean_commit_batch": run["mean_commit_batch"], "failed": run["failed"],
+            "results_match_plain_calls": bool(same),
+            "plain_sequential": {"saturated_jobs_per_s": seq_sat["jobs_per_s"], "jobs_per_s": seq["jobs_per_s"],
+                                 "p50_ms": seq["p50_ms"], "p99_ms": seq["p99_ms"]},
+            "speedup_saturated": sat["jobs_per_s"] / seq_sat["jobs_per_s"],
+        }
     srv.close()
-    service = serving.service_times(ctx, trace, data)
-    seq = serving.sequential_latencies(trace, service)
-    seq_sat = len(trace) / sum(service)
-    return {
-        "workload": f"poisson mix: 40% commits log N U{{12..22}} (10 rounds, 32 queries), 40% multilinear sum-checks "
-                    f"2^U{{16..22}}, 20% fold chains 2^U{{16..22}} x 10 rounds; seed {seed}",
-        "jobs": jobs, "rate_jobs_per_s": rate, "saturated_jobs_per_s": sat["jobs_per_s"],
-        "jobs_per_s": run["jobs_per_s"], "p50_ms": run["p50_ms"], "p99_ms": run["p99_ms"],
-        "mean_commit_batch": run["mean_commit_batch"], "failed": run["failed"],
-        "results_match_alone": bool(same),
-        "plain_sequential": {"saturated_jobs_per_s": seq_sat, "jobs_per_s": seq["jobs_per_s"],
-                             "p50_ms": seq["p50_ms"], "p99_ms": seq["p99_ms"]},
-        "speedup_saturated": sat["jobs_per_s"] / seq_sat,
-    }
+    plain.close()
+    return out
 
 
 # VALU instructions per QM31 butterfly in this build's tile loops (gfx950 assembly): stages run in
@@ -431,6 +432,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=2, help="commits timed on the CPU oracle (0 = skip)")
     ap.add_argument("--batched", type=int, default=1)
     ap.add_argument("--serving", type=int, default=1)
+    ap.add_argument("--serving-only", type=int, default=0, help="only the serving form")
     ap.add_argument("--standalone", type=int, default=1, help="FFT / fold plans at 2^26")
     ap.add_argument("--standalone-only", type=int, default=0,
                     help="profiling aid: only the 2^26 FFT / fold plans (one call each) and the calibration copy")
@@ -451,6 +453,9 @@ def main() -> int:
     CALIBRATION.update(azk.calibrate_valu(ctx))
     if args.keccak_only:
         print(json.dumps({"keccak_batch": keccak_form(ctx), "valu_calibration": CALIBRATION}), flush=True)
+        return 0
+    if args.serving_only:
+        print(json.dumps({"serving": serving_form(ctx)}), flush=True)
         return 0
     if args.basis_only:
         print(json.dumps({"witness_basis": witness_basis_form(ctx)}), flush=True)

@@ -476,6 +476,13 @@ size_t azk_sumcheck_proof_size(uint32_t log_n);
  * AZK_ERR_INVALID_LAYOUT. */
 azk_status azk_sumcheck_prove(azk_sumcheck_plan *plan, const uint32_t *evals, size_t count, const uint32_t *claimed_sum,
                               uint8_t *proof, size_t capacity, size_t *proof_len, azk_stats *stats);
+/* The same with the initial digest supplied by a caller that already holds digestWords of
+ * the words (SumcheckOracle.swift:1288-1299; e.g. an AIR reduction that digests its evaluation
+ * vector, ApplicationTheorem.swift:2357-2362): the serial host hash of the words is skipped.  A
+ * digest that is not the words' gives a proof the verifier refuses. */
+azk_status azk_sumcheck_prove_with_digest(azk_sumcheck_plan *plan, const uint32_t *evals, size_t count,
+                                          const uint8_t *initial_digest, const uint32_t *claimed_sum, uint8_t *proof,
+                                          size_t capacity, size_t *proof_len, azk_stats *stats);
 /* Resident form: words already in device memory.  `initial_digest` is digestWords of
  * the same words when the caller has it, else NULL (the words are read back and hashed). */
 azk_status azk_sumcheck_prove_resident(azk_sumcheck_plan *plan, const azk_buffer *evals, size_t offset,
@@ -539,7 +546,7 @@ typedef struct azk_server_desc {
 } azk_server_desc;
 typedef struct azk_job_info {
     azk_status status;
-    uint32_t kind;            /* 0 = Circle PCS commit, 1 = multilinear sum-check, 2 = fold chain */
+    uint32_t kind;            /* 0 = Circle PCS commit, 1 = multilinear sum-check, 2 = fold chain, 3 = sum-check chunk */
     uint32_t batch_jobs;      /* commits in the device program that ran the job */
     size_t result_bytes;      /* proof bytes (CirclePCSFRIProofCodecV1 / M31MultilinearSumcheckProofCodecV1) */
     uint64_t submit_ns, admit_ns, start_ns, done_ns;  /* azk_now_ns clock */
@@ -550,6 +557,12 @@ void azk_server_destroy(azk_server *server);
 azk_status azk_server_submit_commit(azk_server *server, const azk_circle_prover_desc *desc, const void *coeffs,
                                     size_t bytes, uint64_t *ticket);
 azk_status azk_server_submit_sumcheck(azk_server *server, const uint32_t *evals, size_t count, uint64_t *ticket);
+/* A multilinear sum-check whose initial digest the caller supplies (azk_sumcheck_prove_with_digest). */
+azk_status azk_server_submit_sumcheck_digest(azk_server *server, const uint32_t *evals, size_t count,
+                                             const uint8_t digest[32], uint64_t *ticket);
+/* A sum-check chunk of `rounds` rounds (azk_sumcheck_chunk_prove): the result is the AZKSCV1 proof. */
+azk_status azk_server_submit_sumcheck_chunk(azk_server *server, const uint32_t *evals, size_t count, uint32_t rounds,
+                                            uint64_t *ticket);
 /* A CircleFRIFoldChainPlan job (CircleFRIFoldPlan.swift:132-280): 2^log_n QM31 evaluations
  * folded `rounds` times with the given challenges (4 words per round, copied at submit);
  * the result is the 2^(log_n - rounds) folded QM31 values. */

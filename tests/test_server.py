@@ -13,20 +13,22 @@ def test_trace_is_seeded_and_poisson():
     gaps = np.diff([0.0] + [t for t, _, _ in a])
     assert 0.007 < gaps.mean() < 0.013          # mean inter-arrival 1 / rate
     kinds = [k for _, k, _ in a]
-    assert 120 < kinds.count("commit") < 200     # 40% commits, 40% sum-checks, 20% folds
-    assert 120 < kinds.count("sumcheck") < 200
-    assert 50 < kinds.count("fold") < 110
+    assert 160 < kinds.count("commit") < 240     # half commits, half sum-checks
+    assert kinds.count("commit") + kinds.count("sumcheck") == 400
     assert all(12 <= lg <= 22 for _, k, lg in a if k == "commit")
-    assert all(16 <= lg <= 22 for _, k, lg in a if k in ("sumcheck", "fold"))
+    assert all(16 <= lg <= 22 for _, k, lg in a if k == "sumcheck")
     assert all(t == 0.0 for t, _, _ in serving.make_trace(5, 10, rate=0))
+    b = serving.make_trace(6, 400, rate=0, commit_share=0.4, fold_share=0.2, chunk_share=0.1)
+    kb = [k for _, k, _ in b]
+    assert 50 < kb.count("fold") < 110 and 15 < kb.count("chunk") < 65
 
 
-def test_sequential_model():
-    trace = [(0.0, "commit", 12), (0.5, "commit", 12), (0.6, "sumcheck", 16)]
-    r = serving.sequential_latencies(trace, [1.0, 1.0, 0.5])
-    # starts 0, 1, 2; finishes 1, 2, 2.5
-    assert r["p50_ms"] == pytest.approx(1500.0)
-    assert r["elapsed_s"] == pytest.approx(2.5)
+def test_words_digest_matches_oracle():
+    for n in (1, 1000, 4096):
+        w = inputs.random_m31(9 + n, n)
+        ref = np.zeros(32, np.uint8)
+        O.lib().ora_sumcheck_words_digest(O._p32(np.ascontiguousarray(w, np.uint32)), w.size, O._p8(ref))
+        assert serving.words_digest(w) == ref.tobytes()
 
 
 @pytest.fixture(scope="module")
@@ -50,7 +52,40 @@ def alone(ctx, kind, lg, data):
     if kind == "fold":
         f = azk.CircleFRIFoldChainPlan(ctx, lg, serving.FOLD_ROUNDS)
         return f.execute(data.get(kind, lg), serving.fold_challenges()).tobytes()
+    if kind == "chunk":
+        return azk.MetalSumcheckChunkPlan(ctx, 1 << lg, serving.CHUNK_ROUNDS).prove(data.get(kind, lg))
     return azk.M31MultilinearSumcheckPlan(ctx, lg).prove(data.get(kind, lg))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("supply_digests", [False, True])
+def test_server_timeline_overlap_matches_plain_runs(ctx, supply_digests):
+    """Jobs arrive on a timeline while earlier ones are still on the device (sum-checks of
+    2^16..2^18 words, chunks, commits and folds); every result equals the same job run alone
+    by the plain runner, and the server ran jobs concurrently."""
+    trace = serving.make_trace(21, 40, rate=800.0, commit_logs=(12, 17), sumcheck_logs=(16, 18), fold_logs=(14, 16),
+                               commit_share=0.4, fold_share=0.15, chunk_share=0.15, chunk_logs=(12, 16))
+    assert {k for _, k, _ in trace} == {"commit", "sumcheck", "fold", "chunk"}
+    data = serving.Inputs(trace, seed=11)
+    plain = serving.PlainRunner(ctx, trace, data)
+    want = plain.run([(0.0, k, lg) for _, k, lg in trace], supply_digests=supply_digests)["results"]
+    plain.close()
+    srv = azk.Server(0, max_batch=64, aux_lanes=3)
+    print("server timeline mix, digests supplied:", supply_digests, flush=True)
+    r = serving.run_server(srv, trace, data, collect=True, supply_digests=supply_digests)
+    srv.close()
+    assert r["failed"] == 0
+    for (kind, lg, info, proof), (k2, lg2, ref) in zip(r["results"], want):
+        assert (kind, lg) == (k2, lg2)
+        assert proof == ref, (kind, lg)
+        assert info["kind"] == {"commit": 0, "sumcheck": 1, "fold": 2, "chunk": 3}[kind]
+    spans = sorted((i["start_ns"], i["done_ns"]) for _, _, i, _ in r["results"])
+    assert any(b[0] < a[1] for a, b in zip(spans, spans[1:])), "no two jobs overlapped on the device"
+    for kind, lg, _, proof in r["results"]:
+        if kind == "sumcheck":
+            assert azk.sumcheck_verify(proof)
+        elif kind == "chunk":
+            assert azk.sumcheck_chunk_verify(proof)
 
 
 @pytest.mark.gpu
