@@ -210,6 +210,7 @@ def load() -> C.CDLL:
         "azk_circle_prover_phase_times": (st, [vp, C.POINTER(PhaseTimes)]),
         "azk_circle_batch_create": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp)]),
         "azk_circle_batch_destroy": (None, [vp]),
+        "azk_circle_batch_tune": (st, [vp, C.POINTER(TuningConfig), C.POINTER(TuningResult)]),
         "azk_circle_batch_prove": (st, [vp, C.POINTER(vp), C.POINTER(sz), C.POINTER(sz), sp]),
         "azk_circle_prove_batch": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp),
                                         C.POINTER(sz), C.POINTER(sz), sp]),
@@ -873,6 +874,17 @@ class CircleBatchPlan:
         _ck(self.lib.azk_circle_batch_prove(self.h, self._op, self._caps, self._lens, C.byref(s)))
         self.last_stats = _stats_dict(s)
         return [self._outs[i][: self._lens[i]].tobytes() for i in range(len(self.provers))]
+
+    def tune(self, database_path: str | None = None, force: bool = False, warmup_runs: int = 0,
+             measured_runs: int = 0, max_candidates: int = 0) -> dict:
+        """Races one execution plan for the whole batch program (azk_circle_batch_tune)."""
+        c = TuningConfig(warmup_runs, measured_runs, max_candidates, 0,
+                         database_path.encode() if database_path else None, int(force))
+        r = TuningResult()
+        _ck(self.lib.azk_circle_batch_tune(self.h, C.byref(c), C.byref(r)))
+        out = {k: getattr(r, k) for k, _ in TuningResult._fields_ if k != "winner"}
+        out["winner"] = _plan_dict(r.winner)
+        return out
 
     def close(self) -> None:
         if self.h:

@@ -695,6 +695,25 @@ azk_status prover_create(azk_context *ctx, const azk_circle_prover_desc &d, azk_
     return AZK_OK;
 }
 
+std::vector<std::unique_lock<std::mutex>> batch_lock_provers(azk_circle_batch *b)
+{
+    // Prover locks in one global order (by address): two batches that share provers, listed
+    // in different orders, then cannot each hold one lock while waiting for the other.
+    std::vector<azk_circle_prover *> order(b->provers);
+    std::sort(order.begin(), order.end(), std::less<azk_circle_prover *>());
+    std::vector<std::unique_lock<std::mutex>> locks;
+    for (auto *p : order) locks.emplace_back(p->mu);
+    return locks;
+}
+
+azk_status batch_rebuild(azk_circle_batch *b)
+{
+    b->runner = std::make_unique<Runner>();
+    azk_status s = runner_build(*b->runner, b->ctx, b->jobs);
+    if (!s) s = ensure_graphs(*b->runner, b->ctx->stream);
+    return s;
+}
+
 } // namespace azk::prv
 
 azk_circle_prover::~azk_circle_prover() = default;
@@ -859,11 +878,6 @@ extern "C" azk_status azk_circle_prover_clear(azk_circle_prover *p)
 
 // ====================================================================== batched form
 
-struct azk_circle_batch {
-    azk_context *ctx = nullptr;
-    std::vector<azk_circle_prover *> provers;
-    Runner runner;
-};
 
 extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
                                               const size_t *offsets, uint32_t jobs, azk_circle_batch **out)
@@ -885,9 +899,9 @@ extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers,
     auto *b = new azk_circle_batch();
     b->ctx = ctx;
     b->provers.assign(provers, provers + jobs);
+    b->jobs = list;
     std::lock_guard<std::mutex> gc(ctx->mu);
-    azk_status s = runner_build(b->runner, ctx, list);
-    if (!s) s = ensure_graphs(b->runner, ctx->stream);
+    azk_status s = batch_rebuild(b);
     if (s) {
         delete b;
         return s;
@@ -915,16 +929,13 @@ extern "C" azk_status azk_circle_batch_prove(azk_circle_batch *b, uint8_t *const
     }
     // Prover locks in one global order (by address): two batches that share provers, listed
     // in different orders, then cannot each hold one lock while waiting for the other.
-    std::vector<azk_circle_prover *> order(b->provers);
-    std::sort(order.begin(), order.end(), std::less<azk_circle_prover *>());
-    std::vector<std::unique_lock<std::mutex>> locks;
-    for (auto *p : order) locks.emplace_back(p->mu);
+    std::vector<std::unique_lock<std::mutex>> locks = batch_lock_provers(b);
     HIP_TRY(hipSetDevice(b->ctx->device));
     std::lock_guard<std::mutex> gc(b->ctx->mu);
-    azk_status s = run(b->runner, true, nullptr, stats);
+    azk_status s = run(*b->runner, true, nullptr, stats);
     if (s) return s;
     for (size_t j = 0; j < jobs; ++j) {
-        memcpy(proofs[j], proof_of(b->runner, j), 4 * b->provers[j]->proof_words);
+        memcpy(proofs[j], proof_of(*b->runner, j), 4 * b->provers[j]->proof_words);
         if (lens) lens[j] = 4 * b->provers[j]->proof_words;
     }
     return AZK_OK;

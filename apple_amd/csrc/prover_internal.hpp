@@ -118,6 +118,11 @@ azk_status prove_single(azk_circle_prover *p, const void *src, int mode, uint8_t
 azk_status chain_single(azk_circle_prover *p, const void *src, int mode, uint8_t *roots, uint32_t *challenges,
                         void *final_dst, hipMemcpyKind final_kind, azk_stats *stats);
 
+// Graphs of R's program (captured on first use) and one run of it: graph A, the host step,
+// graph B (or eager launches with optional phase timing when `graph` is false).
+azk_status ensure_graphs(Runner &R, hipStream_t st);
+azk_status run(Runner &R, bool graph, azk_phase_times *phases, azk_stats *stats);
+
 // Descriptor rules of CircleFRISecurityParametersV1 / CirclePCSFRIProofV1 plus the
 // device path's limits (folding step 1, no grinding).
 azk_status validate_desc(const azk_circle_prover_desc &d);
@@ -125,5 +130,23 @@ azk_status validate_desc(const azk_circle_prover_desc &d);
 azk_status prover_create(azk_context *ctx, const azk_circle_prover_desc &d, azk_circle_prover **out);
 // Device bytes a prover of this shape holds (twiddles, layers, trees, staging).
 size_t prover_footprint(const azk_circle_prover_desc &d);
+
+} // namespace azk::prv
+
+// A batch program over several provers (the batched form, prover.hip).
+struct azk_circle_batch {
+    azk_context *ctx = nullptr;
+    std::vector<azk_circle_prover *> provers;
+    std::vector<azk::prv::Job> jobs;
+    std::unique_ptr<azk::prv::Runner> runner;
+};
+
+namespace azk::prv {
+
+// The provers' locks, taken in address order.
+std::vector<std::unique_lock<std::mutex>> batch_lock_provers(azk_circle_batch *b);
+// Rebuilds the batch's program from its jobs with each prover's current execution plan.
+// Caller holds the provers' locks and the context lock.
+azk_status batch_rebuild(azk_circle_batch *b);
 
 } // namespace azk::prv

@@ -175,14 +175,14 @@ void random_coefficients(uint64_t seed, size_t words, std::vector<uint32_t> &out
     }
 }
 
-azk_status persist(const azk_tuning_config &c, const azk_circle_prover *p, const std::vector<Measurement> &ms,
-                   const Measurement &win)
+azk_status persist(const azk_tuning_config &c, const std::string &dev, const std::string &wl,
+                   const std::vector<Measurement> &ms, const Measurement &win)
 {
     if (!c.database_path) return AZK_OK;
     std::lock_guard<std::mutex> g(g_db_mu);
     std::ofstream f(c.database_path, std::ios::app);
     if (!f) return fail(AZK_ERR_COMMAND_FAILED, "cannot open plan database %s", c.database_path);
-    const std::string dev = device_key(p->ctx), wl = workload_key(p->d), bh = build_hash();
+    const std::string bh = build_hash();
     const long long now = (long long)time(nullptr);
     for (const Measurement &m : ms) {
         if (!m.ok) continue;
@@ -196,14 +196,14 @@ azk_status persist(const azk_tuning_config &c, const azk_circle_prover *p, const
 }
 
 // latestWinner: the last winning row for this device, workload, build and protocol.
-bool lookup(const azk_tuning_config &c, const azk_circle_prover *p, ExecPlan &plan, double &median)
+bool lookup(const azk_tuning_config &c, const std::string &dev, const std::string &wl, ExecPlan &plan, double &median)
 {
     if (!c.database_path) return false;
     std::lock_guard<std::mutex> g(g_db_mu);
     std::ifstream f(c.database_path);
     if (!f) return false;
-    const std::string key = "\"device\": " + device_key(p->ctx) + ", \"workload\": " + workload_key(p->d) +
-                            ", \"build_hash\": \"" + build_hash() + "\", \"protocol_hash\": \"" + kProtocol + "\"";
+    const std::string key = "\"device\": " + dev + ", \"workload\": " + wl + ", \"build_hash\": \"" + build_hash() +
+                            "\", \"protocol_hash\": \"" + kProtocol + "\"";
     bool found = false;
     std::string line;
     while (std::getline(f, line)) {
@@ -222,6 +222,21 @@ bool lookup(const azk_tuning_config &c, const azk_circle_prover *p, ExecPlan &pl
         }
     }
     return found;
+}
+
+struct Knob { uint32_t ExecPlan::*field; std::vector<uint32_t> values; };
+const std::vector<Knob> &race_knobs()
+{
+    static const std::vector<Knob> knobs = {
+        {&ExecPlan::log_tree_chunk, {3, 4, 5, 6, 8, 10}},
+        {&ExecPlan::tree_waves, {8, 16}},
+        {&ExecPlan::log_tile_fold, {9, 10, 11}},
+        {&ExecPlan::log_tile_commit, {9, 10, 11}},
+        {&ExecPlan::log_leafwise_below, {17, 18, 19, 20, 21}},
+        {&ExecPlan::log_tile_leafwise, {6, 7, 8}},
+        {&ExecPlan::log_leafwise_grid, {0, 7, 9, 10}},
+    };
+    return knobs;
 }
 
 } // namespace
@@ -247,7 +262,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
 
     ExecPlan stored;
     double stored_median = 0;
-    if (!c.force && lookup(c, p, stored, stored_median)) {
+    if (!c.force && lookup(c, device_key(p->ctx), workload_key(p->d), stored, stored_median)) {
         p->xp = stored;
         p->runner.reset();
         p->drift_on = stored_median > 0;
@@ -274,16 +289,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
     res->default_median_gpu_ns = best.median_gpu;
 
     // coordinate descent: one knob at a time, the others at the best plan so far
-    struct Knob { uint32_t ExecPlan::*field; std::vector<uint32_t> values; };
-    const std::vector<Knob> knobs = {
-        {&ExecPlan::log_tree_chunk, {3, 4, 5, 6, 8, 10}},
-        {&ExecPlan::tree_waves, {8, 16}},
-        {&ExecPlan::log_tile_fold, {9, 10, 11}},
-        {&ExecPlan::log_tile_commit, {9, 10, 11}},
-        {&ExecPlan::log_leafwise_below, {17, 18, 19, 20, 21}},
-        {&ExecPlan::log_tile_leafwise, {6, 7, 8}},
-        {&ExecPlan::log_leafwise_grid, {0, 7, 9, 10}},
-    };
+    const std::vector<Knob> &knobs = race_knobs();
     uint32_t raced = 1;
     for (const Knob &k : knobs) {
         const ExecPlan base = best.plan;
@@ -326,7 +332,7 @@ extern "C" azk_status azk_circle_prover_tune(azk_circle_prover *p, const azk_buf
         }
         p->runner.reset();  // the cached program points at the scratch input
     }
-    s = persist(c, p, ms, best);
+    s = persist(c, device_key(p->ctx), workload_key(p->d), ms, best);
     if (s) return s;
     if (!same_plan(p->xp, best.plan)) {
         p->xp = best.plan;
@@ -355,5 +361,143 @@ extern "C" azk_status azk_circle_prover_drift(const azk_circle_prover *p, azk_dr
         out->relative_drift = std::fabs(p->drift_ema_ns - p->drift_baseline_ns) / p->drift_baseline_ns;
         out->stale = p->drift_samples >= 8 && out->relative_drift > 0.25;
     }
+    return AZK_OK;
+}
+
+// ====================================================================== batch programs
+
+namespace {
+
+// The batch's job shapes in order, as a workload key: {"stage": "circle_pcs_batch", "jobs": J,
+// "shapes": first 16 hex digits of SHA3-256 over (log N, rounds, blowup, queries) per job}.
+std::string batch_workload_key(const azk_circle_batch *b)
+{
+    std::vector<uint8_t> bytes;
+    for (const azk_circle_prover *p : b->provers)
+        for (uint32_t w : {p->d.log_n, p->d.rounds, p->d.log_blowup, p->d.query_count})
+            for (int k = 0; k < 4; ++k) bytes.push_back((uint8_t)(w >> (8 * k)));
+    uint8_t d[32];
+    host::sha3_256(bytes.data(), bytes.size(), d);
+    char hex[17];
+    for (int i = 0; i < 8; ++i) snprintf(hex + 2 * i, 3, "%02x", d[i]);
+    std::ostringstream o;
+    o << "{\"stage\": \"circle_pcs_batch\", \"jobs\": " << b->provers.size() << ", \"shapes\": \"" << hex << "\"}";
+    return o.str();
+}
+
+// Every prover of the batch on `plan`, the batch program rebuilt for it.
+azk_status batch_set_plan(azk_circle_batch *b, const ExecPlan &plan)
+{
+    for (azk_circle_prover *p : b->provers)
+        if (!same_plan(p->xp, plan)) {
+            p->xp = plan;
+            p->runner.reset();  // their own single-job programs follow the plan
+        }
+    return batch_rebuild(b);
+}
+
+azk_status batch_run(azk_circle_batch *b, std::vector<uint8_t> &proofs, azk_stats *st)
+{
+    azk_status s = run(*b->runner, true, nullptr, st);
+    if (s) return s;
+    proofs.clear();
+    for (size_t j = 0; j < b->provers.size(); ++j) {
+        const uint8_t *pr = proof_of(*b->runner, j);
+        proofs.insert(proofs.end(), pr, pr + 4 * b->provers[j]->proof_words);
+    }
+    return AZK_OK;
+}
+
+azk_status batch_measure(azk_circle_batch *b, const ExecPlan &plan, const std::vector<uint8_t> &want,
+                         const azk_tuning_config &c, Measurement &m)
+{
+    m.plan = plan;
+    azk_status s = batch_set_plan(b, plan);
+    if (s) return s;
+    std::vector<uint8_t> got;
+    azk_stats st{};
+    if ((s = batch_run(b, got, &st))) return s;
+    if (got != want) return AZK_OK;  // differential gate: every job's proof
+    for (uint32_t k = 0; k < c.warmup_runs; ++k)
+        if ((s = batch_run(b, got, &st))) return s;
+    std::vector<double> gpu, sub;
+    for (uint32_t k = 0; k < c.measured_runs; ++k) {
+        if ((s = batch_run(b, got, &st))) return s;
+        if (got != want) return AZK_OK;
+        gpu.push_back((double)st.gpu_ns);
+        sub.push_back((double)st.submit_ns);
+    }
+    m.median_gpu = quantile(gpu, 0.5);
+    m.p95_gpu = quantile(gpu, 0.95);
+    m.median_submit = quantile(sub, 0.5);
+    m.ok = true;
+    return AZK_OK;
+}
+
+} // namespace
+
+extern "C" azk_status azk_circle_batch_tune(azk_circle_batch *b, const azk_tuning_config *config, azk_tuning_result *res)
+{
+    CHECK_ARG(b && res);
+    azk_tuning_config c{};
+    if (config) c = *config;
+    if (!c.warmup_runs) c.warmup_runs = 2;
+    if (!c.measured_runs) c.measured_runs = 5;
+    if (!c.max_candidates) c.max_candidates = 28;
+    std::vector<std::unique_lock<std::mutex>> locks = batch_lock_provers(b);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    std::lock_guard<std::mutex> gc(b->ctx->mu);
+    *res = azk_tuning_result{};
+    const std::string dev = device_key(b->ctx), wl = batch_workload_key(b);
+    for (azk_circle_prover *p : b->provers) p->drift_on = false;
+
+    ExecPlan stored;
+    double stored_median = 0;
+    if (!c.force && lookup(c, dev, wl, stored, stored_median)) {
+        azk_status s = batch_set_plan(b, stored);
+        if (s) return s;
+        res->winner = from_exec_plan(stored);
+        res->median_gpu_ns = stored_median;
+        res->from_database = 1;
+        return AZK_OK;
+    }
+    const ExecPlan def;
+    std::vector<uint8_t> want;
+    azk_status s = batch_set_plan(b, def);
+    if (!s) s = batch_run(b, want, nullptr);
+    if (s) return s;
+    std::vector<Measurement> ms;
+    Measurement best;
+    if ((s = batch_measure(b, def, want, c, best))) return s;
+    if (!best.ok) return fail(AZK_ERR_CORRECTNESS, "the default batch program is not deterministic");
+    ms.push_back(best);
+    res->default_median_gpu_ns = best.median_gpu;
+    uint32_t raced = 1;
+    for (const Knob &k : race_knobs()) {
+        const ExecPlan base = best.plan;
+        for (uint32_t v : k.values) {
+            if (raced >= c.max_candidates) break;
+            ExecPlan cand = base;
+            cand.*(k.field) = v;
+            if (std::any_of(ms.begin(), ms.end(), [&](const Measurement &m) { return same_plan(m.plan, cand); }))
+                continue;
+            Measurement m;
+            if ((s = batch_measure(b, cand, want, c, m))) return s;
+            ++raced;
+            ms.push_back(m);
+            if (!m.ok) {
+                ++res->candidates_rejected;
+                continue;
+            }
+            if (better(m, best)) best = m;
+        }
+    }
+    if ((s = batch_set_plan(b, best.plan))) return s;
+    if ((s = persist(c, dev, wl, ms, best))) return s;
+    res->winner = from_exec_plan(best.plan);
+    res->median_gpu_ns = best.median_gpu;
+    res->p95_gpu_ns = best.p95_gpu;
+    res->median_submit_ns = best.median_submit;
+    res->candidates_measured = raced;
     return AZK_OK;
 }

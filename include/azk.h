@@ -448,6 +448,11 @@ azk_status azk_circle_batch_create(azk_circle_prover *const *provers, const azk_
 void azk_circle_batch_destroy(azk_circle_batch *batch);
 azk_status azk_circle_batch_prove(azk_circle_batch *batch, uint8_t *const *proofs, const size_t *capacities,
                                   size_t *proof_lens, azk_stats *stats);
+/* The autotuner on the batch program as a whole (azk_circle_prover_tune's race, gate and store,
+ * MetalProofPlanner.swift:20-117): one execution plan for every job, each candidate's proofs
+ * equal to the default plan's for every job, the batch's graph time as the objective; records
+ * keyed by the batch's job shapes.  The winner is left set on every prover of the batch. */
+azk_status azk_circle_batch_tune(azk_circle_batch *batch, const azk_tuning_config *config, azk_tuning_result *result);
 /* One-shot form: create, prove, destroy. */
 azk_status azk_circle_prove_batch(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
                                   const size_t *offsets, uint32_t jobs, uint8_t *const *proofs,
