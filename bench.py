@@ -101,6 +101,80 @@ def cpu_baseline(coeffs, sample_commits: int) -> dict:
                       f"{dt:.1f} s on one host core"}
 
 
+def host_threads() -> dict:
+    """Host cores: what the OS reports, what this process may use, and the threads the CPU
+    legs run (the GPU box gives one GPU's jobs a share of 16 cores)."""
+    aff = len(os.sched_getaffinity(0))
+    return {"nproc": os.cpu_count(), "affinity": aff, "threads": min(16, aff)}
+
+
+def cpu_baselines_by_config(seed: int = 4242) -> dict:
+    """The oracle (a single-threaded C restatement) on bounded samples of the other configs, on
+    one core and on every core of the share at once.  Reported baselines only."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O  # the oracle is the CPU baseline
+    from apple_amd import serving
+    th = host_threads()
+    n_th = th["threads"]
+    out = {"host": th}
+
+    # config 2: Keccak-f[1600] on 25 x u64 states, SHA3-256 of 136-byte messages (2 perms each)
+    count = 1 << 17
+    rng = np.random.default_rng(2)
+    states = rng.integers(0, 256, size=200 * count, dtype=np.uint8)
+    msgs = rng.integers(0, 256, size=136 * count, dtype=np.uint8)
+
+    def perm(_):
+        O.keccakf_batch(states, count=count)
+
+    def sha(_):
+        O.hash_oneblock_batch(msgs, 136, 136, count, 0x06)
+
+    res = {}
+    for name, fn, unit_per_call in (("keccakf_perms_per_s", perm, count), ("sha3_256_messages_per_s", sha, count)):
+        t0 = time.perf_counter()
+        fn(0)
+        one = unit_per_call / (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(n_th) as ex:
+            list(ex.map(fn, range(n_th)))
+        res[name] = {"one_core": one, "all_cores": n_th * unit_per_call / (time.perf_counter() - t0)}
+    out["keccak_batch"] = {"sample": f"2^17 states / 2^17 x 136-byte messages per call, one call per thread", **res}
+
+    # config 4: the batched shape (64 commits, log N 14..21, seed 4242): one job per task on every core
+    logs = [int(x) for x in np.random.default_rng(seed).integers(14, 22, size=64)]
+
+    def commit(j):
+        ln = logs[j]
+        c = inputs.circle_fft_coefficients(seed=seed + j, log_n=ln)
+        O.fri_prove(O.circle_fft(c, ln).reshape(-1, 4), ln, SECURITY, DIGEST, ROUNDS)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(n_th) as ex:
+        list(ex.map(commit, range(len(logs))))
+    dt = time.perf_counter() - t0
+    out["batched"] = {"commits_per_s_all_cores": len(logs) / dt, "seconds": dt,
+                      "sample": "the whole 64-job batch, one commit per task"}
+
+    # config 5: the serving mix run one job at a time on one core (the first 12 jobs of the trace)
+    trace = serving.make_trace(77, 12, rate=0)
+    data = {}
+    t0 = time.perf_counter()
+    for _, kind, lg in trace:
+        if kind == "commit":
+            c = data.setdefault((kind, lg), inputs.circle_fft_coefficients(seed + lg, lg))
+            O.fri_prove(O.circle_fft(c, lg).reshape(-1, 4), lg, serving.SECURITY, serving.DIGEST,
+                        serving.commit_rounds(lg))
+        else:
+            w = data.setdefault((kind, lg), inputs.random_m31(seed + 100 + lg, 1 << lg))
+            O.sumcheck_multilinear_proof(w)
+    dt = time.perf_counter() - t0
+    out["serving"] = {"jobs_per_s_one_core": len(trace) / dt, "seconds": dt,
+                      "sample": "the first 12 jobs of the seeded mix (half commits, half sum-checks), sequential"}
+    return out
+
+
 def cpu_all_cores(coeffs, threads: int | None = None) -> dict:
     """The same oracle commit on every host core at once: one independent 2^21 commit per
     thread (the oracle's C calls release the interpreter lock), a reported baseline only."""
@@ -108,7 +182,7 @@ def cpu_all_cores(coeffs, threads: int | None = None) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O  # the oracle is the CPU baseline
 
-    n = threads or min(16, len(os.sched_getaffinity(0)))
+    n = threads or host_threads()["threads"]
 
     def one(_):
         cw = O.circle_fft(coeffs, LOG_N)
@@ -118,7 +192,7 @@ def cpu_all_cores(coeffs, threads: int | None = None) -> dict:
     with ThreadPoolExecutor(n) as ex:
         list(ex.map(one, range(n)))
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "commits/s", "cores": n, "kind": "port",
+    return {"value": n / dt, "unit": "commits/s", "cores": n, "kind": "port", **host_threads(),
             "sample": f"{n} full 2^{LOG_N} commits, one per thread, oracle/liboracle.so, {dt:.1f} s"}
 
 
@@ -567,6 +641,7 @@ def main() -> int:
     if args.cpu_sample:
         line["cpu_baseline"] = cpu_baseline(coeffs, args.cpu_sample)
         line["cpu_baseline_all_cores"] = cpu_all_cores(coeffs)
+        line["cpu_baseline_other_configs"] = cpu_baselines_by_config()
     print(json.dumps(line), flush=True)
     return 0
 
