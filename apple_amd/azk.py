@@ -228,6 +228,7 @@ def load() -> C.CDLL:
         "azk_fold_execute_verified": (st, [vp, vp, sz, vp, vp, sz, sp]),
         "azk_witness_basis_execute_verified": (st, [vp, vp, u32, vp, u32, vp, sz, sp]),
         "azk_host_circle_fft": (st, [vp, u32, vp]),
+        "azk_host_circle_prove": (st, [vp, C.POINTER(ProverDesc), vp, sz, C.POINTER(sz)]),
         "azk_circle_grinding_digest_meets_target": (C.c_int, [vp, sz, u32]),
         "azk_circle_polynomial_claim_make": (st, [C.POINTER(PolynomialClaim), vp, vp]),
         "azk_circle_polynomial_claim_digest": (st, [C.POINTER(PolynomialClaim), vp]),
@@ -1112,6 +1113,18 @@ def host_circle_fft(coeffs: np.ndarray, log_n: int) -> np.ndarray:
     out = np.zeros(4 << log_n, np.uint32)
     _ck(load().azk_host_circle_fft(_ptr(c), log_n, _ptr(out)))
     return out
+
+
+def host_circle_prove(coeffs: np.ndarray, log_n: int, rounds: int, security=(2, 32, 1, 0),
+                      public_input_digest: bytes = b"") -> bytes:
+    """CircleFRIProofBuilderV1.prove on the host (the verified prover's CPU gate)."""
+    c = np.ascontiguousarray(coeffs, dtype=np.uint32).reshape(-1)
+    d = prover_desc(log_n, rounds, security, public_input_digest)
+    cap = 64 << 20
+    out = np.zeros(cap, np.uint8)
+    n = C.c_size_t(0)
+    _ck(load().azk_host_circle_prove(_ptr(c), C.byref(d), _ptr(out), cap, C.byref(n)))
+    return out[: n.value].tobytes()
 
 
 def host_fold_chain(evals: np.ndarray, log_n: int, challenges) -> np.ndarray:

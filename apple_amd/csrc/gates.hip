@@ -7,9 +7,11 @@
 #include <thread>
 #include <vector>
 
+#include "circle_frames.hpp"
 #include "host_field.hpp"
 #include "host_gates.hpp"
 #include "host_sha3.hpp"
+#include "prover_internal.hpp"
 #include "runtime_common.hpp"
 
 using namespace azk;
@@ -125,16 +127,149 @@ bool fold_chain(const uint32_t *evals, uint32_t log_n, const uint32_t *challenge
     return true;
 }
 
-void qm31_merkle_root(const uint32_t *leaves, uint64_t count, uint8_t root[32])
+namespace {
+
+// body(lo, hi) over [0, n) on up to 16 host threads, ranges of at least 4096 items.
+template <class F>
+void parallel_for(uint64_t n, F &&body)
 {
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    auto parallel = [hw](uint64_t n, auto &&body) {  // body(lo, hi) over [0, n)
-        const unsigned nt = (unsigned)std::min<uint64_t>(hw, std::max<uint64_t>(1, n / 4096));
-        std::vector<std::thread> pool;
-        for (unsigned t = 1; t < nt; ++t) pool.emplace_back([&, t] { body(n * t / nt, n * (t + 1) / nt); });
-        body(0, n / nt);
-        for (auto &th : pool) th.join();
+    const unsigned nt = (unsigned)std::min<uint64_t>(hw, std::max<uint64_t>(1, n / 4096));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back([&, t] { body(n * t / nt, n * (t + 1) / nt); });
+    body(0, n / nt);
+    for (auto &th : pool) th.join();
+}
+
+// Every node of the SHA3-256 Merkle tree over `count` 16-byte QM31 leaves, level by level
+// from the leaf digests up (level l starts at node 2 count - 2 count >> l).
+std::vector<uint8_t> qm31_merkle_tree(const uint32_t *leaves, uint64_t count)
+{
+    std::vector<uint8_t> nodes(32 * (2 * count - 1));
+    parallel_for(count, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t i = lo; i < hi; ++i) sha3_256((const uint8_t *)(leaves + 4 * i), 16, nodes.data() + 32 * i);
+    });
+    uint64_t at = 0;
+    for (uint64_t m = count; m > 1; m >>= 1) {
+        const uint8_t *lv = nodes.data() + 32 * at;
+        uint8_t *up = nodes.data() + 32 * (at + m);
+        parallel_for(m / 2, [&](uint64_t lo, uint64_t hi) {
+            for (uint64_t i = lo; i < hi; ++i) sha3_256(lv + 64 * i, 64, up + 32 * i);
+        });
+        at += m;
+    }
+    return nodes;
+}
+
+void put32(std::vector<uint8_t> &v, uint32_t x) { for (int k = 0; k < 4; ++k) v.push_back((uint8_t)(x >> (8 * k))); }
+void put64(std::vector<uint8_t> &v, uint64_t x) { for (int k = 0; k < 8; ++k) v.push_back((uint8_t)(x >> (8 * k))); }
+
+} // namespace
+
+bool circle_prove(const uint32_t *coeffs, const azk_circle_prover_desc &d, std::vector<uint8_t> &proof)
+{
+    const uint32_t log_n = d.log_n, R = d.rounds;
+    const uint64_t n = 1ull << log_n;
+    std::vector<std::vector<uint32_t>> layer(R + 1);
+    layer[0].resize(4 * n);
+    circle_fft(coeffs, log_n, layer[0].data());
+    const std::vector<Pt> pts = Domain::canonical_of(log_n).storage_points();
+    std::vector<std::vector<uint8_t>> tree(R);
+    std::vector<uint32_t> ch(4 * R);
+    Transcript t;
+    for (const auto &m : frames::prefix_messages(log_n, d.log_blowup, d.query_count, d.folding_step, d.grinding_bits,
+                                                 d.public_input_digest))
+        t.absorb(m.data(), m.size());
+    for (uint32_t r = 0; r < R; ++r) {  // commit, absorb the root, squeeze, fold (:1189-1260)
+        const uint64_t m = n >> r;
+        tree[r] = qm31_merkle_tree(layer[r].data(), m);
+        const std::vector<uint8_t> cf = frames::commitment_frame(r), xf = frames::challenge_frame(r);
+        t.absorb(cf.data(), cf.size());
+        t.absorb(tree[r].data() + 32 * (2 * m - 2), 32);
+        t.absorb(xf.data(), xf.size());
+        t.squeeze_u32(4, kM31, ch.data() + 4 * r);
+        layer[r + 1].resize(2 * m);
+        if (!fold_round(pts, r, layer[r].data(), m / 2, ch.data() + 4 * r, layer[r + 1].data())) return false;
+    }
+    const uint64_t nf = n >> R;
+    std::vector<uint8_t> f = frames::base_frame(5);  // final layer frame and bytes (:852-859)
+    put64(f, nf);
+    put64(f, 16 * nf);
+    t.absorb(f.data(), f.size());
+    t.absorb(reinterpret_cast<const uint8_t *>(layer[R].data()), 16 * nf);
+    uint64_t nonce = 0;
+    if (d.grinding_bits) {  // findGrindingNonce, counting up from 0 (:753-794)
+        f = frames::base_frame(7);
+        put32(f, d.grinding_bits);
+        put32(f, 8);
+        t.absorb(f.data(), f.size());
+        for (;; ++nonce) {
+            Transcript c = t;
+            c.absorb(reinterpret_cast<const uint8_t *>(&nonce), 8);
+            if (azk_circle_grinding_digest_meets_target(reinterpret_cast<const uint8_t *>(c.s), 32, d.grinding_bits)) break;
+        }
+        t.absorb(reinterpret_cast<const uint8_t *>(&nonce), 8);
+    }
+    f = frames::base_frame(6);  // query frame, then the pair indices (:708-717)
+    put32(f, d.query_count);
+    put64(f, n / 2);
+    t.absorb(f.data(), f.size());
+    std::vector<uint32_t> pairs(d.query_count);
+    t.squeeze_u32(d.query_count, (uint32_t)(n / 2), pairs.data());
+    // CirclePCSFRIProofCodecV1.encode (:475-535)
+    proof.clear();
+    const uint8_t magic[8] = {0x41, 0x5a, 0x4b, 0x50, 0x46, 0x56, 0x31, 0x00};
+    proof.insert(proof.end(), magic, magic + 8);
+    put32(proof, 1);
+    put32(proof, 1);
+    const std::vector<uint8_t> desc = frames::descriptor(log_n);
+    put32(proof, (uint32_t)desc.size());
+    proof.insert(proof.end(), desc.begin(), desc.end());
+    for (uint32_t w : {d.log_blowup, d.query_count, d.folding_step, d.grinding_bits}) put32(proof, w);
+    put32(proof, 32);
+    proof.insert(proof.end(), d.public_input_digest, d.public_input_digest + 32);
+    put32(proof, R);
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint64_t m = n >> r;
+        proof.insert(proof.end(), tree[r].begin() + 32 * (2 * m - 2), tree[r].begin() + 32 * (2 * m - 1));
+    }
+    put64(proof, nf);
+    for (uint32_t w : layer[R]) put32(proof, w);
+    put32(proof, d.query_count);
+    auto opening = [&](uint32_t r, uint64_t leaf) {  // leaf index, value, siblings bottom-up
+        const uint64_t m = n >> r;
+        const uint32_t depth = log_n - r;
+        put64(proof, leaf);
+        for (int k = 0; k < 4; ++k) put32(proof, layer[r][4 * leaf + k]);
+        put32(proof, depth);
+        uint64_t at = 0, idx = leaf;
+        for (uint64_t lv = m; lv > 1; lv >>= 1, idx >>= 1) {
+            const uint8_t *sib = tree[r].data() + 32 * (at + (idx ^ 1));
+            proof.insert(proof.end(), sib, sib + 32);
+            at += lv;
+        }
     };
+    for (uint32_t q = 0; q < d.query_count; ++q) {  // makeQuery (:1278-1331)
+        uint64_t pair = pairs[q];
+        put64(proof, pair);
+        put32(proof, R);
+        for (uint32_t r = 0; r < R; ++r, pair >>= 1) {
+            put32(proof, r);
+            put64(proof, pair);
+            opening(r, 2 * pair);
+            opening(r, 2 * pair + 1);
+        }
+    }
+    if (d.grinding_bits) {  // no claimed openings, then the nonce (:516-525)
+        put32(proof, 0);
+        put64(proof, nonce);
+    }
+    return true;
+}
+
+void qm31_merkle_root(const uint32_t *leaves, uint64_t count, uint8_t root[32])
+{
+    auto parallel = [](uint64_t n, auto &&body) { parallel_for(n, body); };
     std::vector<uint8_t> level(32 * count), up;
     parallel(count, [&](uint64_t lo, uint64_t hi) {
         for (uint64_t i = lo; i < hi; ++i) sha3_256((const uint8_t *)(leaves + 4 * i), 16, level.data() + 32 * i);
@@ -191,6 +326,26 @@ bool basis_consistent(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_
 } // namespace azk::host
 
 extern "C" {
+
+azk_status azk_host_circle_prove(const uint32_t *coeffs, const azk_circle_prover_desc *desc, uint8_t *proof,
+                                 size_t capacity, size_t *len)
+{
+    CHECK_ARG(coeffs && desc && proof && len);
+    azk_status s = azk::prv::validate_desc(*desc);
+    if (s) return s;
+    if (desc->log_n > 24) return rt::fail(AZK_ERR_INVALID_LAYOUT, "the host proof builder stops at 2^24");
+    if (!all_canonical(coeffs, (size_t)4 << desc->log_n)) return rt::fail(AZK_ERR_INVALID_LAYOUT, "non-canonical coefficient");
+    std::vector<uint8_t> out;
+    try {
+        if (!host::circle_prove(coeffs, *desc, out)) return rt::fail(AZK_ERR_INVALID_LAYOUT, "zero fold coordinate");
+    } catch (const std::bad_alloc &) {
+        return rt::fail(AZK_ERR_ALLOCATION, "host proof builder: out of host memory");
+    }
+    if (capacity < out.size()) return rt::fail(AZK_ERR_INVALID_LAYOUT, "proof capacity %zu < %zu", capacity, out.size());
+    memcpy(proof, out.data(), out.size());
+    *len = out.size();
+    return AZK_OK;
+}
 
 azk_status azk_host_circle_fft(const uint32_t *coeffs, uint32_t log_n, uint32_t *out)
 {

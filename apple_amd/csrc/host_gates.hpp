@@ -9,6 +9,7 @@
 
 #include <vector>
 
+#include "../../include/azk.h"
 #include "host_field.hpp"
 
 namespace azk::host {
@@ -37,5 +38,10 @@ void qm31_merkle_root(const uint32_t *leaves, uint64_t count, uint8_t root[32]);
 // below 2^log_n / p^4 < 2^-98.
 bool basis_consistent(const uint32_t *x, uint32_t nx, const uint32_t *y, uint32_t ny, uint32_t log_n,
                       const uint32_t *basis, uint32_t points = 2);
+
+// CircleFRIProofBuilderV1.prove (CircleProofFormat.swift:1172-1395) of the Circle FFT of the
+// line-basis coefficients, on the host: the proof bytes the prover with descriptor `d` must
+// emit (no claimed openings).  False on a zero fold coordinate.
+bool circle_prove(const uint32_t *coeffs, const azk_circle_prover_desc &d, std::vector<uint8_t> &proof);
 
 } // namespace azk::host

@@ -21,11 +21,13 @@
 // The transcript prefix (header, domain, security, public-input frames, digest,
 // commitment frame 0) is constant per prover and absorbed once on the host.
 #include <functional>
+#include <new>
 #include <memory>
 #include <thread>
 
 #include "circle_frames.hpp"
 
+#include "host_gates.hpp"
 #include "host_sha3.hpp"
 #include "prover_internal.hpp"
 
@@ -823,13 +825,23 @@ extern "C" azk_status azk_circle_prove_fft_coefficients(azk_circle_prover *p, co
 // proveCircleFFTCoefficientsResidentVerified (CircleCodeword.swift:1082-1113): the proof
 // must verify and every opened first-layer value must equal the host Circle FFT of the
 // coefficients (the reference rebuilds the whole proof on the CPU).
+// proveVerified (CircleCodeword.swift:1152-1177): the whole proof rebuilt on the host from the
+// coefficients must equal the device's byte for byte, and the verifier must accept it.
 static azk_status gate_proof(azk_circle_prover *p, const uint32_t *coeffs, const uint8_t *proof, size_t len)
 {
+    std::vector<uint8_t> want;
+    try {
+        if (!host::circle_prove(coeffs, p->d, want))
+            return fail(AZK_ERR_CORRECTNESS, "the CPU gate could not fold this domain");
+    } catch (const std::bad_alloc &) {
+        return fail(AZK_ERR_ALLOCATION, "the CPU gate ran out of host memory");
+    }
+    if (want.size() != len || memcmp(want.data(), proof, len))
+        return fail(AZK_ERR_CORRECTNESS, "Circle PCS proof differs from the CPU gate's rebuild");
     int ok = 0;
-    azk_status s = azk_circle_verify_with_coefficients(proof, len, p->d.public_input_digest, coeffs,
-                                                       (size_t)4 << p->d.log_n, &ok);
+    azk_status s = azk_circle_verify(proof, len, p->d.public_input_digest, &ok);
     if (s) return s;
-    if (!ok) return fail(AZK_ERR_CORRECTNESS, "Circle PCS proof rejected by the CPU gate");
+    if (!ok) return fail(AZK_ERR_CORRECTNESS, "Circle PCS proof rejected by the CPU verifier");
     return AZK_OK;
 }
 

@@ -410,6 +410,11 @@ azk_status azk_circle_verify_contract(const uint8_t *proof, size_t len, const az
  * CircleCodewordOracle.evaluateWithCircleFFT (CircleCodeword.swift:72-98): N = 2^log_n
  * interleaved line-basis coefficients -> N evaluations, canonical domain, storage order. */
 azk_status azk_host_circle_fft(const uint32_t *coeffs, uint32_t log_n, uint32_t *evals);
+/* CircleFRIProofBuilderV1.prove (CircleProofFormat.swift:1172-1395) of the Circle FFT of the
+ * coefficients, on the host (log N <= 24): the proof the *_verified prover entry points compare
+ * the device's with, byte for byte (proveVerified, CircleCodeword.swift:1152-1177). */
+azk_status azk_host_circle_prove(const uint32_t *coeffs, const azk_circle_prover_desc *desc, uint8_t *proof,
+                                 size_t capacity, size_t *len);
 /* CircleFRILayerOracleV1.fold (CircleDomain.swift:513-544): rounds x 4 challenge words. */
 azk_status azk_host_fold_chain(const uint32_t *evals, uint32_t log_n, const uint32_t *challenges, uint32_t rounds,
                                uint32_t *out);

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import pyoracle as O
-from apple_amd import azk
+from apple_amd import azk, inputs
 
 P = O.P
 HERE = os.path.dirname(__file__)
@@ -163,3 +163,26 @@ def test_verified_prover_at_scale(ctx):
     prover = azk.CircleCodewordPCSFRIProverV1(ctx, 21, 10, (2, 32, 1, 0), DIGEST)
     proof = prover.prove_fft_coefficients_resident(ctx.upload(c), verified=True)
     assert azk.circle_verify(proof, DIGEST)
+
+
+# ------------------------------------------------------------------ host proof builder
+@pytest.mark.parametrize("log_n,rounds,sec", [(4, 1, (2, 3, 1, 0)), (5, 3, (2, 5, 1, 0)), (10, 7, (2, 32, 1, 0)),
+                                              (12, 10, (2, 32, 1, 0)), (9, 5, (2, 6, 1, 7))])
+def test_host_proof_builder_matches_oracle(log_n, rounds, sec):
+    """The verified prover's gate: the host rebuild of the whole proof equals the oracle's
+    (whose proof builder is pinned by the reference's proof digests and corpus)."""
+    digest = bytes(0x40 + i for i in range(32))
+    coeffs = inputs.circle_fft_coefficients(seed=30 + log_n, log_n=log_n)
+    want = O.fri_prove(O.circle_fft(coeffs, log_n).reshape(-1, 4), log_n, sec, digest, rounds)[0]
+    assert azk.host_circle_prove(coeffs, log_n, rounds, sec, digest) == want
+
+
+def test_host_proof_builder_grinding_and_bounds():
+    """Grinding nonce search on the host (8 bits) equals the oracle's; rounds above log N are
+    refused as an error, not clamped."""
+    digest = bytes(0x50 + i for i in range(32))
+    c = inputs.circle_fft_coefficients(seed=5, log_n=6)
+    assert azk.host_circle_prove(c, 6, 3, (2, 5, 1, 8), digest) == \
+        O.fri_prove(O.circle_fft(c, 6).reshape(-1, 4), 6, (2, 5, 1, 8), digest, 3)[0]
+    with pytest.raises(azk.AZKError):
+        azk.host_circle_prove(c[:32], 5, 6, (2, 5, 1, 0), digest)  # rounds > log N
