@@ -21,6 +21,32 @@
 
 namespace azk {
 
+// HBM through global (address space 1) pointers, never flat ones: a flat access also
+// counts against lgkmcnt, so every later LDS wait of the wave (one per tree level, one
+// per FFT stage pair) would wait for the store to reach memory as well.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef __attribute__((address_space(1))) u32x2 gu32x2;
+typedef __attribute__((address_space(1))) u32x4 gu32x4;
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *glob(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *glob(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+__device__ __forceinline__ uint4 ld_g4(const uint4 *p)
+{
+    const u32x4 v = *(const gu32x4 *)glob(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_g4(uint4 *p, uint4 v) { *(gu32x4 *)glob(p) = u32x4{v.x, v.y, v.z, v.w}; }
+
+
 // ============================================================== batch hashing
 
 // One Keccak-f[1600] per lane over strided 25 x u64 states; output padding zeroed.
@@ -74,7 +100,7 @@ template <bool kAligned>
 __device__ __forceinline__ void load_message(const uint8_t *__restrict__ p, uint32_t len, uint32_t (&w)[34])
 {
     if constexpr (kAligned) {
-        const uint32_t *q = reinterpret_cast<const uint32_t *>(p);
+        const auto *q = glob(reinterpret_cast<const uint32_t *>(p));
 #pragma unroll
         for (int k = 0; k < 34; ++k) {
             uint32_t lo = 4u * k;
@@ -89,7 +115,7 @@ __device__ __forceinline__ void load_message(const uint8_t *__restrict__ p, uint
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 uint32_t idx = 4u * k + b;
-                if (idx < len) v |= (uint32_t)p[idx] << (8 * b);
+                if (idx < len) v |= (uint32_t)glob(p)[idx] << (8 * b);
             }
             w[k] = v;
         }
@@ -120,6 +146,11 @@ __global__ __launch_bounds__(256) void k_oneblock_hash(const uint8_t *__restrict
 }
 
 
+// Kernels that run the pinned-register Keccak rounds (100 VGPRs of state, rho/pi output and
+// temporaries) are held to 128 VGPRs: four waves per SIMD.  Left to itself the compiler
+// takes 131-136 for the fold and raw-leaf Merkle kernels (three waves per SIMD).
+#define AZK_FOUR_WAVES __attribute__((amdgpu_waves_per_eu(4, 8)))
+
 // ============================================================== descriptor lookup
 
 // Finds the descriptor whose block range holds blockIdx.x (tables are sorted by
@@ -141,11 +172,12 @@ __device__ __forceinline__ const D &find_desc(const D *__restrict__ t, uint32_t 
 
 // Node `idx` of level `l` in a full-tree array of a tree with `m` leaves:
 // levels are stored bottom-up, level l starting at node 2m - (2m >> l).
-__device__ __forceinline__ uint32_t *node_ptr(uint32_t *tree, uint64_t m, int l, uint64_t idx)
+__device__ __forceinline__ gu32 *node_ptr(uint32_t *tree, uint64_t m, int l, uint64_t idx)
 {
-    return tree + 8 * ((2 * m - ((2 * m) >> l)) + idx);
+    return glob(tree) + 8 * ((2 * m - ((2 * m) >> l)) + idx);
 }
 
+// A node in LDS (st_node / ld_node on a uint32_t *) or in HBM (on a gu32 *).
 __device__ __forceinline__ void st_node(uint32_t *p, const uint32_t (&d)[8])
 {
     reinterpret_cast<uint4 *>(p)[0] = make_uint4(d[0], d[1], d[2], d[3]);
@@ -154,6 +186,16 @@ __device__ __forceinline__ void st_node(uint32_t *p, const uint32_t (&d)[8])
 __device__ __forceinline__ void ld_node(const uint32_t *p, uint32_t (&d)[8])
 {
     uint4 x = reinterpret_cast<const uint4 *>(p)[0], y = reinterpret_cast<const uint4 *>(p)[1];
+    d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
+}
+__device__ __forceinline__ void st_node(gu32 *p, const uint32_t (&d)[8])
+{
+    reinterpret_cast<gu32x4 *>(p)[0] = u32x4{d[0], d[1], d[2], d[3]};
+    reinterpret_cast<gu32x4 *>(p)[1] = u32x4{d[4], d[5], d[6], d[7]};
+}
+__device__ __forceinline__ void ld_node(const gu32 *p, uint32_t (&d)[8])
+{
+    const u32x4 x = reinterpret_cast<const gu32x4 *>(p)[0], y = reinterpret_cast<const gu32x4 *>(p)[1];
     d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
 }
 
@@ -191,8 +233,8 @@ __device__ __forceinline__ void wave_parents(uint32_t *A, uint32_t half, uint32_
         if (c.own && c.word < 4) {
             left[2 * c.word] = lo;
             left[2 * c.word + 1] = hi;
-            uint32_t *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
-            reinterpret_cast<uint2 *>(node)[c.word] = make_uint2(lo, hi);
+            gu32 *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
+            reinterpret_cast<gu32x2 *>(node)[c.word] = u32x2{lo, hi};
         }
     }
 }
@@ -214,7 +256,7 @@ __device__ __forceinline__ void pair_parent(uint32_t *A, uint32_t q, uint32_t s,
     a[8] = hi ? 0u : 0x06u;           // SHA3 padding after 64 bytes
     a[16] = hi ? 0x80000000u : 0u;    // final bit of the 136-byte block
     kf_h(a, hi);
-    uint32_t *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
+    gu32 *node = node_ptr(tree, m, l + 1, (g >> 1) + q);
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
         left[2 * w + hi] = a[w];
@@ -263,9 +305,10 @@ __device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t cnt, int l, uin
 
 // Hashes the T QM31 leaves held in LDS (leaf i at words 4i..4i+3, global leaf index
 // g0 + i) and builds the tile's subtree up to its root, writing every node.  Leaf pair
-// p's 32 bytes are overwritten in place by its parent.  The first leaf digest is stored
-// before the second leaf is hashed: with both permutations in flight the compiler needs
-// 155 VGPRs for the kernel instead of 114 (four waves per SIMD).
+// p's 32 bytes are overwritten in place by its parent.  Nothing but the loop index is live
+// across a permutation: the second leaf is read after the first is hashed, and the first
+// digest waits in the pair's LDS slot (the permutation's own state and temporaries take
+// 100 VGPRs, and 128 is the budget for four waves per SIMD).
 __device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t T, uint64_t g0, uint32_t *tree, uint64_t m)
 {
     if (T == 1) {
@@ -277,13 +320,16 @@ __device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t T, uint64_t
         return;
     }
     for (uint32_t p = threadIdx.x; p < T / 2; p += blockDim.x) {
-        uint4 e0 = reinterpret_cast<const uint4 *>(A)[2 * p], e1 = reinterpret_cast<const uint4 *>(A)[2 * p + 1];
-        uint32_t v0[4] = {e0.x, e0.y, e0.z, e0.w}, v1[4] = {e1.x, e1.y, e1.z, e1.w};
-        uint32_t h0[8], h1[8], par[8];
+        const uint4 e0 = reinterpret_cast<const uint4 *>(A)[2 * p];
+        uint32_t v0[4] = {e0.x, e0.y, e0.z, e0.w}, h0[8], h1[8], par[8];
         sha3_leaf16(v0, h0);
         st_node(node_ptr(tree, m, 0, g0 + 2 * p), h0);
+        const uint4 e1 = reinterpret_cast<const uint4 *>(A)[2 * p + 1];
+        uint32_t v1[4] = {e1.x, e1.y, e1.z, e1.w};
+        st_node(A + 8 * p, h0);  // over both leaves of the pair; e1 is in registers
         sha3_leaf16(v1, h1);
         st_node(node_ptr(tree, m, 0, g0 + 2 * p + 1), h1);
+        ld_node(A + 8 * p, h0);
         sha3_parent(h0, h1, par);
         st_node(node_ptr(tree, m, 1, (g0 >> 1) + p), par);
         st_node(A + 8 * p, par);
@@ -314,27 +360,32 @@ __device__ __forceinline__ void merkle_raw_tile(const MerkleRawDesc &d, uint32_t
         }
         return;
     }
+    // The first leaf's digest waits in the pair's LDS slot (which the parent overwrites)
+    // while the second leaf is hashed, so no digest is live across a permutation.
     for (uint32_t p = threadIdx.x; p < T / 2; p += blockDim.x) {
-        uint32_t h[2][8];
 #pragma unroll 1
         for (int s = 0; s < 2; ++s) {
-            uint32_t w[34];
+            uint32_t w[34], h[8];
             load_message<kAligned>(d.leaves + (g0 + 2 * p + s) * (uint64_t)stride, len, w);
-            sponge_oneblock(w, len, 0x06u, h[s]);
+            sponge_oneblock(w, len, 0x06u, h);
+            st_node(node_ptr(tree, m, 0, g0 + 2 * p + s), h);
+            if (s == 0) {
+                st_node(A + 8 * p, h);
+            } else {
+                uint32_t h0[8], par[8];
+                ld_node(A + 8 * p, h0);
+                sha3_parent(h0, h, par);
+                st_node(node_ptr(tree, m, 1, (g0 >> 1) + p), par);
+                st_node(A + 8 * p, par);
+            }
         }
-        uint32_t par[8];
-        st_node(node_ptr(tree, m, 0, g0 + 2 * p), h[0]);
-        st_node(node_ptr(tree, m, 0, g0 + 2 * p + 1), h[1]);
-        sha3_parent(h[0], h[1], par);
-        st_node(node_ptr(tree, m, 1, (g0 >> 1) + p), par);
-        st_node(A + 8 * p, par);
     }
     __syncthreads();
     reduce_lds(A, T / 2, 1, g0 >> 1, tree, m);
 }
 
 template <bool kAligned>
-__global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *__restrict__ tab, uint32_t ntab)
+__global__ AZK_FOUR_WAVES __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *__restrict__ tab, uint32_t ntab)
 {
     uint32_t tile;
     const MerkleRawDesc &d = find_desc(tab, ntab, tile);
@@ -343,7 +394,7 @@ __global__ __launch_bounds__(256) void k_merkle_raw_tiles(const MerkleRawDesc *_
 
 // One job's tiles with the descriptor as a kernel argument (no table in device memory).
 template <bool kAligned>
-__global__ __launch_bounds__(256) void k_merkle_raw_tiles1(const MerkleRawDesc d)
+__global__ AZK_FOUR_WAVES __launch_bounds__(256) void k_merkle_raw_tiles1(const MerkleRawDesc d)
 {
     merkle_raw_tile<kAligned>(d, blockIdx.x);
 }
@@ -376,7 +427,7 @@ __device__ __forceinline__ void tree_reduce(const TreeDesc &d, uint32_t chunk)
     }
 }
 
-__global__ __launch_bounds__(1024) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
+__global__ AZK_FOUR_WAVES __launch_bounds__(1024) void k_tree_reduce(const TreeDesc *__restrict__ tab, uint32_t ntab)
 {
     uint32_t chunk;
     const TreeDesc &d = find_desc(tab, ntab, chunk);
@@ -385,7 +436,7 @@ __global__ __launch_bounds__(1024) void k_tree_reduce(const TreeDesc *__restrict
 
 // A launch of one tree step takes its descriptor as a kernel argument: the node loads
 // then wait on no table read (one dependent HBM round trip less per launch).
-__global__ __launch_bounds__(1024) void k_tree_reduce1(const TreeDesc d)
+__global__ AZK_FOUR_WAVES __launch_bounds__(1024) void k_tree_reduce1(const TreeDesc d)
 {
     tree_reduce(d, blockIdx.x);
 }
@@ -454,7 +505,6 @@ __device__ __forceinline__ void butterfly(uint4 *s, uint32_t i1, uint32_t i2, ui
     s[i2] = st_qm(qm_sub(l, r));
 }
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4 global_u32x4;
 typedef __attribute__((address_space(1))) const uint32_t global_u32;
 
@@ -555,7 +605,7 @@ __device__ __forceinline__ void load_tile(uint4 *s, const uint4 *__restrict__ sr
         for (uint32_t j = 0; j < 8; ++j)
             if (j < J) s[threadIdx.x + B * j] = make_uint4(v[j].x, v[j].y, v[j].z, v[j].w);
     } else if (threadIdx.x < T) {
-        s[threadIdx.x] = src[index(threadIdx.x)];
+        s[threadIdx.x] = ld_g4(src + index(threadIdx.x));
     }
 }
 
@@ -581,7 +631,7 @@ __global__ __launch_bounds__(1024) void k_fft_pass(const FftPassDesc *__restrict
     __syncthreads();
     fft_tile_stages(s, tws, k, m);
     uint4 *dst = d.dst;
-    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) dst[at(L)] = s[L];
+    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) st_g4(dst + at(L), s[L]);
 }
 
 // Final pass: stages log_t - 1 .. 0 on a contiguous tile of T = 2^log_t elements,
@@ -590,6 +640,9 @@ __global__ __launch_bounds__(1024) void k_fft_pass(const FftPassDesc *__restrict
 // and commits the hashes (SHA3PrehashedLeavesMerkleCommitPlan).  With run_fft == 0
 // the tile is only hashed (a codeword supplied by the caller).  The twiddles sit in
 // the tree's ping-pong region until the subtree is built.
+// kTree: some entry of the launch hashes its tile.  The form without the hashing code
+// needs a third of the registers (a codeword alone: twice the waves per SIMD).
+template <bool kTree>
 __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restrict__ tab, uint32_t ntab)
 {
     extern __shared__ uint32_t lds[];
@@ -607,11 +660,13 @@ __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restric
     if (d.run_fft) {
         fft_tile_stages(s, lds + 4 * T, log_t, 0);
         uint4 *dst = d.dst;
-        for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) dst[base + L] = s[L];
+        for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) st_g4(dst + base + L, s[L]);
         // every wave has read its elements before the subtree overwrites leaf pairs in place
         __syncthreads();
     }
-    if (tree) tile_tree_qm31(lds, T, base, tree, n);
+    if constexpr (kTree) {
+        if (tree) tile_tree_qm31(lds, T, base, tree, n);
+    }
 }
 
 // ============================================================== FRI fold
@@ -642,9 +697,9 @@ __device__ __forceinline__ void fold_tile(const FoldDesc &d, uint32_t tile)
     uint4 *s = reinterpret_cast<uint4 *>(lds);
     for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) {
         uint64_t g = base + L;
-        qm31 o = fri_fold_pair(ld_qm(in[2 * g]), ld_qm(in[2 * g + 1]), tw[g], c);
+        qm31 o = fri_fold_pair(ld_qm(ld_g4(in + 2 * g)), ld_qm(ld_g4(in + 2 * g + 1)), glob(tw)[g], c);
         uint4 ov = st_qm(o);
-        out[g] = ov;
+        st_g4(out + g, ov);
         if (tree) s[L] = ov;
     }
     if (!tree) return;
@@ -652,7 +707,7 @@ __device__ __forceinline__ void fold_tile(const FoldDesc &d, uint32_t tile)
     tile_tree_qm31(lds, T, base, tree, d.n_out);
 }
 
-__global__ __launch_bounds__(kFoldThreads) void k_fold(const FoldDesc *__restrict__ tab, uint32_t ntab)
+__global__ AZK_FOUR_WAVES __launch_bounds__(kFoldThreads) void k_fold(const FoldDesc *__restrict__ tab, uint32_t ntab)
 {
     uint32_t tile;
     const FoldDesc &d = find_desc(tab, ntab, tile);
@@ -668,7 +723,7 @@ __device__ __forceinline__ FoldDesc pick(const FoldPair &p, uint32_t &local)
     return d;
 }
 
-__global__ __launch_bounds__(kFoldThreads) void k_fold_pair(const FoldPair p)
+__global__ AZK_FOUR_WAVES __launch_bounds__(kFoldThreads) void k_fold_pair(const FoldPair p)
 {
     uint32_t tile;
     const FoldDesc d = pick(p, tile);
@@ -714,8 +769,8 @@ __device__ __forceinline__ void fold_leafwise_tile(const FoldDesc &d, uint32_t t
         const bool hi = tid & 1u;
         if (L < T) {
             const uint64_t g = base + L;
-            const qm31 o = fri_fold_pair(ld_qm(d.in[2 * g]), ld_qm(d.in[2 * g + 1]), d.tw[g], c);
-            if (!hi) d.out[g] = st_qm(o);
+            const qm31 o = fri_fold_pair(ld_qm(ld_g4(d.in + 2 * g)), ld_qm(ld_g4(d.in + 2 * g + 1)), glob(d.tw)[g], c);
+            if (!hi) st_g4(d.out + g, st_qm(o));
             uint32_t a[25];
 #pragma unroll
             for (int w = 0; w < 25; ++w) a[w] = 0;
@@ -724,7 +779,7 @@ __device__ __forceinline__ void fold_leafwise_tile(const FoldDesc &d, uint32_t t
             a[2] = hi ? 0u : 0x06u;
             a[16] = hi ? 0x80000000u : 0u;
             kf_h(a, hi);
-            uint32_t *node = node_ptr(tree, n_out, 0, g);
+            gu32 *node = node_ptr(tree, n_out, 0, g);
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
                 node[2 * w + hi] = a[w];
@@ -734,8 +789,8 @@ __device__ __forceinline__ void fold_leafwise_tile(const FoldDesc &d, uint32_t t
     } else if (tid < T) {
         const uint32_t L = tid;
         const uint64_t g = base + L;
-        qm31 o = fri_fold_pair(ld_qm(d.in[2 * g]), ld_qm(d.in[2 * g + 1]), d.tw[g], c);
-        d.out[g] = st_qm(o);
+        qm31 o = fri_fold_pair(ld_qm(ld_g4(d.in + 2 * g)), ld_qm(ld_g4(d.in + 2 * g + 1)), glob(d.tw)[g], c);
+        st_g4(d.out + g, st_qm(o));
         if (tree) {
             uint32_t v[4] = {o.a, o.b, o.c, o.d}, h[8];
             sha3_leaf16(v, h);
@@ -775,7 +830,7 @@ __device__ __forceinline__ void t_absorb(uint32_t &lo, uint32_t &hi, const WaveL
     uint64_t off = 0;
     while (off + 136 <= len) {
         if (in_rate) {
-            const uint64_t w = src[off / 8 + c.word];
+            const uint64_t w = glob(src)[off / 8 + c.word];
             lo ^= (uint32_t)w;
             hi ^= (uint32_t)(w >> 32);
         }
@@ -787,7 +842,7 @@ __device__ __forceinline__ void t_absorb(uint32_t &lo, uint32_t &hi, const WaveL
         const uint32_t b0 = 8u * c.word;
         uint64_t w = 0;
         if (b0 < tc) {
-            w = src[off / 8 + c.word];
+            w = glob(src)[off / 8 + c.word];
             if (tc - b0 < 8) w &= (1ull << (8 * (tc - b0))) - 1;
         }
         if (b0 <= tc && tc < b0 + 8) w ^= (uint64_t)suffix << (8 * (tc - b0));
@@ -812,7 +867,7 @@ __device__ __forceinline__ void t_squeeze(uint32_t lo, uint32_t hi, const WaveLa
         const uint64_t w = (wi & 1) ? wh : wl;
         ++cand;
         if (w < limit) {
-            if (threadIdx.x == 0) dst[got] = (uint32_t)(w % modulus);
+            if (threadIdx.x == 0) glob(dst)[got] = (uint32_t)(w % modulus);
             ++got;
         }
     }
@@ -827,7 +882,7 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
     const bool holds = c.word < 25;
     uint32_t lo = 0, hi = 0;
     if (holds) {
-        const uint64_t v = state[c.word];
+        const uint64_t v = glob(state)[c.word];
         lo = (uint32_t)v;
         hi = (uint32_t)(v >> 32);
     }
@@ -837,7 +892,7 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
             lo = hi = 0;
         } else if (op.kind == TOP_LOAD_STATE) {
             if (holds) {
-                const uint64_t v = reinterpret_cast<const uint64_t *>(op.src)[c.word];
+                const uint64_t v = glob(reinterpret_cast<const uint64_t *>(op.src))[c.word];
                 lo = (uint32_t)v;
                 hi = (uint32_t)(v >> 32);
             }
@@ -847,7 +902,7 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
             t_squeeze(lo, hi, c, (uint32_t)op.n, op.modulus, reinterpret_cast<uint32_t *>(op.dst));
         }
     }
-    if (c.own) state[c.word] = (uint64_t)lo | ((uint64_t)hi << 32);
+    if (c.own) glob(state)[c.word] = (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 __global__ __launch_bounds__(64) void k_transcript(const TranscriptDesc *__restrict__ tab, uint32_t ntab)
@@ -991,10 +1046,11 @@ void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t 
     hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(threads), ((size_t)16 << max_log_t) + 4096, st, t, n);
 }
 
-void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
+void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool tree)
 {
     if (!blocks) return;
-    hipLaunchKernelGGL(k_fft_final, dim3(blocks), dim3(256), fft_tile_lds_bytes(max_log_t), st, t, n);
+    if (tree) hipLaunchKernelGGL(k_fft_final<true>, dim3(blocks), dim3(256), fft_tile_lds_bytes(max_log_t), st, t, n);
+    else hipLaunchKernelGGL(k_fft_final<false>, dim3(blocks), dim3(256), fft_tile_lds_bytes(max_log_t), st, t, n);
 }
 
 static bool fold_pair(const FoldDesc *host, uint32_t n, FoldPair &p)

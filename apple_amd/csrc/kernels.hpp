@@ -127,7 +127,8 @@ void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t 
 void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t,
                        const MerkleRawDesc *single = nullptr);
 void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
-void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
+// tree: some entry hashes its tile (FftFinalDesc::tree set).
+void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool tree);
 // `host`: host copy of the table; a launch of at most two entries passes them by value.
 void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t,
                  const FoldDesc *host = nullptr);

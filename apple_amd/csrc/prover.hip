@@ -326,7 +326,8 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
     clk.mark(PH_START);
     for (auto &l : R.fft_pass) launch_fft_pass(st, tab(R, l), l.n(), l.blocks, l.max_log);
     clk.mark(PH_FFT);
-    launch_fft_final(st, tab(R, R.fft_final), R.fft_final.n(), R.fft_final.blocks, R.fft_final.max_log);
+    launch_fft_final(st, tab(R, R.fft_final), R.fft_final.n(), R.fft_final.blocks, R.fft_final.max_log,
+                     any_tree(R.fft_final));
     clk.mark(PH_COMMIT0);
     for (auto &l : R.tree0) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves, l.v.data());
     clk.mark(PH_TREE);

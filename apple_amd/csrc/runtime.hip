@@ -64,8 +64,9 @@ std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t l
     for (uint32_t p = 0; p < P; ++p) {
         uint32_t k = S / P + (p < S % P ? 1 : 0);
         uint32_t lo = hi - k;
-        // 2^11-element tiles unless the pass needs more rows; columns come from the lo low bits
-        const uint32_t log_t = std::max(kDefaultLogTile, k + 3);
+        // 2^12-element tiles with at least 16 contiguous elements (256 bytes) per row, up to
+        // the 2^13 cap; columns come from the lo low bits
+        const uint32_t log_t = std::min(log_t_pass, std::max(kDefaultLogTile + 1, k + 4));
         v.push_back({lo, k, std::min(log_t - k, lo)});
         hi = lo;
     }
@@ -757,7 +758,7 @@ static azk_status codeword_enqueue(azk_codeword_plan *p, const void *src, void *
     OneShot &o = p->shot;
     if (!o.matches(src, dst)) {  // tables are reused while the buffers stay the same
         o.reset();
-        plan_fft(o.passes, o.final_, src, dst, (const uint32_t *)p->tw.p, p->log_n, kDefaultLogTile, true, nullptr);
+        plan_fft(o.passes, o.final_, src, dst, (const uint32_t *)p->tw.p, p->log_n, kFftOnlyLogTile, true, nullptr);
         for (auto &l : o.passes) l.off = o.arena.put(l.v);
         o.final_.off = o.arena.put(o.final_.v);
         azk_status s = o.arena.upload();
@@ -766,7 +767,8 @@ static azk_status codeword_enqueue(azk_codeword_plan *p, const void *src, void *
     }
     hipStream_t st = p->ctx->stream;
     for (auto &l : o.passes) launch_fft_pass(st, o.arena.at<FftPassDesc>(l.off), l.n(), l.blocks, l.max_log);
-    launch_fft_final(st, o.arena.at<FftFinalDesc>(o.final_.off), o.final_.n(), o.final_.blocks, o.final_.max_log);
+    launch_fft_final(st, o.arena.at<FftFinalDesc>(o.final_.off), o.final_.n(), o.final_.blocks, o.final_.max_log,
+                     any_tree(o.final_));
     return AZK_OK;
 }
 

@@ -78,6 +78,10 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr uint32_t kP = 0x7fffffffu;
 constexpr uint32_t kDefaultLogTile = 11;    // 2048-element tiles: 40 KB LDS (final FFT pass), 32 KB (folds)
+// Final-pass tiles of a codeword that is not hashed: 20 KB of LDS, eight blocks (eight waves
+// per SIMD) per CU.  At 2^26 the final pass takes 0.59-0.65 ms against 0.68-0.75 ms with
+// 2^11-element tiles (four waves per SIMD); the passes above take 16 stages instead of 15.
+constexpr uint32_t kFftOnlyLogTile = 10;
 constexpr uint32_t kUpperLogChunk = 10;     // largest upper-tree reduction chunk
 // Commit trees: 8-node chunks, so every level of a step runs in the one-state-per-wave
 // form with one parent per wave (a 2^21 commit measured 2% faster than 16-node chunks).
@@ -233,6 +237,14 @@ struct Launch {
     bool empty() const { return v.empty(); }
     uint32_t n() const { return (uint32_t)v.size(); }
 };
+
+// Whether some final-FFT entry of a launch hashes its tile.
+inline bool any_tree(const Launch<FftFinalDesc> &l)
+{
+    for (const FftFinalDesc &d : l.v)
+        if (d.tree) return true;
+    return false;
+}
 
 // Upper-tree schedule of one tree: launches of <= 2^max_log_c-node chunks from `level`
 // to the root; step k of several trees can share one launch.
