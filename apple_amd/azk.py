@@ -189,6 +189,8 @@ def load() -> C.CDLL:
         "azk_codeword_plan_destroy": (None, [vp]),
         "azk_codeword_execute_resident": (st, [vp, vp, sz, vp, sz, sp]),
         "azk_codeword_execute": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_codeword_execute_m31_resident": (st, [vp, vp, sz, vp, sz, sp]),
+        "azk_codeword_execute_m31": (st, [vp, vp, sz, vp, sz, sp]),
         "azk_codeword_read_twiddles": (st, [vp, vp, sz]),
         "azk_witness_basis_plan_create": (st, [vp, u32, C.POINTER(vp)]),
         "azk_witness_basis_plan_destroy": (None, [vp]),
@@ -204,6 +206,8 @@ def load() -> C.CDLL:
         "azk_circle_prove_fft_coefficients_resident": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
         "azk_circle_prove_evaluations_resident": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
         "azk_circle_prove_fft_coefficients": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_circle_prove_fft_coefficients_m31_resident": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
+        "azk_circle_prove_fft_coefficients_m31": (st, [vp, vp, sz, vp, sz, C.POINTER(sz), sp]),
         "azk_circle_prover_clear": (st, [vp]),
         "azk_circle_prover_set_graph": (st, [vp, C.c_int]),
         "azk_circle_prover_set_timing": (st, [vp, C.c_int]),
@@ -589,6 +593,21 @@ class CircleCodewordPlan(_Plan):
         _ck(self.ctx._lib.azk_codeword_execute_resident(self.h, coeffs.h, coeff_offset, out.h, out_offset, C.byref(s)))
         return _stats_dict(s)
 
+    def execute_m31(self, coeffs: np.ndarray) -> np.ndarray:
+        """N M31 coefficients (the first components of a QM31 lift) -> N M31 evaluations."""
+        a = np.ascontiguousarray(coeffs, dtype=np.uint32)
+        out = np.zeros(1 << self.log_n, np.uint32)
+        s = Stats()
+        _ck(self.ctx._lib.azk_codeword_execute_m31(self.h, _ptr(a), a.size, _ptr(out), out.size, C.byref(s)))
+        self.last_stats = _stats_dict(s)
+        return out
+
+    def execute_m31_resident(self, coeffs: Buffer, out: Buffer, coeff_offset: int = 0, out_offset: int = 0) -> dict:
+        s = Stats()
+        _ck(self.ctx._lib.azk_codeword_execute_m31_resident(self.h, coeffs.h, coeff_offset, out.h, out_offset,
+                                                              C.byref(s)))
+        return _stats_dict(s)
+
     def read_twiddles(self) -> np.ndarray:
         n = (1 << self.log_n) - 1
         out = np.zeros(max(n, 1), np.uint32)
@@ -774,6 +793,20 @@ class CircleCodewordPCSFRIProverV1(_Plan):
         fn = (self.ctx._lib.azk_circle_prove_fft_coefficients_resident_verified if verified
               else self.ctx._lib.azk_circle_prove_fft_coefficients_resident)
         rc = fn(self.h, coeffs.h, offset, _ptr(self._out), self.proof_size, C.byref(n), C.byref(s))
+        return self._finish(rc, n, s)
+
+    def prove_fft_coefficients_m31_resident(self, coeffs: Buffer, offset: int = 0) -> bytes:
+        """The same proof from N M31 coefficients (each lifted to (c, 0, 0, 0))."""
+        n, s = C.c_size_t(0), Stats()
+        rc = self.ctx._lib.azk_circle_prove_fft_coefficients_m31_resident(self.h, coeffs.h, offset, _ptr(self._out),
+                                                                         self.proof_size, C.byref(n), C.byref(s))
+        return self._finish(rc, n, s)
+
+    def prove_fft_coefficients_m31(self, coeffs: np.ndarray) -> bytes:
+        a = np.ascontiguousarray(coeffs, dtype=np.uint32)
+        n, s = C.c_size_t(0), Stats()
+        rc = self.ctx._lib.azk_circle_prove_fft_coefficients_m31(self.h, _ptr(a), a.size, _ptr(self._out),
+                                                                self.proof_size, C.byref(n), C.byref(s))
         return self._finish(rc, n, s)
 
     def prove_evaluations_resident(self, evals: Buffer, offset: int = 0) -> bytes:

@@ -498,11 +498,41 @@ __global__ __launch_bounds__(256) void k_inverse(const uint32_t *__restrict__ in
 __device__ __forceinline__ qm31 ld_qm(const uint4 &v) { return {v.x, v.y, v.z, v.w}; }
 __device__ __forceinline__ uint4 st_qm(qm31 q) { return make_uint4(q.a, q.b, q.c, q.d); }
 
-__device__ __forceinline__ void butterfly(uint4 *s, uint32_t i1, uint32_t i2, uint32_t w)
+// Element forms of the FFT.  QmElem: a QM31 value (16 bytes).  M31Elem: one M31 word, for
+// a codeword of M31 coefficients -- the QM31 lift (a, 0, 0, 0) that the reference transforms
+// keeps its three upper components at zero through every butterfly (the twiddles are M31),
+// so only the first component is computed and moved: a quarter of the arithmetic and bytes.
+struct QmElem {
+    typedef uint4 S;
+    typedef qm31 V;
+    static constexpr uint32_t kMaxPerThread = 8;
+    __device__ static V ld(const S &x) { return ld_qm(x); }
+    __device__ static S st(V v) { return st_qm(v); }
+    __device__ static V scale(V v, uint32_t w) { return qm_scale(v, w); }
+    __device__ static V add(V a, V b) { return qm_add(a, b); }
+    __device__ static V sub(V a, V b) { return qm_sub(a, b); }
+    __device__ static S load_global(const S *p) { return ld_g4(p); }
+    __device__ static void store_global(S *p, S v) { st_g4(p, v); }
+};
+struct M31Elem {
+    typedef uint32_t S;
+    typedef uint32_t V;
+    static constexpr uint32_t kMaxPerThread = 16;
+    __device__ static V ld(const S &x) { return x; }
+    __device__ static S st(V v) { return v; }
+    __device__ static V scale(V v, uint32_t w) { return m31_mul(v, w); }
+    __device__ static V add(V a, V b) { return m31_add(a, b); }
+    __device__ static V sub(V a, V b) { return m31_sub(a, b); }
+    __device__ static S load_global(const S *p) { return *glob(p); }
+    __device__ static void store_global(S *p, S v) { *glob(p) = v; }
+};
+
+template <class E>
+__device__ __forceinline__ void butterfly(typename E::S *s, uint32_t i1, uint32_t i2, uint32_t w)
 {
-    qm31 l = ld_qm(s[i1]), r = qm_scale(ld_qm(s[i2]), w);
-    s[i1] = st_qm(qm_add(l, r));
-    s[i2] = st_qm(qm_sub(l, r));
+    const typename E::V l = E::ld(s[i1]), r = E::scale(E::ld(s[i2]), w);
+    s[i1] = E::st(E::add(l, r));
+    s[i2] = E::st(E::sub(l, r));
 }
 
 typedef __attribute__((address_space(1))) const u32x4 global_u32x4;
@@ -521,15 +551,15 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *tws, const uint32_t *__
         const uint32_t level = 31 - __builtin_clz(i), sg = lo + k - 1 - level;
         return (sg ? n - (n >> sg) : 0) + (high << level) + (i - (1u << level));
     };
-    const uint32_t S = 1u << k, B = blockDim.x;  // S <= 8B
+    const uint32_t S = 1u << k, B = blockDim.x;  // S <= 16B
     if (S >= B) {
         const uint32_t J = S / B;
-        uint32_t v[8];
+        uint32_t v[16];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
+        for (uint32_t j = 0; j < 16; ++j)
             if (j < J) v[j] = g[src(threadIdx.x + B * j)];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
+        for (uint32_t j = 0; j < 16; ++j)
             if (j < J) tws[threadIdx.x + B * j] = v[j];
     } else if (threadIdx.x < S) {
         tws[threadIdx.x] = g[src(threadIdx.x)];
@@ -549,7 +579,8 @@ __device__ __forceinline__ void stage_twiddles(uint32_t *tws, const uint32_t *__
 // (0/1 and 2/3, twiddle indices 2 hp and 2 hp + 1) run in registers between one LDS load
 // and one store: half the LDS traffic and barriers of one stage per round trip.  An odd
 // stage count leaves stage 0 as a single radix-2 stage.
-__device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *tws, uint32_t k, uint32_t m)
+template <class E>
+__device__ __forceinline__ void fft_tile_stages(typename E::S *s, const uint32_t *tws, uint32_t k, uint32_t m)
 {
     int t = (int)k - 1;
     for (; t >= 1; t -= 2) {
@@ -558,20 +589,20 @@ __device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *tws, u
         for (uint32_t g = threadIdx.x; g < quarter; g += blockDim.x) {
             const uint32_t hp = g >> sh;
             const uint32_t i0 = (hp << (sh + 2)) | (g & (d - 1));
-            qm31 a0 = ld_qm(s[i0]), a1 = ld_qm(s[i0 + d]), a2 = ld_qm(s[i0 + 2 * d]), a3 = ld_qm(s[i0 + 3 * d]);
+            typename E::V a0 = E::ld(s[i0]), a1 = E::ld(s[i0 + d]), a2 = E::ld(s[i0 + 2 * d]), a3 = E::ld(s[i0 + 3 * d]);
             const uint32_t w = tw_hi[hp], wa = tw_lo[2 * hp], wb = tw_lo[2 * hp + 1];
-            qm31 r = qm_scale(a2, w);
-            a2 = qm_sub(a0, r);
-            a0 = qm_add(a0, r);
-            r = qm_scale(a3, w);
-            a3 = qm_sub(a1, r);
-            a1 = qm_add(a1, r);
-            r = qm_scale(a1, wa);
-            s[i0 + d] = st_qm(qm_sub(a0, r));
-            s[i0] = st_qm(qm_add(a0, r));
-            r = qm_scale(a3, wb);
-            s[i0 + 3 * d] = st_qm(qm_sub(a2, r));
-            s[i0 + 2 * d] = st_qm(qm_add(a2, r));
+            typename E::V r = E::scale(a2, w);
+            a2 = E::sub(a0, r);
+            a0 = E::add(a0, r);
+            r = E::scale(a3, w);
+            a3 = E::sub(a1, r);
+            a1 = E::add(a1, r);
+            r = E::scale(a1, wa);
+            s[i0 + d] = E::st(E::sub(a0, r));
+            s[i0] = E::st(E::add(a0, r));
+            r = E::scale(a3, wb);
+            s[i0 + 3 * d] = E::st(E::sub(a2, r));
+            s[i0 + 2 * d] = E::st(E::add(a2, r));
         }
         __syncthreads();
     }
@@ -581,39 +612,43 @@ __device__ __forceinline__ void fft_tile_stages(uint4 *s, const uint32_t *tws, u
         for (uint32_t b = threadIdx.x; b < half; b += blockDim.x) {
             const uint32_t hp = b >> sh;
             const uint32_t i1 = (hp << (sh + 1)) | (b & ((1u << sh) - 1));
-            butterfly(s, i1, i1 + (1u << sh), tw_t[hp]);
+            butterfly<E>(s, i1, i1 + (1u << sh), tw_t[hp]);
         }
         __syncthreads();
     }
 }
 
-// Loads T <= 8 x blockDim elements of a tile into LDS (index(L) maps tile slot L to the
-// global element).  For T >= blockDim the trip count is uniform, so a thread's loads are
-// all in flight (global, not flat) before the first LDS store.
-template <class F>
-__device__ __forceinline__ void load_tile(uint4 *s, const uint4 *__restrict__ src, uint32_t T, F &&index)
+// Loads T <= E::kMaxPerThread x blockDim elements of a tile into LDS (index(L) maps tile
+// slot L to the global element).  For T >= blockDim the trip count is uniform, so a thread's
+// loads are all in flight (global, not flat) before the first LDS store.
+template <class E, class F>
+__device__ __forceinline__ void load_tile(typename E::S *s, const typename E::S *__restrict__ src, uint32_t T, F &&index)
 {
     const uint32_t B = blockDim.x;
+    constexpr uint32_t JM = E::kMaxPerThread;
     if (T >= B) {
-        const global_u32x4 *g = (const global_u32x4 *)src;
         const uint32_t J = T / B;
-        u32x4 v[8];
+        typename E::S v[JM];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
-            if (j < J) v[j] = g[index(threadIdx.x + B * j)];
+        for (uint32_t j = 0; j < JM; ++j)
+            if (j < J) v[j] = E::load_global(src + index(threadIdx.x + B * j));
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
-            if (j < J) s[threadIdx.x + B * j] = make_uint4(v[j].x, v[j].y, v[j].z, v[j].w);
+        for (uint32_t j = 0; j < JM; ++j)
+            if (j < J) s[threadIdx.x + B * j] = v[j];
     } else if (threadIdx.x < T) {
-        s[threadIdx.x] = ld_g4(src + index(threadIdx.x));
+        s[threadIdx.x] = E::load_global(src + index(threadIdx.x));
     }
 }
 
 // Non-final pass (lo >= 1): tile index -> (high bits, column chunk).  LDS: T elements,
-// then the tile's 2^k <= 2^10 twiddle slots.  T / 8 threads per block, 256 to 1024.
+// then the tile's 2^k <= 2^10 twiddle slots.  T / 8 (QM31) or T / 16 (M31) threads per
+// block, 256 to 1024.  For M31 the descriptor's src and dst are M31 word arrays.
+template <class E>
 __global__ __launch_bounds__(1024) void k_fft_pass(const FftPassDesc *__restrict__ tab, uint32_t ntab)
 {
-    extern __shared__ uint4 s[];
+    extern __shared__ uint4 sraw[];
+    typedef typename E::S S;
+    S *s = reinterpret_cast<S *>(sraw);
     uint32_t tile;
     const FftPassDesc &d = find_desc(tab, ntab, tile);
     const uint32_t lo = d.lo, k = d.k, m = d.m;
@@ -627,11 +662,11 @@ __global__ __launch_bounds__(1024) void k_fft_pass(const FftPassDesc *__restrict
     uint32_t *tws = reinterpret_cast<uint32_t *>(s + T);
     stage_twiddles(tws, d.tw, n, lo, k, high);
     auto at = [&](uint32_t L) { return base | ((uint64_t)(L >> m) << lo) | (L & cmask); };
-    load_tile(s, d.src, T, at);
+    load_tile<E>(s, reinterpret_cast<const S *>(d.src), T, at);
     __syncthreads();
-    fft_tile_stages(s, tws, k, m);
-    uint4 *dst = d.dst;
-    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) st_g4(dst + at(L), s[L]);
+    fft_tile_stages<E>(s, tws, k, m);
+    S *dst = reinterpret_cast<S *>(d.dst);
+    for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) E::store_global(dst + at(L), s[L]);
 }
 
 // Final pass: stages log_t - 1 .. 0 on a contiguous tile of T = 2^log_t elements,
@@ -655,10 +690,10 @@ __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restric
     const uint64_t base = (uint64_t)tile * T;
     uint32_t *tree = d.tree;
     if (d.run_fft) stage_twiddles(lds + 4 * T, d.tw, n, 0, log_t, tile);
-    load_tile(s, d.src, T, [&](uint32_t L) { return base + L; });
+    load_tile<QmElem>(s, d.src, T, [&](uint32_t L) { return base + L; });
     __syncthreads();
     if (d.run_fft) {
-        fft_tile_stages(s, lds + 4 * T, log_t, 0);
+        fft_tile_stages<QmElem>(s, lds + 4 * T, log_t, 0);
         uint4 *dst = d.dst;
         for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) st_g4(dst + base + L, s[L]);
         // every wave has read its elements before the subtree overwrites leaf pairs in place
@@ -666,6 +701,53 @@ __global__ __launch_bounds__(256) void k_fft_final(const FftFinalDesc *__restric
     }
     if constexpr (kTree) {
         if (tree) tile_tree_qm31(lds, T, base, tree, n);
+    }
+}
+
+// The final pass over M31 words (a codeword of M31 coefficients; src holds M31 words).
+// Without a tree the tile's M31 outputs go to dst as M31 words (LDS: T words, then T twiddle
+// words).  With one, dst is the QM31 first layer: the tile is widened to (a, 0, 0, 0) in LDS
+// (the M31 tile sits in the top quarter of the 4T-word leaf region of k_fft_final and every
+// thread holds its elements in registers across the widening barrier), written out, and
+// hashed into the tile's subtree exactly as k_fft_final does.  256 threads, T <= 2^12.
+template <bool kTree>
+__global__ __launch_bounds__(256) void k_fft_final_m31(const FftFinalDesc *__restrict__ tab, uint32_t ntab)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t tile;
+    const FftFinalDesc &d = find_desc(tab, ntab, tile);
+    const uint32_t log_t = d.log_t, T = 1u << log_t;
+    const uint64_t n = 1ull << d.log_n;
+    const uint64_t base = (uint64_t)tile * T;
+    uint32_t *s = kTree ? lds + 3 * T : lds;
+    uint32_t *tws = kTree ? lds + 4 * T : lds + T;
+    stage_twiddles(tws, d.tw, n, 0, log_t, tile);
+    load_tile<M31Elem>(s, reinterpret_cast<const uint32_t *>(d.src), T, [&](uint32_t L) { return base + L; });
+    __syncthreads();
+    fft_tile_stages<M31Elem>(s, tws, log_t, 0);
+    if constexpr (!kTree) {
+        uint32_t *dst = reinterpret_cast<uint32_t *>(d.dst);
+        for (uint32_t L = threadIdx.x; L < T; L += blockDim.x) glob(dst)[base + L] = s[L];
+    } else {
+        constexpr uint32_t JM = 16;
+        const uint32_t B = blockDim.x;
+        uint32_t v[JM];
+#pragma unroll
+        for (uint32_t j = 0; j < JM; ++j)
+            if (threadIdx.x + B * j < T) v[j] = s[threadIdx.x + B * j];
+        __syncthreads();  // every element is in a register before the widening overwrites them
+        uint4 *leaves = reinterpret_cast<uint4 *>(lds);
+#pragma unroll
+        for (uint32_t j = 0; j < JM; ++j) {
+            const uint32_t L = threadIdx.x + B * j;
+            if (L < T) {
+                const uint4 e = make_uint4(v[j], 0u, 0u, 0u);
+                leaves[L] = e;
+                st_g4(d.dst + base + L, e);
+            }
+        }
+        __syncthreads();
+        if (d.tree) tile_tree_qm31(lds, T, base, d.tree, n);
     }
 }
 
@@ -1038,19 +1120,28 @@ void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint3
     }
 }
 
-void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t)
+void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool m31)
 {
     if (!blocks) return;
     // T elements + 2^k <= 2^10 twiddle words (passes have at most 10 stages)
-    const uint32_t threads = std::min(1024u, std::max(256u, (1u << max_log_t) / 8));
-    hipLaunchKernelGGL(k_fft_pass, dim3(blocks), dim3(threads), ((size_t)16 << max_log_t) + 4096, st, t, n);
+    if (m31) {
+        const uint32_t threads = std::min(1024u, std::max(256u, (1u << max_log_t) / 16));
+        hipLaunchKernelGGL(k_fft_pass<M31Elem>, dim3(blocks), dim3(threads), ((size_t)4 << max_log_t) + 4096, st, t, n);
+    } else {
+        const uint32_t threads = std::min(1024u, std::max(256u, (1u << max_log_t) / 8));
+        hipLaunchKernelGGL(k_fft_pass<QmElem>, dim3(blocks), dim3(threads), ((size_t)16 << max_log_t) + 4096, st, t, n);
+    }
 }
 
-void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool tree)
+void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool tree,
+                      bool m31)
 {
     if (!blocks) return;
-    if (tree) hipLaunchKernelGGL(k_fft_final<true>, dim3(blocks), dim3(256), fft_tile_lds_bytes(max_log_t), st, t, n);
-    else hipLaunchKernelGGL(k_fft_final<false>, dim3(blocks), dim3(256), fft_tile_lds_bytes(max_log_t), st, t, n);
+    const size_t lds = fft_tile_lds_bytes(max_log_t);
+    if (m31 && tree) hipLaunchKernelGGL(k_fft_final_m31<true>, dim3(blocks), dim3(256), lds, st, t, n);
+    else if (m31) hipLaunchKernelGGL(k_fft_final_m31<false>, dim3(blocks), dim3(256), (size_t)8 << max_log_t, st, t, n);
+    else if (tree) hipLaunchKernelGGL(k_fft_final<true>, dim3(blocks), dim3(256), lds, st, t, n);
+    else hipLaunchKernelGGL(k_fft_final<false>, dim3(blocks), dim3(256), lds, st, t, n);
 }
 
 static bool fold_pair(const FoldDesc *host, uint32_t n, FoldPair &p)

@@ -126,9 +126,12 @@ void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t 
 // `single`: host copy of the table's only entry (n == 1), passed by value (`t` may then be null).
 void launch_merkle_raw(hipStream_t st, const MerkleRawDesc *t, uint32_t n, uint32_t blocks, bool aligned, uint32_t max_log_t,
                        const MerkleRawDesc *single = nullptr);
-void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t);
-// tree: some entry hashes its tile (FftFinalDesc::tree set).
-void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool tree);
+void launch_fft_pass(hipStream_t st, const FftPassDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool m31);
+// tree: some entry hashes its tile (FftFinalDesc::tree set).  m31: the entries transform M31
+// words (src and pass buffers hold N M31 words; dst is the QM31 first layer with a tree,
+// N M31 words without), tiles of at most 2^12 elements.
+void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t, bool tree,
+                      bool m31);
 // `host`: host copy of the table; a launch of at most two entries passes them by value.
 void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t,
                  const FoldDesc *host = nullptr);

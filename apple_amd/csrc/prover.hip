@@ -226,8 +226,12 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
         const uint32_t R_j = p->d.rounds;
         const TranscriptOp *ops = (const TranscriptOp *)p->ops.p;
         const ExecPlan &xp = p->xp;
-        plan_fft(R.fft_pass, R.fft_final, j.src, p->layers[0], (const uint32_t *)p->fft_tw.p, p->d.log_n, xp.log_tile_commit,
-                 j.mode == 0, p->trees[0]);
+        if (j.mode == 2)  // M31 words through the passes (in layer 1, unused until the first fold)
+            plan_fft(R.fft_pass_m31, R.fft_final_m31, j.src, p->layers[0], (const uint32_t *)p->fft_tw.p, p->d.log_n,
+                     xp.log_tile_commit, true, p->trees[0], true, p->layers[1]);
+        else
+            plan_fft(R.fft_pass, R.fft_final, j.src, p->layers[0], (const uint32_t *)p->fft_tw.p, p->d.log_n,
+                     xp.log_tile_commit, j.mode == 0, p->trees[0]);
         // round r's transcript program runs in the block that computes root r when the
         // root comes from an upper-tree step, and as its own launch otherwise
         auto prog = [&](uint32_t r) { return TranscriptDesc{ops + p->prog_a[r].first, p->tstate, p->prog_a[r].second, 0}; };
@@ -306,6 +310,8 @@ azk_status runner_build(Runner &R, azk_context *ctx, const std::vector<Job> &job
     // pack every table into one device buffer
     for (auto &l : R.fft_pass) l.off = R.tables.put(l.v);
     R.fft_final.off = R.tables.put(R.fft_final.v);
+    for (auto &l : R.fft_pass_m31) l.off = R.tables.put(l.v);
+    R.fft_final_m31.off = R.tables.put(R.fft_final_m31.v);
     for (auto &l : R.tree0) l.off = R.tables.put(l.v);
     for (uint32_t r = 0; r < R.rounds; ++r) {
         R.tr_a[r].off = R.tables.put(R.tr_a[r].v);
@@ -324,10 +330,13 @@ const D *tab(const Runner &R, const Launch<D> &l) { return R.tables.at<D>(l.off)
 azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
 {
     clk.mark(PH_START);
-    for (auto &l : R.fft_pass) launch_fft_pass(st, tab(R, l), l.n(), l.blocks, l.max_log);
+    for (auto &l : R.fft_pass) launch_fft_pass(st, tab(R, l), l.n(), l.blocks, l.max_log, false);
+    for (auto &l : R.fft_pass_m31) launch_fft_pass(st, tab(R, l), l.n(), l.blocks, l.max_log, true);
     clk.mark(PH_FFT);
     launch_fft_final(st, tab(R, R.fft_final), R.fft_final.n(), R.fft_final.blocks, R.fft_final.max_log,
-                     any_tree(R.fft_final));
+                     any_tree(R.fft_final), false);
+    launch_fft_final(st, tab(R, R.fft_final_m31), R.fft_final_m31.n(), R.fft_final_m31.blocks, R.fft_final_m31.max_log,
+                     any_tree(R.fft_final_m31), true);
     clk.mark(PH_COMMIT0);
     for (auto &l : R.tree0) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves, l.v.data());
     clk.mark(PH_TREE);
@@ -639,6 +648,8 @@ void runner_reset(Runner &R)
     R.tables.host.clear();
     R.fft_pass.clear();
     R.fft_final = {};
+    R.fft_pass_m31.clear();
+    R.fft_final_m31 = {};
     R.tree0.clear();
     R.rounds = 0;
     R.tr_a.clear();
@@ -793,6 +804,38 @@ extern "C" azk_status azk_circle_prove_fft_coefficients_resident(azk_circle_prov
     HIP_TRY(hipSetDevice(p->ctx->device));
     std::lock_guard<std::mutex> gc(p->ctx->mu);
     return prove_single(p, (uint8_t *)coeffs->ptr + off, 0, proof, cap, len, stats);
+}
+
+// The same proof from M31 line-basis coefficients: N words, coefficient i's QM31 lift is
+// (coeffs[i], 0, 0, 0).  The FFT runs on M31 words (a quarter of the QM31 FFT's arithmetic
+// and bytes); the first layer, its leaves and everything after are the QM31 path's.
+extern "C" azk_status azk_circle_prove_fft_coefficients_m31_resident(azk_circle_prover *p, const azk_buffer *coeffs,
+                                                                     size_t off, uint8_t *proof, size_t cap, size_t *len,
+                                                                     azk_stats *stats)
+{
+    CHECK_ARG(p && coeffs);
+    std::lock_guard<std::mutex> g(p->mu);
+    azk_status s = check_range(coeffs, off, (size_t)4 << p->d.log_n);
+    if (s) return s;
+    if (((uintptr_t)coeffs->ptr + off) % 4) return fail(AZK_ERR_INVALID_LAYOUT, "coefficients must be 4-byte aligned");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    std::lock_guard<std::mutex> gc(p->ctx->mu);
+    return prove_single(p, (uint8_t *)coeffs->ptr + off, 2, proof, cap, len, stats);
+}
+
+extern "C" azk_status azk_circle_prove_fft_coefficients_m31(azk_circle_prover *p, const uint32_t *coeffs, size_t words,
+                                                            uint8_t *proof, size_t cap, size_t *len, azk_stats *stats)
+{
+    CHECK_ARG(p && coeffs);
+    std::lock_guard<std::mutex> g(p->mu);
+    const size_t need = (size_t)4 << p->d.log_n;
+    if (words * 4 < need) return fail(AZK_ERR_INVALID_LAYOUT, "coefficient words %zu < %zu", words, need / 4);
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    azk_status s = p->coeff_staging.ensure(need);
+    if (s) return s;
+    std::lock_guard<std::mutex> gc(p->ctx->mu);
+    HIP_TRY(hipMemcpyAsync(p->coeff_staging.p, coeffs, need, hipMemcpyHostToDevice, p->ctx->stream));
+    return prove_single(p, p->coeff_staging.p, 2, proof, cap, len, stats);
 }
 
 extern "C" azk_status azk_circle_prove_evaluations_resident(azk_circle_prover *p, const azk_buffer *evals, size_t off,

@@ -65,16 +65,16 @@ namespace azk::prv {
 struct Job {
     azk_circle_prover *p;
     const void *src;
-    int mode;  // 0: src = line-basis coefficients, 1: src = codeword
-    const uint4 *layer0() const { return mode == 0 ? p->layers[0] : (const uint4 *)src; }
+    int mode;  // 0: src = line-basis coefficients, 1: src = codeword, 2: src = M31 line-basis coefficients
+    const uint4 *layer0() const { return mode == 1 ? (const uint4 *)src : p->layers[0]; }
 };
 
 struct Runner {
     azk_context *ctx = nullptr;
     std::vector<Job> jobs;
     TableArena tables;
-    std::vector<Launch<FftPassDesc>> fft_pass;
-    Launch<FftFinalDesc> fft_final;
+    std::vector<Launch<FftPassDesc>> fft_pass, fft_pass_m31;
+    Launch<FftFinalDesc> fft_final, fft_final_m31;
     std::vector<Launch<TreeDesc>> tree0;
     uint32_t rounds = 0;
     std::vector<Launch<TranscriptDesc>> tr_a;

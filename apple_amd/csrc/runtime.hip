@@ -53,20 +53,22 @@ bool overlaps(const azk_buffer *a, size_t ao, size_t an, const azk_buffer *b, si
     uintptr_t a0 = (uintptr_t)a->ptr + ao, b0 = (uintptr_t)b->ptr + bo;
     return a0 < b0 + bn && b0 < a0 + an;
 }
-std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t log_t_pass)
+std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, bool m31)
 {
     std::vector<FftPass> v;
     uint32_t lt = std::min(log_t_final, log_n);
     uint32_t S = log_n - lt;
     if (!S) return v;
-    const uint32_t kmax = log_t_pass - 3;
+    // QM31: tiles of 2^12 (64 KB) to 2^13 elements with >= 16 columns (256-byte rows), k <= 10.
+    // M31: tiles of 2^13 (32 KB) to 2^14 elements with >= 64 columns (256-byte rows) where the
+    // cap allows, else >= 16; k <= 10.
+    const uint32_t cap = m31 ? kMaxPassLogTileM31 : kMaxPassLogTile, min_t = m31 ? 13 : 12, want_m = m31 ? 6 : 4;
+    const uint32_t kmax = 10;
     uint32_t P = (S + kmax - 1) / kmax, hi = log_n;
     for (uint32_t p = 0; p < P; ++p) {
         uint32_t k = S / P + (p < S % P ? 1 : 0);
         uint32_t lo = hi - k;
-        // 2^12-element tiles with at least 16 contiguous elements (256 bytes) per row, up to
-        // the 2^13 cap; columns come from the lo low bits
-        const uint32_t log_t = std::min(log_t_pass, std::max(kDefaultLogTile + 1, k + 4));
+        const uint32_t log_t = std::min(cap, std::max(min_t, k + want_m));  // columns from the lo low bits
         v.push_back({lo, k, std::min(log_t - k, lo)});
         hi = lo;
     }
@@ -141,18 +143,19 @@ std::vector<TreeStep> tree_schedule(uint64_t m, uint32_t level, uint32_t max_log
 }
 
 void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &final_, const void *src, void *dst,
-              const uint32_t *tw, uint32_t log_n, uint32_t log_t, bool run_fft, uint32_t *tree)
+              const uint32_t *tw, uint32_t log_n, uint32_t log_t, bool run_fft, uint32_t *tree, bool m31, void *scratch)
 {
     const uint32_t lt = std::min(log_t, log_n);
     const void *cur = src;
+    void *mid = m31 && scratch ? scratch : dst;  // where the passes leave the codeword for the final pass
     if (run_fft) {
-        auto fp = fft_passes(log_n, log_t);
+        auto fp = fft_passes(log_n, log_t, m31);
         if (passes.size() < fp.size()) passes.resize(fp.size());
         for (size_t i = 0; i < fp.size(); ++i) {
-            FftPassDesc d{(const uint4 *)cur, (uint4 *)dst, tw, log_n, fp[i].lo, fp[i].k, fp[i].m, 0, 0};
+            FftPassDesc d{(const uint4 *)cur, (uint4 *)mid, tw, log_n, fp[i].lo, fp[i].k, fp[i].m, 0, 0};
             const uint32_t tlog = fp[i].k + fp[i].m;
             passes[i].add(d, (uint32_t)(1ull << (log_n - tlog)), tlog);
-            cur = dst;
+            cur = mid;
         }
     }
     FftFinalDesc f{(const uint4 *)cur, (uint4 *)dst, tw, tree, log_n, lt, run_fft ? 1u : 0u, 0};
@@ -748,17 +751,22 @@ struct azk_codeword_plan {
     azk_context *ctx;
     uint32_t log_n;
     DeviceMem tw, in, out;
-    OneShot shot;
+    OneShot shot, shot_m31;
     std::mutex mu;
 };
 
-// Circle FFT of one job through the table-driven pass/final kernels.
-static azk_status codeword_enqueue(azk_codeword_plan *p, const void *src, void *dst)
+// Final-pass tiles of the M31 codeword: 2^12 words (16 KB plus 16 KB of twiddles).
+constexpr uint32_t kFftM31LogTile = 12;
+
+// Circle FFT of one job through the table-driven pass/final kernels (m31: M31 words in and
+// out, the passes in place in dst).
+static azk_status codeword_enqueue(azk_codeword_plan *p, const void *src, void *dst, bool m31)
 {
-    OneShot &o = p->shot;
+    OneShot &o = m31 ? p->shot_m31 : p->shot;
     if (!o.matches(src, dst)) {  // tables are reused while the buffers stay the same
         o.reset();
-        plan_fft(o.passes, o.final_, src, dst, (const uint32_t *)p->tw.p, p->log_n, kFftOnlyLogTile, true, nullptr);
+        plan_fft(o.passes, o.final_, src, dst, (const uint32_t *)p->tw.p, p->log_n, m31 ? kFftM31LogTile : kFftOnlyLogTile,
+                 true, nullptr, m31, dst);
         for (auto &l : o.passes) l.off = o.arena.put(l.v);
         o.final_.off = o.arena.put(o.final_.v);
         azk_status s = o.arena.upload();
@@ -766,9 +774,9 @@ static azk_status codeword_enqueue(azk_codeword_plan *p, const void *src, void *
         o.keep(src, dst);
     }
     hipStream_t st = p->ctx->stream;
-    for (auto &l : o.passes) launch_fft_pass(st, o.arena.at<FftPassDesc>(l.off), l.n(), l.blocks, l.max_log);
+    for (auto &l : o.passes) launch_fft_pass(st, o.arena.at<FftPassDesc>(l.off), l.n(), l.blocks, l.max_log, m31);
     launch_fft_final(st, o.arena.at<FftFinalDesc>(o.final_.off), o.final_.n(), o.final_.blocks, o.final_.max_log,
-                     any_tree(o.final_));
+                     any_tree(o.final_), m31);
     return AZK_OK;
 }
 
@@ -776,7 +784,7 @@ extern "C" azk_status azk_codeword_plan_create(azk_context *ctx, uint32_t log_n,
 {
     CHECK_ARG(ctx && out);
     if (log_n < 1 || log_n > 28) return fail(AZK_ERR_INVALID_LAYOUT, "domain log size %u outside 1..28", log_n);
-    auto *p = new azk_codeword_plan{ctx, log_n, {}, {}, {}, {}, {}};
+    auto *p = new azk_codeword_plan{ctx, log_n, {}, {}, {}, {}, {}, {}};
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> gc(ctx->mu);
     azk_status s = materialize_twiddles(ctx, log_n, 1, p->tw, nullptr);
@@ -802,7 +810,26 @@ extern "C" azk_status azk_codeword_execute_resident(azk_codeword_plan *p, const 
     HIP_TRY(hipSetDevice(p->ctx->device));
     std::lock_guard<std::mutex> gc(p->ctx->mu);
     return run_timed(p->ctx, stats, [&]() {
-        return codeword_enqueue(p, (uint8_t *)coeffs->ptr + coff, (uint8_t *)out->ptr + ooff);
+        return codeword_enqueue(p, (uint8_t *)coeffs->ptr + coff, (uint8_t *)out->ptr + ooff, false);
+    });
+}
+
+extern "C" azk_status azk_codeword_execute_m31_resident(azk_codeword_plan *p, const azk_buffer *coeffs, size_t coff,
+                                                        azk_buffer *out, size_t ooff, azk_stats *stats)
+{
+    CHECK_ARG(p && coeffs && out);
+    std::lock_guard<std::mutex> g(p->mu);
+    const size_t bytes = (size_t)4 << p->log_n;
+    azk_status s = check_range(coeffs, coff, bytes);
+    if (!s) s = check_range(out, ooff, bytes);
+    if (s) return s;
+    if (overlaps(coeffs, coff, bytes, out, ooff, bytes)) return fail(AZK_ERR_INVALID_LAYOUT, "coefficient and output ranges alias");
+    if (((uintptr_t)coeffs->ptr + coff) % 4 || ((uintptr_t)out->ptr + ooff) % 4)
+        return fail(AZK_ERR_INVALID_LAYOUT, "M31 buffers must be 4-byte aligned");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    std::lock_guard<std::mutex> gc(p->ctx->mu);
+    return run_timed(p->ctx, stats, [&]() {
+        return codeword_enqueue(p, (uint8_t *)coeffs->ptr + coff, (uint8_t *)out->ptr + ooff, true);
     });
 }
 
@@ -820,7 +847,28 @@ extern "C" azk_status azk_codeword_execute(azk_codeword_plan *p, const void *coe
     std::lock_guard<std::mutex> gc(p->ctx->mu);
     return run_timed(p->ctx, stats, [&]() -> azk_status {
         HIP_TRY(hipMemcpyAsync(p->in.p, coeffs, bytes, hipMemcpyHostToDevice, p->ctx->stream));
-        azk_status e = codeword_enqueue(p, p->in.p, p->out.p);
+        azk_status e = codeword_enqueue(p, p->in.p, p->out.p, false);
+        if (e) return e;
+        HIP_TRY(hipMemcpyAsync(evals, p->out.p, bytes, hipMemcpyDeviceToHost, p->ctx->stream));
+        return AZK_OK;
+    });
+}
+
+extern "C" azk_status azk_codeword_execute_m31(azk_codeword_plan *p, const uint32_t *coeffs, size_t in_words,
+                                               uint32_t *evals, size_t out_words, azk_stats *stats)
+{
+    CHECK_ARG(p && coeffs && evals);
+    std::lock_guard<std::mutex> g(p->mu);
+    const size_t words = (size_t)1 << p->log_n, bytes = 4 * words;
+    if (in_words < words || out_words < words) return fail(AZK_ERR_INVALID_LAYOUT, "host arrays shorter than N words");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    azk_status s = p->in.ensure(bytes);
+    if (!s) s = p->out.ensure(bytes);
+    if (s) return s;
+    std::lock_guard<std::mutex> gc(p->ctx->mu);
+    return run_timed(p->ctx, stats, [&]() -> azk_status {
+        HIP_TRY(hipMemcpyAsync(p->in.p, coeffs, bytes, hipMemcpyHostToDevice, p->ctx->stream));
+        azk_status e = codeword_enqueue(p, p->in.p, p->out.p, true);
         if (e) return e;
         HIP_TRY(hipMemcpyAsync(evals, p->out.p, bytes, hipMemcpyDeviceToHost, p->ctx->stream));
         return AZK_OK;

@@ -186,12 +186,14 @@ azk_status run_timed(azk_context *ctx, azk_stats *stats, F &&enqueue)
 struct FftPass { uint32_t lo, k, m; };
 
 // Stages log_n-1 .. lt (lt = min(log_t_final, log_n)) split into the fewest balanced
-// passes of at most log_t_pass - 3 stages.  A pass of k stages runs on tiles of
-// 2^max(11, k + 3) elements: 2^k strided rows x >= 8 contiguous columns (128 bytes), so the
-// 10 stages above a 2^21 codeword's final pass take one 2^13-element pass (132 KB of LDS,
-// one block per CU) instead of two passes over HBM.
-constexpr uint32_t kMaxPassLogTile = 13;
-std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, uint32_t log_t_pass = kMaxPassLogTile);
+// passes of at most 10 stages.  A QM31 pass of k stages runs on tiles of
+// 2^min(13, max(12, k + 4)) elements: 2^k strided rows x >= 8 contiguous columns, 16 (256
+// bytes) below the cap -- at 2^26 the first pass takes 0.50 ms against 0.59 ms with 128-byte
+// rows -- and the 10 stages above a 2^21 codeword's final pass take one 2^13-element pass
+// (132 KB of LDS, one block per CU) instead of two passes over HBM.  An M31 pass (m31)
+// runs on 2^min(14, max(13, k + 6)) words.
+constexpr uint32_t kMaxPassLogTile = 13, kMaxPassLogTileM31 = 14;
+std::vector<FftPass> fft_passes(uint32_t log_n, uint32_t log_t_final, bool m31 = false);
 
 // FFT twiddles (and, with `fold_tw`, the first `rounds` stages inverted = FRI fold twiddles).
 azk_status materialize_twiddles(azk_context *ctx, uint32_t log_n, uint32_t rounds, DeviceMem &fft_tw, DeviceMem *fold_tw);
@@ -267,8 +269,12 @@ ExecPlan to_exec_plan(const azk_circle_exec_plan &x);
 azk_circle_exec_plan from_exec_plan(const ExecPlan &x);
 
 // Appends the FFT of one job (src -> dst, optional leaf-hash tree) to pass/final launches.
+// m31: src is N M31 words (the coefficients' first components; the QM31 lift is (a, 0, 0, 0)),
+// the passes work in `scratch` (N M31 words, dst if null) and the final pass writes dst:
+// the QM31 first layer with a tree, N M31 words without.  The launches take only one kind.
 void plan_fft(std::vector<Launch<FftPassDesc>> &passes, Launch<FftFinalDesc> &final_, const void *src, void *dst,
-              const uint32_t *tw, uint32_t log_n, uint32_t log_t, bool run_fft, uint32_t *tree);
+              const uint32_t *tw, uint32_t log_n, uint32_t log_t, bool run_fft, uint32_t *tree, bool m31 = false,
+              void *scratch = nullptr);
 
 // Appends one fold round of one job to the big-tile or latency-form launch; returns the
 // log tile size (the level whose nodes remain for the upper tree).

@@ -179,6 +179,15 @@ azk_status azk_codeword_execute(azk_codeword_plan *plan, const void *coeffs, siz
  * Circle FFT of the coefficients (azk_host_circle_fft). */
 azk_status azk_codeword_execute_verified(azk_codeword_plan *plan, const void *coeffs, size_t in_bytes, void *evals,
                                          size_t out_bytes, azk_stats *stats);
+/* The codeword of M31 coefficients: N M31 words in (coefficient i's QM31 lift is (c_i, 0, 0, 0)),
+ * N M31 words out (the first components of the QM31 codeword; the other three are zero).  No
+ * counterpart in the reference, which lifts M31 coefficients to QM31 and runs the QM31 FFT;
+ * for callers whose coefficients are M31 (a quarter of the arithmetic and bytes).  Buffers
+ * 4-byte aligned, not aliased. */
+azk_status azk_codeword_execute_m31_resident(azk_codeword_plan *plan, const azk_buffer *coeffs, size_t coeff_offset,
+                                             azk_buffer *out, size_t out_offset, azk_stats *stats);
+azk_status azk_codeword_execute_m31(azk_codeword_plan *plan, const uint32_t *coeffs, size_t in_words, uint32_t *evals,
+                                    size_t out_words, azk_stats *stats);
 /* FFT twiddles (N - 1 words) as materialized on the device (readCodewordTwiddles :818-820). */
 azk_status azk_codeword_read_twiddles(azk_codeword_plan *plan, uint32_t *out, size_t words);
 
@@ -267,6 +276,16 @@ size_t azk_circle_prover_proof_size(const azk_circle_prover *prover);
 azk_status azk_circle_prove_fft_coefficients_resident(azk_circle_prover *prover, const azk_buffer *coeffs,
                                                       size_t offset, uint8_t *proof, size_t capacity,
                                                       size_t *proof_len, azk_stats *stats);
+/* proveCircleFFTCoefficientsResident (CircleCodeword.swift:1007-1018) for M31 coefficients:
+ * N M31 words, coefficient i lifted to (coeffs[i], 0, 0, 0) -- the same proof bytes as
+ * azk_circle_prove_fft_coefficients_resident on the lifted array, with the FFT on M31 words.
+ * The reference has no M31-input entry; it is for callers whose coefficients are M31 (the
+ * benchmark configuration's are).  Host-array form: `words` >= N. */
+azk_status azk_circle_prove_fft_coefficients_m31_resident(azk_circle_prover *prover, const azk_buffer *coeffs,
+                                                          size_t offset, uint8_t *proof, size_t capacity,
+                                                          size_t *proof_len, azk_stats *stats);
+azk_status azk_circle_prove_fft_coefficients_m31(azk_circle_prover *prover, const uint32_t *coeffs, size_t words,
+                                                 uint8_t *proof, size_t capacity, size_t *proof_len, azk_stats *stats);
 /* prove(evaluationsBuffer:offset:) (CircleFRIFoldPlan.swift:782-848). */
 azk_status azk_circle_prove_evaluations_resident(azk_circle_prover *prover, const azk_buffer *evals, size_t offset,
                                                  uint8_t *proof, size_t capacity, size_t *proof_len,
