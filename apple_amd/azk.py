@@ -213,6 +213,7 @@ def load() -> C.CDLL:
         "azk_circle_prover_set_timing": (st, [vp, C.c_int]),
         "azk_circle_prover_phase_times": (st, [vp, C.POINTER(PhaseTimes)]),
         "azk_circle_batch_create": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp)]),
+        "azk_circle_batch_create_m31": (st, [C.POINTER(vp), C.POINTER(vp), C.POINTER(sz), u32, C.POINTER(vp)]),
         "azk_circle_batch_destroy": (None, [vp]),
         "azk_circle_batch_tune": (st, [vp, C.POINTER(TuningConfig), C.POINTER(TuningResult)]),
         "azk_circle_batch_prove": (st, [vp, C.POINTER(vp), C.POINTER(sz), C.POINTER(sz), sp]),
@@ -886,7 +887,8 @@ def prove_batch(provers: list, coeff_buffers: list, offsets=None) -> tuple[list[
 class CircleBatchPlan:
     """A fixed set of jobs captured once as one graph (azk_circle_batch_*)."""
 
-    def __init__(self, provers: list, coeff_buffers: list, offsets=None):
+    def __init__(self, provers: list, coeff_buffers: list, offsets=None, m31: bool = False):
+        """m31: the buffers hold each job's N M31 coefficients (azk_circle_batch_create_m31)."""
         self.lib = load()
         self.provers, self.buffers = list(provers), list(coeff_buffers)
         n = len(self.provers)
@@ -894,7 +896,8 @@ class CircleBatchPlan:
         ph = (vp * n)(*[p.h for p in self.provers])
         bh = (vp * n)(*[b.h for b in self.buffers])
         h = vp()
-        _ck(self.lib.azk_circle_batch_create(ph, bh, offs, n, C.byref(h)))
+        create = self.lib.azk_circle_batch_create_m31 if m31 else self.lib.azk_circle_batch_create
+        _ck(create(ph, bh, offs, n, C.byref(h)))
         self.h = h
         _track(self)
         self._outs = [np.zeros(p.proof_size, np.uint8) for p in self.provers]

@@ -935,21 +935,23 @@ extern "C" azk_status azk_circle_prover_clear(azk_circle_prover *p)
 // ====================================================================== batched form
 
 
-extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
-                                              const size_t *offsets, uint32_t jobs, azk_circle_batch **out)
+// mode 0: QM31 line-basis coefficients (16 bytes each); mode 2: M31 words (4 bytes each).
+static azk_status batch_create(azk_circle_prover *const *provers, const azk_buffer *const *coeffs, const size_t *offsets,
+                               uint32_t jobs, int mode, azk_circle_batch **out)
 {
     CHECK_ARG(provers && coeffs && offsets && out && jobs);
     azk_context *ctx = provers[0]->ctx;
+    const size_t word = mode == 2 ? 4 : 16;
     std::vector<Job> list;
     for (uint32_t j = 0; j < jobs; ++j) {
         CHECK_ARG(provers[j] && coeffs[j]);
         if (provers[j]->ctx != ctx) return fail(AZK_ERR_INVALID_ARGUMENT, "all provers must share one context");
         for (uint32_t k = 0; k < j; ++k)
             if (provers[k] == provers[j]) return fail(AZK_ERR_INVALID_ARGUMENT, "a prover appears twice in one batch");
-        azk_status s = check_range(coeffs[j], offsets[j], (size_t)16 << provers[j]->d.log_n);
+        azk_status s = check_range(coeffs[j], offsets[j], word << provers[j]->d.log_n);
         if (s) return s;
-        if (((uintptr_t)coeffs[j]->ptr + offsets[j]) % 16) return fail(AZK_ERR_INVALID_LAYOUT, "job %u misaligned", j);
-        list.push_back(Job{provers[j], (const uint8_t *)coeffs[j]->ptr + offsets[j], 0});
+        if (((uintptr_t)coeffs[j]->ptr + offsets[j]) % word) return fail(AZK_ERR_INVALID_LAYOUT, "job %u misaligned", j);
+        list.push_back(Job{provers[j], (const uint8_t *)coeffs[j]->ptr + offsets[j], mode});
     }
     HIP_TRY(hipSetDevice(ctx->device));
     auto *b = new azk_circle_batch();
@@ -964,6 +966,18 @@ extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers,
     }
     *out = b;
     return AZK_OK;
+}
+
+extern "C" azk_status azk_circle_batch_create(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
+                                              const size_t *offsets, uint32_t jobs, azk_circle_batch **out)
+{
+    return batch_create(provers, coeffs, offsets, jobs, 0, out);
+}
+
+extern "C" azk_status azk_circle_batch_create_m31(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
+                                                  const size_t *offsets, uint32_t jobs, azk_circle_batch **out)
+{
+    return batch_create(provers, coeffs, offsets, jobs, 2, out);
 }
 
 extern "C" void azk_circle_batch_destroy(azk_circle_batch *b)

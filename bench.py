@@ -1,10 +1,12 @@
 """Benchmark: resident Circle-PCS commit superstep on one MI355X.
 
 Workload (one step = one commit): P(x), Q(x) of 2^19 M31 coefficients each (SplitMix64,
-seeded), lifted to QM31 and laid out as interleaved line-basis coefficients, Circle FFT to
-a 2^21 codeword on the canonical bit-reversed domain, SHA3-256 Merkle commitment of the
-16-byte QM31 leaves, 10 Circle FRI fold rounds with on-device Fiat-Shamir challenges,
-32 query openings; only the proof bytes (roots, final layer, openings) are read back.
+seeded), laid out as interleaved line-basis coefficients (the reference lifts them to QM31
+(a, 0, 0, 0)), Circle FFT to a 2^21 codeword on the canonical bit-reversed domain, SHA3-256
+Merkle commitment of the 16-byte QM31 leaves, 10 Circle FRI fold rounds with on-device
+Fiat-Shamir challenges, 32 query openings; only the proof bytes (roots, final layer,
+openings) are read back.  The timed entry takes the M31 words (its FFT runs on M31 words);
+the QM31-input entry on the lifted array gives the same proof and is timed beside it.
 Inputs are resident in HBM before the timed region.  Every proof of the timed region is
 byte-identical to the first, which is checked against the oracle (--check).
 
@@ -32,14 +34,16 @@ LOG_N, ROUNDS, SECURITY = 21, 10, (2, 32, 1, 0)  # log blowup 2, 32 queries, fol
 DIGEST = bytes(0xA0 + i for i in range(32))
 HBM_PEAK_GBS = 8000.0
 # VALU roof of the hashing kernels, in Keccak-f[1600] permutations per second.  gfx950 issues a
-# wave64 v_xor_b32 / v_bitop3_b32 in 2 SIMD cycles and v_alignbit_b32 in 4 (measured live by
-# azk_calibrate_valu).  The one-lane permutation (gfx950 assembly of kf_v) is, per round,
-# 70 v_bitop3_b32 + 62 v_xor_b32 + 58 v_alignbit_b32: 24 x (132 x 2 + 58 x 4) = 11,904 issue
-# cycles per wave of 64 permutations.  At the 2.4 GHz peak clock on 256 CUs x 4 SIMDs:
+# wave64 v_xor_b32 / v_bitop3_b32 in 2 SIMD cycles (a v_bitop3_b32 whose three VGPR sources
+# share one parity in 4) and v_alignbit_b32 in 4 (measured live by azk_calibrate_valu).  The
+# one-lane permutation kf_v runs its middle 22 rounds on hand-placed registers (keccak_round_asm.hpp):
+# 120 v_bitop3_b32 + 2 v_xor_b32 + 58 v_alignbit_b32 per round, no same-parity three-source op;
+# its first and last rounds are compiled (70 v_bitop3_b32 + 62 v_xor_b32 + 58 v_alignbit_b32).
+# 22 x (122 x 2 + 58 x 4) + 2 x (132 x 2 + 58 x 4) = 11,464 issue cycles per wave of 64
+# permutations.  At the 2.4 GHz peak clock on 256 CUs x 4 SIMDs:
 SIMDS, PEAK_CLOCK_HZ = 256 * 4, 2.4e9
-KECCAK_ROUND_DOUBLE_RATE, KECCAK_ROUND_ALIGNBIT, KECCAK_ROUNDS = 132, 58, 24
-KECCAK_WAVE_CYCLES = KECCAK_ROUNDS * (2 * KECCAK_ROUND_DOUBLE_RATE + 4 * KECCAK_ROUND_ALIGNBIT)
-KECCAK_PEAK_GPERMS = SIMDS * PEAK_CLOCK_HZ * 64 / KECCAK_WAVE_CYCLES / 1e9   # 13.2 G permutations/s
+KECCAK_WAVE_CYCLES = 22 * (2 * 122 + 4 * 58) + 2 * (2 * 132 + 4 * 58)
+KECCAK_PEAK_GPERMS = SIMDS * PEAK_CLOCK_HZ * 64 / KECCAK_WAVE_CYCLES / 1e9   # 13.7 G permutations/s
 CALIBRATION: dict = {}
 
 
@@ -198,22 +202,25 @@ def cpu_all_cores(coeffs, threads: int | None = None) -> dict:
 
 def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
     """64 independent commits, log sizes uniform in 14..21 (seeded), one batch program vs
-    the same jobs proved one after another; every batched proof equals its plain proof."""
+    the same jobs proved one after another; every batched proof equals its plain proof.  Both
+    sides take each job's M31 coefficients (the QM31-input batch is timed beside)."""
     rng = np.random.default_rng(seed)
     logs = [int(x) for x in rng.integers(14, 22, size=jobs)]
-    provers, bufs = [], []
+    provers, bufs, bufs_qm = [], [], []
     for j, ln in enumerate(logs):
         pr = azk.CircleCodewordPCSFRIProverV1(ctx, ln, ROUNDS, SECURITY, DIGEST)
         provers.append(pr)
-        bufs.append(ctx.upload(inputs.circle_fft_coefficients(seed=seed + j, log_n=ln)))
-    plain = [pr.prove_fft_coefficients_resident(b) for pr, b in zip(provers, bufs)]  # warm + reference
+        c = inputs.circle_fft_coefficients(seed=seed + j, log_n=ln)
+        bufs.append(ctx.upload(np.ascontiguousarray(c[:, 0])))
+        bufs_qm.append(ctx.upload(c))
+    plain = [pr.prove_fft_coefficients_m31_resident(b) for pr, b in zip(provers, bufs)]  # warm + reference
     t_plain = []
     for _ in range(reps):
         t0 = time.perf_counter()
         for pr, b in zip(provers, bufs):
-            pr.prove_fft_coefficients_resident(b)
+            pr.prove_fft_coefficients_m31_resident(b)
         t_plain.append(time.perf_counter() - t0)
-    plan = azk.CircleBatchPlan(provers, bufs)
+    plan = azk.CircleBatchPlan(provers, bufs, m31=True)
     if plan.prove() != plain:
         raise SystemExit("FATAL: batched proofs differ from plain single-job proofs")
     t_batch = []
@@ -222,15 +229,26 @@ def batched_form(ctx, seed: int = 4242, jobs: int = 64, reps: int = 3) -> dict:
         plan.prove()
         t_batch.append(time.perf_counter() - t0)
     plan.close()
+    plan_qm = azk.CircleBatchPlan(provers, bufs_qm)
+    if plan_qm.prove() != plain:
+        raise SystemExit("FATAL: QM31-input batched proofs differ from the M31-input proofs")
+    t_qm = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plan_qm.prove()
+        t_qm.append(time.perf_counter() - t0)
+    plan_qm.close()
     for pr in provers:
         pr.close()
     plain_rate = jobs / float(np.median(t_plain))
     batch_rate = jobs / float(np.median(t_batch))
     nbytes = sum(commit_bytes(ln, ROUNDS) for ln in logs)
     return {"workload": "64 circle_pcs_commit jobs, log_n ~ U{14..21}", "jobs": jobs,
+            "entry": "azk_circle_batch_create_m31 (plain: azk_circle_prove_fft_coefficients_m31_resident)",
             "plain_commits_per_s": plain_rate, "batched_commits_per_s": batch_rate,
             "speedup": batch_rate / plain_rate, "batched_gbps": nbytes / float(np.median(t_batch)) / 1e9,
-            "batch_ms": 1e3 * float(np.median(t_batch)), "bit_exact_vs_plain": True}
+            "batch_ms": 1e3 * float(np.median(t_batch)), "bit_exact_vs_plain": True,
+            "qm31_input_batched_commits_per_s": jobs / float(np.median(t_qm))}
 
 
 def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 300) -> dict:
@@ -628,14 +646,16 @@ def main() -> int:
     perms0 = n + (n - (n >> 11))
     t0_s = phases["commit0_ms"] / 1e3
     # the launch's 11 FFT stages are not counted: a lower bound on its VALU use
-    roofline = keccak_roofline(perms0, t0_s, kernel="k_fft_final (final FFT stages + leaf hashes + tile subtrees)",
-                               traffic=pmc_traffic("k_fft_final"),
+    roofline = keccak_roofline(perms0, t0_s, kernel="k_fft_final_m31<true> (final M31 FFT stages, QM31 leaves (a, 0, 0, 0), "
+                                                    "leaf hashes, tile subtrees)",
+                               traffic=pmc_traffic("k_fft_final_m31<true>"),
                                traffic_source="profiles/pmc_bytes_per_dispatch_2^21.txt (rocprofv3 --pmc FETCH_SIZE, "
                                               "WRITE_SIZE)",
                                algorithmic_permutations_per_launch=perms0, avg_launch_ms=t0_s * 1e3)
-    # the same launch against HBM: 2^21 x 16 B codeword read and written, 2^19 x 16 B twiddles,
-    # every tree node below the tile roots written (32 B each) -- far below the HBM roof
-    hbm_bytes = 2 * n * 16 + (n >> 2) * 16 + perms0 * 32
+    # the same launch against HBM: 2^21 M31 words read, the 2^21 x 16 B QM31 layer written,
+    # 2^21 x 4 B twiddles, every tree node below the tile roots written (32 B each) -- far below
+    # the HBM roof
+    hbm_bytes = n * 4 + n * 16 + n * 4 + perms0 * 32
     roofline["hbm_view"] = {"bound": "hbm", "achieved": hbm_bytes / t0_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": hbm_bytes / t0_s / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": hbm_bytes}
 

@@ -469,6 +469,10 @@ azk_status azk_chain_execute_verified(azk_chain_plan *plan, const uint32_t *eval
 typedef struct azk_circle_batch azk_circle_batch;
 azk_status azk_circle_batch_create(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
                                    const size_t *offsets, uint32_t jobs, azk_circle_batch **out);
+/* The same batch from M31 coefficients (N words per job, as azk_circle_prove_fft_coefficients_m31_resident:
+ * each job's proof equals its plain prove; 4-byte aligned buffers).  No reference counterpart. */
+azk_status azk_circle_batch_create_m31(azk_circle_prover *const *provers, const azk_buffer *const *coeffs,
+                                       const size_t *offsets, uint32_t jobs, azk_circle_batch **out);
 void azk_circle_batch_destroy(azk_circle_batch *batch);
 azk_status azk_circle_batch_prove(azk_circle_batch *batch, uint8_t *const *proofs, const size_t *capacities,
                                   size_t *proof_lens, azk_stats *stats);

@@ -118,6 +118,14 @@ def test_batch_of_64_jobs_log_14_to_21(ctx):
     plan = azk.CircleBatchPlan(provers, bufs)
     assert plan.prove() == plain
     assert plan.prove() == plain  # graph replay
+    # the same batch from the M31 words: the same proofs
+    m31 = [ctx.upload(np.ascontiguousarray(c[:, 0])) for c in coeffs]
+    plan_m31 = azk.CircleBatchPlan(provers, m31, m31=True)
+    assert plan_m31.prove() == plain
+    assert plan_m31.prove() == plain
+    plan_m31.close()
+    for b in m31:
+        b.free()
     checked = 0
     for j, ln in enumerate(logs):
         if ln <= 17:
