@@ -268,7 +268,7 @@ __device__ __forceinline__ void pair_parent(uint32_t *A, uint32_t q, uint32_t s,
 // whose first node has global index `g` in that level, up to one node at A[0..7].  Every
 // computed level is written to the tree in HBM.  By the number of parents of a level:
 // more than half the threads, one lane per parent (~10 us a level); down to kWaveParents
-// per wave, two lanes per parent (5.4 us); the serial top, one wave per parent (2.9 us).
+// per wave, two lanes per parent (5.4 us); the serial top, one wave per parent (2.5 us).
 #ifndef AZK_WAVE_PARENTS
 #define AZK_WAVE_PARENTS 2
 #endif
