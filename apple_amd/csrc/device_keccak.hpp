@@ -186,11 +186,17 @@ __device__ __forceinline__ void kround_h(uint32_t (&a)[25], uint32_t rc)
     a[0] ^= rc;
 }
 
-// `hi`: this lane holds the high halves (the odd lane of its pair).
+// `hi`: this lane holds the high halves (the odd lane of its pair).  Every round is
+// unrolled, the round constants scalar operands: 5.36 us per dependent permutation on one
+// wave against 5.83 us for a loop unrolled twice.
 __device__ __forceinline__ void kf_h(uint32_t (&a)[25], bool hi)
 {
-#pragma unroll 2
-    for (int r = 0; r < 24; ++r) kround_h(a, hi ? kRcHi[r] : kRcLo[r]);
+#pragma unroll
+    for (int r = 0; r < 24; ++r) {
+        uint32_t rl = kRcLo[r], rh = kRcHi[r];
+        asm volatile("" : "+s"(rl), "+s"(rh));
+        kround_h(a, hi ? rh : rl);
+    }
 }
 
 __device__ __forceinline__ uint32_t bperm(int addr, uint32_t v)
