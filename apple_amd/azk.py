@@ -254,6 +254,7 @@ def load() -> C.CDLL:
         "azk_server_create": (st, [C.POINTER(ServerDesc), C.POINTER(vp)]),
         "azk_server_destroy": (None, [vp]),
         "azk_server_submit_commit": (st, [vp, C.POINTER(ProverDesc), vp, sz, C.POINTER(C.c_uint64)]),
+        "azk_server_submit_commit_m31": (st, [vp, C.POINTER(ProverDesc), vp, sz, C.POINTER(C.c_uint64)]),
         "azk_server_submit_sumcheck": (st, [vp, vp, sz, C.POINTER(C.c_uint64)]),
         "azk_server_submit_sumcheck_digest": (st, [vp, vp, sz, vp, C.POINTER(C.c_uint64)]),
         "azk_server_submit_sumcheck_chunk": (st, [vp, vp, sz, u32, C.POINTER(C.c_uint64)]),
@@ -1256,6 +1257,14 @@ class Server:
         a = np.ascontiguousarray(coeffs, dtype=np.uint32)
         t = C.c_uint64(0)
         _ck(self.lib.azk_server_submit_commit(self.h, C.byref(desc), a.ctypes.data, a.nbytes, C.byref(t)))
+        self._keep[t.value] = a  # read in place until the job is done
+        return t.value
+
+    def submit_commit_m31(self, desc: ProverDesc, coeffs: np.ndarray) -> int:
+        """A commit from M31 line-basis coefficients (the proof of their QM31 lift)."""
+        a = np.ascontiguousarray(coeffs, dtype=np.uint32)
+        t = C.c_uint64(0)
+        _ck(self.lib.azk_server_submit_commit_m31(self.h, C.byref(desc), a.ctypes.data, a.size, C.byref(t)))
         self._keep[t.value] = a  # read in place until the job is done
         return t.value
 

@@ -13,8 +13,11 @@
 //    twiddles and input staging) stay cached per shape and are reused.
 //  * auxiliary lanes (threads, one stream each) run sum-check jobs (the words are hashed
 //    on the host, the serial initial digest, while the device rounds run, unless the caller
-//    supplied that digest), sum-check chunk jobs and fold-chain jobs.  An idle lane keeps its plan and buffers for the next job of the same shape
-//    unless another job waits for memory.
+//    supplied that digest), sum-check chunk jobs and fold-chain jobs.  A lane keeps the plan
+//    and buffers of every shape it has run for later jobs of that shape, and gives them
+//    back (least recently used first) when another job waits for memory.
+//  * commit inputs are QM31 line-basis coefficients or M31 words (the FFT then runs on the
+//    M31 words, a quarter of the upload); a batch may hold both kinds.
 //  * admission: a job is admitted when the bytes of live device state (cached provers,
 //    sum-check plans) plus its own need fit the budget; idle cached state is evicted
 //    first when it does not.  A job larger than the whole budget fails with
@@ -40,6 +43,7 @@ struct SJob {
     uint32_t kind = 0;
     azk_circle_prover_desc desc{};
     const void *input = nullptr;
+    int mode = 0;      // commit input: 0 = QM31 line-basis coefficients, 2 = M31 words
     size_t count = 0;  // sum-check words
     uint32_t log_n = 0, rounds = 0;   // sum-check and fold shapes
     std::vector<uint32_t> challenges;  // fold: 4 words per round, copied at submit
@@ -71,10 +75,14 @@ size_t fold_need(uint32_t log_n, uint32_t rounds)
     return 8 * n + 16 * n + 16 * n + 16 * (n >> rounds) + 64 * 1024;
 }
 
-// The device state an auxiliary lane keeps between jobs of the same shape.
+// The device state an auxiliary lane keeps between jobs of the same shape.  A lane keeps one
+// per shape it has run (sum-check plans are small next to the budget), so a mix of shapes
+// does not free and re-allocate device memory (a hipFree waits for the whole device) on
+// every change; under memory pressure the least recently used ones go first.
 struct AuxCache {
     uint32_t kind = ~0u, log_n = 0, rounds = 0;
     size_t bytes = 0;
+    uint64_t last_use = 0;
     azk_sumcheck_plan *sc = nullptr;
     azk_sumcheck_chunk_plan *chunk = nullptr;
     azk_fold_plan *fold = nullptr;
@@ -89,6 +97,14 @@ struct AuxCache {
         if (in) azk_buffer_free(in);
         if (out) azk_buffer_free(out);
         *this = AuxCache{};
+    }
+};
+
+struct AuxKey {
+    uint32_t kind, log_n, rounds;
+    bool operator<(const AuxKey &o) const
+    {
+        return kind != o.kind ? kind < o.kind : log_n != o.log_n ? log_n < o.log_n : rounds < o.rounds;
     }
 };
 
@@ -148,10 +164,13 @@ void evict_idle(azk_server *s, size_t want)
     }
 }
 
-// Admits pending commits (first fit in arrival order) and enqueues their uploads.
+// Admits pending commits (first fit in arrival order) and enqueues their uploads, up to one
+// device program ahead: the jobs of the next batch get their provers and uploads while the
+// current one runs, and later arrivals wait unadmitted (no provers built or bytes moved for
+// jobs that cannot start yet).
 void admit_commits(azk_server *s)
 {
-    for (;;) {
+    while (s->ready.size() < s->d.max_batch) {
         SJob *j = nullptr;
         azk_circle_prover *p = nullptr;
         bool create = false;
@@ -188,7 +207,7 @@ void admit_commits(azk_server *s)
             }
         }
         j->prover = p;
-        const size_t bytes = (size_t)16 << j->desc.log_n;
+        const size_t bytes = (size_t)(j->mode == 2 ? 4 : 16) << j->desc.log_n;
         azk_status st = p->coeff_staging.ensure(bytes);
         hipError_t e = hipSuccess;
         if (!st) e = hipMemcpyAsync(p->coeff_staging.p, j->input, bytes, hipMemcpyHostToDevice, s->upload);
@@ -209,7 +228,7 @@ azk_status run_batch(azk_server *s, std::vector<SJob *> &batch)
 {
     hipStream_t st = s->ctx->stream;
     std::vector<Job> list;
-    for (SJob *j : batch) list.push_back(Job{j->prover, j->prover->coeff_staging.p, 0});
+    for (SJob *j : batch) list.push_back(Job{j->prover, j->prover->coeff_staging.p, j->mode});
     runner_reset(s->runner);
     azk_status r = runner_build(s->runner, s->ctx, list);
     if (r) return r;
@@ -357,10 +376,33 @@ azk_status run_aux(azk_server *s, size_t lane, AuxCache &c, SJob *j)
     return st;
 }
 
+AuxKey key_of(const SJob *j) { return AuxKey{j->kind, j->log_n, j->rounds}; }
+
+// The lane's least recently used cached state other than `keep` (end() if none).
+std::map<AuxKey, AuxCache>::iterator lru_victim(std::map<AuxKey, AuxCache> &caches, const AuxKey *keep)
+{
+    auto best = caches.end();
+    for (auto it = caches.begin(); it != caches.end(); ++it) {
+        if (keep && !(it->first < *keep) && !(*keep < it->first)) continue;
+        if (best == caches.end() || it->second.last_use < best->second.last_use) best = it;
+    }
+    return best;
+}
+
 void aux_lane(azk_server *s, size_t lane)
 {
     (void)hipSetDevice(s->d.device);
-    AuxCache cache;
+    std::map<AuxKey, AuxCache> caches;  // this lane's only: read and changed by this thread alone
+    uint64_t uses = 0;
+    // Frees one cached state (outside the server lock) and gives its bytes back.
+    auto drop = [&](std::unique_lock<std::mutex> &g, std::map<AuxKey, AuxCache>::iterator it) {
+        const size_t b = it->second.bytes;
+        g.unlock();
+        it->second.release();
+        caches.erase(it);
+        g.lock();
+        s->held -= b;
+    };
     for (;;) {
         SJob *j = nullptr;
         {
@@ -368,15 +410,20 @@ void aux_lane(azk_server *s, size_t lane)
             for (;;) {
                 if (!s->pend_aux.empty()) {
                     SJob *c = s->pend_aux.front();
-                    const bool same = cache.holds(c);
-                    const size_t freed = (!cache.empty() && !same) ? cache.bytes : 0;
+                    const AuxKey k = key_of(c);
+                    const bool same = caches.count(k) != 0;
                     const size_t add = same ? 0 : c->need;
-                    if (s->held - freed + add <= s->budget || c->need > s->budget) {
+                    if (s->held + add <= s->budget || c->need > s->budget) {
                         j = c;
                         s->pend_aux.pop_front();
                         if (s->aux_waiting == c->need) s->aux_waiting = 0;
-                        if (c->need <= s->budget) s->held = s->held - freed + add;
+                        if (c->need <= s->budget) s->held += add;
                         break;
+                    }
+                    auto victim = lru_victim(caches, &k);
+                    if (victim != caches.end()) {  // this lane's other shapes go first
+                        drop(g, victim);
+                        continue;
                     }
                     s->aux_waiting = c->need;  // ask the commit lane to evict idle provers
                     s->cv_commit.notify_one();
@@ -384,16 +431,12 @@ void aux_lane(azk_server *s, size_t lane)
                     continue;
                 }
                 if (s->stop) break;
-                if (!cache.empty() && (s->commit_blocked || s->aux_waiting)) {
+                if (!caches.empty() && (s->commit_blocked || s->aux_waiting)) {
                     // idle while another job waits for memory: give the cached bytes back
-                    const size_t b = cache.bytes;
-                    g.unlock();
-                    cache.release();
-                    g.lock();
-                    s->held -= b;
+                    drop(g, lru_victim(caches, nullptr));
                     continue;
                 }
-                if (!cache.empty()) s->cv_aux.wait_for(g, std::chrono::milliseconds(2));
+                if (!caches.empty()) s->cv_aux.wait_for(g, std::chrono::milliseconds(2));
                 else s->cv_aux.wait(g);
             }
         }
@@ -403,27 +446,61 @@ void aux_lane(azk_server *s, size_t lane)
             continue;
         }
         j->t_admit = now_ns();
-        const bool had = cache.holds(j);
-        azk_status st = run_aux(s, lane, cache, j);
-        if (st && !had && cache.empty()) {  // the cache was not built: its reservation goes back
-            std::lock_guard<std::mutex> g(s->mu);
-            s->held -= j->need;
+        const AuxKey k = key_of(j);
+        AuxCache &c = caches[k];
+        const bool had = !c.empty();
+        azk_status st = run_aux(s, lane, c, j);
+        c.last_use = ++uses;
+        if (c.empty()) {  // not built: the reservation goes back
+            caches.erase(k);
+            if (!had) {
+                std::lock_guard<std::mutex> g(s->mu);
+                s->held -= j->need;
+            }
         }
         j->batch_jobs = 1;
         std::string m = st ? azk_last_error() : "";
         finish(s, j, st, m.c_str());
     }
-    if (!cache.empty()) {
-        const size_t b = cache.bytes;
-        cache.release();
-        std::lock_guard<std::mutex> g(s->mu);
-        s->held -= b;
+    size_t b = 0;
+    for (auto &kv : caches) {
+        b += kv.second.bytes;
+        kv.second.release();
     }
+    std::lock_guard<std::mutex> g(s->mu);
+    s->held -= b;
 }
 
 } // namespace
 
 namespace {
+
+// mode 0: `bytes` of QM31 line-basis coefficients (16 per element); mode 2: M31 words (4 each).
+azk_status submit_commit(azk_server *s, const azk_circle_prover_desc *desc, const void *coeffs, size_t bytes, int mode,
+                         uint64_t *ticket)
+{
+    azk_status st = validate_desc(*desc);
+    if (st) return st;
+    const size_t want = (size_t)(mode == 2 ? 4 : 16) << desc->log_n;
+    if (bytes < want) return fail(AZK_ERR_INVALID_LAYOUT, "coefficient bytes %zu < %zu", bytes, want);
+    auto j = std::make_unique<SJob>();
+    j->kind = KIND_COMMIT;
+    j->desc = *desc;
+    j->input = coeffs;
+    j->mode = mode;
+    j->need = prover_footprint(*desc);
+    j->t_submit = now_ns();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
+        j->ticket = s->next_ticket++;
+        *ticket = j->ticket;
+        s->pend_commit.push_back(j.get());
+        s->jobs.emplace(j->ticket, std::move(j));
+    }
+    s->cv_commit.notify_one();
+    return AZK_OK;
+}
 
 azk_status submit_aux(azk_server *s, std::unique_ptr<SJob> j, uint64_t *ticket)
 {
@@ -537,26 +614,15 @@ azk_status azk_server_submit_commit(azk_server *s, const azk_circle_prover_desc 
                                     uint64_t *ticket)
 {
     CHECK_ARG(s && desc && coeffs && ticket);
-    azk_status st = validate_desc(*desc);
-    if (st) return st;
-    if (bytes < ((size_t)16 << desc->log_n))
-        return fail(AZK_ERR_INVALID_LAYOUT, "coefficient bytes %zu < %zu", bytes, (size_t)16 << desc->log_n);
-    auto j = std::make_unique<SJob>();
-    j->kind = KIND_COMMIT;
-    j->desc = *desc;
-    j->input = coeffs;
-    j->need = prover_footprint(*desc);
-    j->t_submit = now_ns();
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        if (s->stop) return fail(AZK_ERR_INVALID_ARGUMENT, "server is stopping");
-        j->ticket = s->next_ticket++;
-        *ticket = j->ticket;
-        s->pend_commit.push_back(j.get());
-        s->jobs.emplace(j->ticket, std::move(j));
-    }
-    s->cv_commit.notify_one();
-    return AZK_OK;
+    return submit_commit(s, desc, coeffs, bytes, 0, ticket);
+}
+
+azk_status azk_server_submit_commit_m31(azk_server *s, const azk_circle_prover_desc *desc, const uint32_t *coeffs,
+                                        size_t words, uint64_t *ticket)
+{
+    CHECK_ARG(s && desc && coeffs && ticket);
+    if (words > SIZE_MAX / 4) return fail(AZK_ERR_INVALID_LAYOUT, "coefficient word count %zu", words);
+    return submit_commit(s, desc, coeffs, words * 4, 2, ticket);
 }
 
 azk_status azk_server_submit_sumcheck_digest(azk_server *s, const uint32_t *evals, size_t count, const uint8_t digest[32],

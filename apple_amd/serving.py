@@ -72,9 +72,11 @@ def make_trace(seed: int, jobs: int, rate: float, commit_logs=(12, 22), sumcheck
 
 
 class Inputs:
-    """One pinned input per (kind, log size) seen in a trace, and the sum-checks' digests."""
+    """One pinned input per (kind, log size) seen in a trace, and the sum-checks' digests.
+    With `m31_commits` a commit's input is its 2^k M31 line-basis coefficients (the words
+    whose QM31 lift the default form holds: the same proof from a quarter of the bytes)."""
 
-    def __init__(self, trace, seed: int = 1):
+    def __init__(self, trace, seed: int = 1, m31_commits: bool = False):
         self.arrays: dict = {}
         self.digests: dict = {}
         self._pins = []
@@ -83,6 +85,8 @@ class Inputs:
                 continue
             if kind == "commit":
                 host = inputs.circle_fft_coefficients(seed + lg, lg)
+                if m31_commits:
+                    host = np.ascontiguousarray(host[:, 0])
             elif kind == "fold":
                 host = inputs.random_m31(seed + 200 + lg, 4 << lg)
             elif kind == "chunk":
@@ -139,7 +143,8 @@ def run_server(server: azk.Server, trace, data: Inputs, collect: bool = False, s
             d = descs.get(lg)
             if d is None:
                 d = descs[lg] = azk.prover_desc(lg, commit_rounds(lg), SECURITY, DIGEST)
-            tk = server.submit_commit(d, data.get(kind, lg))
+            a = data.get(kind, lg)
+            tk = server.submit_commit_m31(d, a) if a.ndim == 1 else server.submit_commit(d, a)
         elif kind == "fold":
             tk = server.submit_fold(lg, FOLD_ROUNDS, data.get(kind, lg), fold_challenges())
         elif kind == "chunk":
@@ -180,7 +185,10 @@ class PlainRunner:
         if kind == "commit":
             p = azk.CircleCodewordPCSFRIProverV1(self.ctx, lg, commit_rounds(lg), SECURITY, DIGEST)
             self._objs.append(p)
-            f = (lambda sd, p=p, a=a: p.prove_fft_coefficients(a))
+            if a.ndim == 1:  # M31 words
+                f = (lambda sd, p=p, a=a: p.prove_fft_coefficients_m31(a))
+            else:
+                f = (lambda sd, p=p, a=a: p.prove_fft_coefficients(a))
         elif kind == "fold":
             fp = azk.CircleFRIFoldChainPlan(self.ctx, lg, FOLD_ROUNDS)
             self._objs.append(fp)

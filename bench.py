@@ -260,12 +260,16 @@ def serving_form(ctx, seed: int = 77, jobs: int = 200, sat_jobs: int = 300) -> d
     the host inside each job, and digests supplied by the caller (device work only)."""
     from apple_amd import serving
     shapes = [(0.0, "commit", lg) for lg in range(12, 23)] + [(0.0, "sumcheck", lg) for lg in range(16, 23)]
-    data = serving.Inputs(shapes, seed=seed)
+    data = serving.Inputs(shapes, seed=seed, m31_commits=True)
     plain = serving.PlainRunner(ctx, shapes, data)
     # 12 sum-check lanes: each hashes its job's words on one host core (the box gives 16)
     srv = azk.Server(0, max_batch=64, aux_lanes=12)
     serving.run_server(srv, shapes + shapes, data)  # warm: provers and plans of every shape
-    out = {"workload": f"poisson mix: 50% Circle PCS commits log N U{{12..22}} (10 rounds, 32 queries), 50% multilinear "
+    # steady state: the server holds the provers and plans a saturated batch needs (several of
+    # one shape in one device program), as a serving process does after its first batches
+    serving.run_server(srv, serving.make_trace(seed, sat_jobs, rate=0), data)
+    out = {"workload": f"poisson mix: 50% Circle PCS commits log N U{{12..22}} from M31 coefficients (10 rounds, 32 "
+                       f"queries), 50% multilinear "
                        f"sum-checks 2^U{{16..22}} words; seed {seed}; rate 0.8 x saturated",
            "jobs": jobs}
     for variant, supply in (("host_digests", False), ("supplied_digests", True)):

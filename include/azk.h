@@ -589,6 +589,11 @@ azk_status azk_server_create(const azk_server_desc *desc, azk_server **out);
 void azk_server_destroy(azk_server *server);
 azk_status azk_server_submit_commit(azk_server *server, const azk_circle_prover_desc *desc, const void *coeffs,
                                     size_t bytes, uint64_t *ticket);
+/* A commit from M31 line-basis coefficients (at least 2^log_n words; coefficient i's QM31 lift is
+ * (coeffs[i], 0, 0, 0)), as azk_circle_prove_fft_coefficients_m31: the proof bytes equal those of
+ * azk_server_submit_commit on the lifted array, from a quarter of the upload. */
+azk_status azk_server_submit_commit_m31(azk_server *server, const azk_circle_prover_desc *desc, const uint32_t *coeffs,
+                                        size_t words, uint64_t *ticket);
 azk_status azk_server_submit_sumcheck(azk_server *server, const uint32_t *evals, size_t count, uint64_t *ticket);
 /* A multilinear sum-check whose initial digest the caller supplies (azk_sumcheck_prove_with_digest). */
 azk_status azk_server_submit_sumcheck_digest(azk_server *server, const uint32_t *evals, size_t count,

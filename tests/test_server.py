@@ -164,3 +164,49 @@ def test_server_grinding_job(ctx):
     p = azk.CircleCodewordPCSFRIProverV1(ctx, 11, 5, sec, serving.DIGEST)
     assert got == p.prove_fft_coefficients(coeffs)
     assert azk.circle_verify(got, serving.DIGEST)
+
+
+@pytest.mark.gpu
+def test_server_m31_commits_match_lifted_commits(ctx):
+    """Commits from M31 words and from their QM31 lift, submitted together (one device
+    program may hold both forms), give the plain QM31 prover's proof bytes."""
+    trace = [(0.0, "commit", lg) for lg in (10, 12, 11, 13, 9)]
+    q = serving.Inputs(trace, seed=8)
+    m = serving.Inputs(trace, seed=8, m31_commits=True)
+    srv = azk.Server(0, max_batch=64, aux_lanes=1)
+    print("server M31 and QM31 commits", flush=True)
+    tickets = []
+    for _, _, lg in trace:
+        d = azk.prover_desc(lg, serving.commit_rounds(lg), serving.SECURITY, serving.DIGEST)
+        assert m.get("commit", lg).shape == (1 << lg,)
+        tickets.append((lg, srv.submit_commit_m31(d, m.get("commit", lg)), srv.submit_commit(d, q.get("commit", lg))))
+    infos, got = [], []
+    for lg, t1, t2 in tickets:
+        infos += [srv.wait(t1), srv.wait(t2)]
+        got.append((lg, srv.take(t1), srv.take(t2)))
+    with pytest.raises(azk.AZKError):  # fewer words than the codeword has coefficients
+        srv.submit_commit_m31(azk.prover_desc(10, 3, serving.SECURITY, serving.DIGEST), np.zeros(512, np.uint32))
+    srv.close()
+    assert all(i["status"] == 0 for i in infos)
+    assert max(i["batch_jobs"] for i in infos) > 1
+    for lg, a, b in got:
+        want = alone(ctx, "commit", lg, q)
+        assert a == want and b == want, lg
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_server_sumcheck_shapes_under_tight_budget(ctx, lanes):
+    """Sum-check lanes with room for about three plans run a cycle of shapes: each lane keeps
+    the plans of the shapes it ran and gives back the least recently used ones when the next
+    shape does not fit; every proof equals the plain call's."""
+    trace = [(0.0, "sumcheck", lg) for lg in (12, 13, 14, 12, 14, 13, 15, 12, 15, 14)]
+    data = serving.Inputs(trace, seed=6)
+    srv = azk.Server(0, max_batch=64, aux_lanes=lanes, hbm_budget=600 << 10)
+    print("server sum-check shapes, lanes", lanes, flush=True)
+    r = serving.run_server(srv, trace, data, collect=True, supply_digests=True)
+    srv.close()
+    assert r["failed"] == 0
+    want = {lg: alone(ctx, "sumcheck", lg, data) for lg in {lg for _, _, lg in trace}}
+    for kind, lg, info, proof in r["results"]:
+        assert proof == want[lg], lg
