@@ -1,6 +1,6 @@
 """Sizes past the oracle's reach, through size-independent checks with the library's CPU gates
 (which tests/test_gates.py pins to the oracle and to the reference's digests): the codeword
-plan at 2^24 and 2^26 (the bench's standalone shape) equal to the host Circle FFT, the fold
+plan at 2^22, 2^24 and 2^26 (the bench's standalone shape) equal to the host Circle FFT, the fold
 chain at 2^24 equal to the host fold, and a 2^24 commit (10 rounds, 32 queries) whose proof
 verifies and whose every opened first-layer value equals the host FFT of its coefficients."""
 import numpy as np
@@ -20,7 +20,7 @@ def ctx():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("log_n", [24, 26])
+@pytest.mark.parametrize("log_n", [22, 24, 26])
 def test_codeword_matches_host_fft(ctx, log_n):
     print(f"codeword 2^{log_n} vs host FFT", flush=True)
     c = np.random.default_rng(log_n).integers(0, P, size=(1 << log_n, 4), dtype=np.uint32)
