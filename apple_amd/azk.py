@@ -180,6 +180,7 @@ def load() -> C.CDLL:
         "azk_merkle_commit_resident": (st, [vp, vp, sz, vp, sp]),
         "azk_merkle_commit_verified": (st, [vp, vp, sz, vp, sp]),
         "azk_merkle_open": (st, [vp, u32, vp, sz, vp]),
+        "azk_merkle_open_batch": (st, [vp, vp, sz, vp, sz, vp]),
         "azk_merkle_plan_clear": (st, [vp]),
         "azk_merkle_commit_noalloc": (st, [vp, vp, sz, u32, u32, u32, vp, sz, vp]),
         "azk_prehashed_plan_create": (st, [vp, u32, C.POINTER(vp)]),
@@ -538,6 +539,16 @@ class SHA3RawLeavesMerkleCommitPlan(_Plan):
         root = np.zeros(32, np.uint8)
         _ck(self.ctx._lib.azk_merkle_open(self.h, index, _ptr(sib), 32 * self.depth, _ptr(root)))
         return sib[: 32 * self.depth].tobytes(), root.tobytes()
+
+    def open_batch(self, indices) -> tuple[list[bytes], bytes]:
+        """Openings of every index in one call: (sibling paths, root)."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint32)
+        per = 32 * self.depth
+        sib = np.zeros(max(per * idx.size, 1), np.uint8)
+        root = np.zeros(32, np.uint8)
+        _ck(self.ctx._lib.azk_merkle_open_batch(self.h, _ptr(idx) if idx.size else None, idx.size, _ptr(sib),
+                                                 per * idx.size, _ptr(root)))
+        return [sib[per * k: per * (k + 1)].tobytes() for k in range(idx.size)], root.tobytes()
 
     def clear(self) -> None:
         _ck(self.ctx._lib.azk_merkle_plan_clear(self.h))

@@ -1060,6 +1060,20 @@ __global__ __launch_bounds__(256) void k_merkle_path(const uint32_t *__restrict_
     out[t] = tree[8 * ((2 * m - ((2 * m) >> lv)) + ((leaf >> lv) ^ 1ull)) + wd];
 }
 
+// Sibling paths of `count` leaves in one launch: word t of the output is word t % (8 depth) of
+// leaf t / (8 depth)'s path (the leaf indices are below m, checked by the host).
+__global__ __launch_bounds__(256) void k_merkle_paths(const uint32_t *__restrict__ tree, uint64_t m, uint32_t depth,
+                                                      const uint32_t *__restrict__ leaves, uint32_t count,
+                                                      uint32_t *__restrict__ out)
+{
+    const uint32_t per = 8 * depth;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)per * count) return;
+    const uint32_t k = (uint32_t)(t / per), r = (uint32_t)(t % per), lv = r / 8, wd = r % 8;
+    const uint64_t leaf = leaves[k];
+    out[t] = tree[8 * ((2 * m - ((2 * m) >> lv)) + ((leaf >> lv) ^ 1ull)) + wd];
+}
+
 // ============================================================== host launchers
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
@@ -1090,6 +1104,14 @@ void launch_oneblock_hash(hipStream_t st, const void *in, uint32_t stride, uint3
 void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out)
 {
     hipLaunchKernelGGL(k_merkle_path, dim3(blocks_for(8 * depth + 8, 256)), dim3(256), 0, st, tree, m, depth, leaf, out);
+}
+
+void launch_merkle_paths(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, const uint32_t *leaves,
+                         uint32_t count, uint32_t *out)
+{
+    const uint64_t words = (uint64_t)8 * depth * count;
+    if (!words) return;
+    hipLaunchKernelGGL(k_merkle_paths, dim3(blocks_for(words, 256)), dim3(256), 0, st, tree, m, depth, leaves, count, out);
 }
 
 void launch_fft_twiddles(hipStream_t st, uint32_t *out, uint32_t log_n)

@@ -119,6 +119,8 @@ void launch_keccakf_batch(hipStream_t st, const void *in, uint32_t in_stride, vo
 void launch_oneblock_hash(hipStream_t st, const void *in, uint32_t stride, uint32_t len, void *out, uint32_t out_stride,
                           uint32_t count, uint32_t suffix);
 void launch_merkle_path(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, uint64_t leaf, uint32_t *out);
+void launch_merkle_paths(hipStream_t st, const uint32_t *tree, uint64_t m, uint32_t depth, const uint32_t *leaves,
+                         uint32_t count, uint32_t *out);
 void launch_fft_twiddles(hipStream_t st, uint32_t *out, uint32_t log_n);
 void launch_inverse(hipStream_t st, const uint32_t *in, uint32_t *out, uint64_t n);
 

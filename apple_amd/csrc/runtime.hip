@@ -545,7 +545,7 @@ struct azk_merkle_plan {
     azk_context *ctx;
     azk_merkle_desc d;
     uint32_t log_m, log_t;
-    DeviceMem tree, staging, path;
+    DeviceMem tree, staging, path, paths;
     bool committed = false;
     std::mutex mu;
 };
@@ -582,7 +582,7 @@ extern "C" azk_status azk_merkle_plan_create(azk_context *ctx, const azk_merkle_
         return fail(AZK_ERR_UNSUPPORTED_ONE_BLOCK_LENGTH, "leaves are hashed as one block (<= 136 bytes), got %u",
                     desc->leaf_length);
     if (desc->leaf_stride < desc->leaf_length) return fail(AZK_ERR_INVALID_LAYOUT, "leaf stride shorter than leaf length");
-    auto *p = new azk_merkle_plan{ctx, *desc, ilog2(desc->leaf_count), 0, {}, {}, {}, false, {}};
+    auto *p = new azk_merkle_plan{ctx, *desc, ilog2(desc->leaf_count), 0, {}, {}, {}, {}, false, {}};
     uint32_t lt = desc->subtree_log ? desc->subtree_log : merkle_default_log_tile(p->log_m);
     if (lt > kDefaultLogTile) lt = kDefaultLogTile;  // 16 x 2^11 bytes of LDS per block
     p->log_t = std::min(lt, p->log_m);
@@ -675,6 +675,36 @@ extern "C" azk_status azk_merkle_open(azk_merkle_plan *p, uint32_t index, uint8_
     HIP_TRY(hipStreamSynchronize(p->ctx->stream));
     if (need) memcpy(siblings, h.data(), need);
     if (root) memcpy(root, h.data() + need, 32);
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_merkle_open_batch(azk_merkle_plan *p, const uint32_t *indices, size_t count,
+                                            uint8_t *siblings, size_t siblings_bytes, uint8_t root[32])
+{
+    CHECK_ARG(p && (indices || !count));
+    std::lock_guard<std::mutex> g(p->mu);
+    if (!p->committed) return fail(AZK_ERR_INVALID_LAYOUT, "no resident tree: commit before opening");
+    if (count > (1u << 24)) return fail(AZK_ERR_INVALID_LAYOUT, "%zu openings in one call (at most 2^24)", count);
+    for (size_t k = 0; k < count; ++k)
+        if (indices[k] >= p->d.leaf_count)
+            return fail(AZK_ERR_INVALID_LAYOUT, "leaf index %u >= %u (opening %zu)", indices[k], p->d.leaf_count, k);
+    const size_t per = (size_t)32 * p->log_m, need = per * count;
+    if (siblings_bytes < need || (need && !siblings)) return fail(AZK_ERR_INVALID_LAYOUT, "sibling buffer too small");
+    HIP_TRY(hipSetDevice(p->ctx->device));
+    std::lock_guard<std::mutex> gc(p->ctx->mu);
+    // device words: the indices (4 bytes each, 256-byte aligned), then the paths
+    const size_t idx_bytes = (4 * count + 255) & ~(size_t)255;
+    azk_status s = p->paths.ensure(std::max<size_t>(idx_bytes + need, 256));
+    if (s) return s;
+    hipStream_t st = p->ctx->stream;
+    uint32_t *dev_idx = (uint32_t *)p->paths.p, *dev_out = (uint32_t *)((uint8_t *)p->paths.p + idx_bytes);
+    if (count) HIP_TRY(hipMemcpyAsync(dev_idx, indices, 4 * count, hipMemcpyHostToDevice, st));
+    launch_merkle_paths(st, (const uint32_t *)p->tree.p, p->d.leaf_count, p->log_m, dev_idx, (uint32_t)count, dev_out);
+    HIP_TRY(hipGetLastError());
+    if (need) HIP_TRY(hipMemcpyAsync(siblings, dev_out, need, hipMemcpyDeviceToHost, st));
+    if (root) HIP_TRY(hipMemcpyAsync(root, (const uint8_t *)p->tree.p + (size_t)32 * (2 * (uint64_t)p->d.leaf_count - 2), 32,
+                                     hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     return AZK_OK;
 }
 

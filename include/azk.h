@@ -143,6 +143,12 @@ azk_status azk_merkle_commit_verified(azk_merkle_plan *plan, const void *leaves,
  * MerkleOracle.openingSHA3_256 (MerkleOracle.swift:53-110). */
 azk_status azk_merkle_open(azk_merkle_plan *plan, uint32_t index, uint8_t *siblings, size_t siblings_bytes,
                            uint8_t root[32]);
+/* Openings of `count` leaves in one call (one gather launch, one copy back): path k occupies
+ * siblings[k * 32 log2(leaf_count) ...], each as azk_merkle_open writes it; root as there.  No
+ * single counterpart in the reference, whose openRawLeaf (:195-226) opens one leaf per call;
+ * any index >= leaf_count rejects the whole call (AZK_ERR_INVALID_LAYOUT). */
+azk_status azk_merkle_open_batch(azk_merkle_plan *plan, const uint32_t *indices, size_t count, uint8_t *siblings,
+                                 size_t siblings_bytes, uint8_t root[32]);
 azk_status azk_merkle_plan_clear(azk_merkle_plan *plan);
 /* No counterpart in the reference: an allocation-free variant of azk_merkle_commit_resident
  * for callers that cannot allocate device memory (code under hipGraph capture, serving loops

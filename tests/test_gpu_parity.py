@@ -96,11 +96,15 @@ def test_merkle_commit_and_openings(ctx, count, length, stride):
     root = plan.commit(raw)
     assert root == O.merkle_root(raw, count, stride, length)
     idx = inputs.splitmix64(count, 16) % np.uint64(count)
-    for i in [0, count - 1, *[int(x) for x in idx]]:
+    order = [0, count - 1, *[int(x) for x in idx]]
+    for i in order:
         sib, r2 = plan.open(i)
         want_sib, want_root = O.merkle_opening(raw, count, stride, length, i)
         assert sib == want_sib and r2 == want_root == root
         assert O.merkle_verify(raw[i * stride:i * stride + length].tobytes(), i, sib, root)
+    # every opening in one call (repeats included) equals the one-at-a-time form
+    paths, r3 = plan.open_batch(order + order[:3])
+    assert r3 == root and paths == [plan.open(i)[0] for i in order + order[:3]]
 
 
 @pytest.mark.parametrize("length", [0, 32, 64, 128, 135, 136])
@@ -128,9 +132,14 @@ def test_merkle_resident_offset_and_clear(ctx):
     buf.upload(raw, offset=256)
     plan = azk.SHA3RawLeavesMerkleCommitPlan(ctx, count, length, length)
     assert plan.commit_resident(buf, 256) == O.merkle_root(raw, count, length, length)
+    assert plan.open_batch([])[0] == []
+    with pytest.raises(azk.AZKError):
+        plan.open_batch([1, count])  # one index out of range rejects the call
     plan.clear()
     with pytest.raises(azk.AZKError):
         plan.open(0)
+    with pytest.raises(azk.AZKError):
+        plan.open_batch([0])
     with pytest.raises(azk.AZKError):
         plan.commit_resident(buf, 512)  # range past the end
 
