@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device_field.hpp"
 #include "device_keccak.hpp"
@@ -583,6 +584,52 @@ template <class E>
 __device__ __forceinline__ void fft_tile_stages(typename E::S *s, const uint32_t *tws, uint32_t k, uint32_t m)
 {
     int t = (int)k - 1;
+    // M31 words: radix-8 rounds (stages t, t - 1, t - 2 between one LDS load and one store) while
+    // a block has at least one group of eight per thread: group g holds i0 + j 2^sh (sh = t - 2 + m,
+    // j < 8); stage t pairs j with j + 4 (twiddle hp), stage t - 1 pairs j with j + 2 (twiddle
+    // 2 hp + j/4) and stage t - 2 pairs j with j + 1 (twiddle 4 hp + j/2), the same twiddle slots
+    // as one stage at a time.  Measured on 2^26 codewords: 0.628 ms against 0.645 for M31 words;
+    // for QM31 values (eight in 32 VGPRs per thread) 1.60 ms against 1.57-1.60, so QM31 stays
+    // radix-4.
+    if (std::is_same<E, M31Elem>::value && k + m >= 3 && (1u << (k + m - 3)) >= blockDim.x) {
+        for (; t >= 2; t -= 3) {
+            const uint32_t groups = 1u << (k + m - 3), sh = (uint32_t)t - 2 + m, d = 1u << sh;
+            const uint32_t *tw_t = tws + (1u << (k - 1 - t)), *tw_u = tws + (1u << (k - t)),
+                           *tw_v = tws + (1u << (k + 1 - t));
+            for (uint32_t g = threadIdx.x; g < groups; g += blockDim.x) {
+                const uint32_t hp = g >> sh;
+                const uint32_t i0 = (hp << (sh + 3)) | (g & (d - 1));
+                typename E::V a[8];
+#pragma unroll
+                for (uint32_t j = 0; j < 8; ++j) a[j] = E::ld(s[i0 + j * d]);
+                const uint32_t w = tw_t[hp];
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j) {
+                    const typename E::V r = E::scale(a[j + 4], w);
+                    a[j + 4] = E::sub(a[j], r);
+                    a[j] = E::add(a[j], r);
+                }
+#pragma unroll
+                for (uint32_t h = 0; h < 2; ++h) {
+                    const uint32_t wu = tw_u[2 * hp + h];
+#pragma unroll
+                    for (uint32_t j = 4 * h; j < 4 * h + 2; ++j) {
+                        const typename E::V r = E::scale(a[j + 2], wu);
+                        a[j + 2] = E::sub(a[j], r);
+                        a[j] = E::add(a[j], r);
+                    }
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    const uint32_t wv = tw_v[4 * hp + q];
+                    const typename E::V r = E::scale(a[2 * q + 1], wv);
+                    s[i0 + (2 * q + 1) * d] = E::st(E::sub(a[2 * q], r));
+                    s[i0 + 2 * q * d] = E::st(E::add(a[2 * q], r));
+                }
+            }
+            __syncthreads();
+        }
+    }
     for (; t >= 1; t -= 2) {
         const uint32_t quarter = 1u << (k + m - 2), sh = (uint32_t)t - 1 + m, d = 1u << sh;
         const uint32_t *tw_hi = tws + (1u << (k - 1 - t)), *tw_lo = tws + (1u << (k - t));
