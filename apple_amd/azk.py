@@ -130,7 +130,12 @@ VALU_CALIBRATION_INSTRUCTIONS = ("v_xor_b32", "v_bitop3_b32", "v_alignbit_b32", 
 class PhaseTimes(C.Structure):
     _fields_ = [("fft_ns", C.c_uint64), ("commit0_ns", C.c_uint64), ("fold_ns", C.c_uint64),
                 ("tree_ns", C.c_uint64), ("transcript_ns", C.c_uint64), ("emit_ns", C.c_uint64),
-                ("host_ns", C.c_uint64), ("total_ns", C.c_uint64)]
+                ("host_ns", C.c_uint64), ("total_ns", C.c_uint64), ("tree_launches", C.c_uint32),
+                ("tree_levels", C.c_uint32)]
+
+
+class LatencyCalibration(C.Structure):
+    _fields_ = [("wave_perm_us", C.c_double), ("pair_perm_us", C.c_double), ("graph_kernel_us", C.c_double)]
 
 
 _lib = None
@@ -266,6 +271,7 @@ def load() -> C.CDLL:
         "azk_now_ns": (C.c_uint64, []),
         "azk_host_alloc": (st, [sz, C.POINTER(vp)]),
         "azk_calibrate_valu": (st, [vp, C.POINTER(ValuCalibration)]),
+        "azk_calibrate_latency": (st, [vp, C.POINTER(LatencyCalibration)]),
         "azk_sumcheck_chunk_plan_create": (st, [vp, C.c_uint64, u32, C.POINTER(vp)]),
         "azk_sumcheck_chunk_plan_destroy": (None, [vp]),
         "azk_sumcheck_chunk_coefficient_words": (sz, [C.c_uint64, u32]),
@@ -1090,6 +1096,14 @@ def calibrate_valu(ctx: "Context") -> dict:
             "cycles_per_wave64_instruction": {n: round(c.cycles_per_instr[k], 3)
                                               for k, n in enumerate(VALU_CALIBRATION_INSTRUCTIONS)},
             "clock_mhz": {n: round(c.clock_mhz[k], 1) for k, n in enumerate(VALU_CALIBRATION_INSTRUCTIONS)}}
+
+
+def calibrate_latency(ctx: "Context") -> dict:
+    """Dependent-permutation latency of the wave and two-lane Keccak forms, and the cost of one
+    kernel of a captured graph (measurement aid)."""
+    c = LatencyCalibration()
+    _ck(load().azk_calibrate_latency(ctx.h, C.byref(c)))
+    return {"wave_perm_us": c.wave_perm_us, "pair_perm_us": c.pair_perm_us, "graph_kernel_us": c.graph_kernel_us}
 
 
 def grinding_digest_meets_target(digest: bytes, bits: int) -> bool:

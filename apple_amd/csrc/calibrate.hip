@@ -70,6 +70,38 @@ __global__ __launch_bounds__(256) void k_cal_keccak(uint32_t *out, uint32_t perm
     out[t] = x;
 }
 
+// `perms` dependent permutations of one state on one wave in the one-state-per-wave form; the
+// 100 MHz clock is read by lane 0 around the chain.
+__global__ __launch_bounds__(64) void k_lat_wave(uint32_t *out, uint32_t perms, unsigned long long *clk)
+{
+    const WaveLane c = wave_lane();
+    uint32_t lo = c.word < 25 ? c.word * 2654435761u : 0u, hi = c.word < 25 ? c.word ^ 0x5bd1e995u : 0u;
+    unsigned long long r0 = 0;
+    if (threadIdx.x == 0) r0 = wall_clock64();
+    for (uint32_t i = 0; i < perms; ++i) kf_w(lo, hi, c);
+    if (threadIdx.x == 0) clk[0] = wall_clock64() - r0;
+    out[threadIdx.x] = lo ^ hi;
+}
+
+// The same with 32 states on one wave in the two-lanes-per-state form.
+__global__ __launch_bounds__(64) void k_lat_pair(uint32_t *out, uint32_t perms, unsigned long long *clk)
+{
+    const bool hi = threadIdx.x & 1u;
+    uint32_t a[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) a[k] = (threadIdx.x >> 1) * 2654435761u + k * (hi ? 40503u : 7919u);
+    unsigned long long r0 = 0;
+    if (threadIdx.x == 0) r0 = wall_clock64();
+    for (uint32_t i = 0; i < perms; ++i) kf_h(a, hi);
+    if (threadIdx.x == 0) clk[0] = wall_clock64() - r0;
+    uint32_t x = 0;
+#pragma unroll
+    for (int k = 0; k < 25; ++k) x ^= a[k];
+    out[threadIdx.x] = x;
+}
+
+__global__ void k_lat_empty() {}
+
 typedef void (*cal_fn)(uint32_t *, uint32_t, unsigned long long *);
 
 // Best of `reps` launches of `grid` x 256 lanes: elapsed ms and the clock (MHz) of that launch.
@@ -131,5 +163,61 @@ extern "C" azk_status azk_calibrate_valu(azk_context *ctx, azk_valu_calibration 
         out->cycles_per_instr[k] = ms * 1e-3 * simds * mhz * 1e6 / instrs;
         out->clock_mhz[k] = mhz;
     }
+    return AZK_OK;
+}
+
+extern "C" azk_status azk_calibrate_latency(azk_context *ctx, azk_latency_calibration *out)
+{
+    CHECK_ARG(ctx && out);
+    memset(out, 0, sizeof *out);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    constexpr uint32_t kChain = 241, kNodes = 64, kReps = 5;
+    DeviceMem buf;
+    azk_status s = buf.ensure(64 * 4 + 64);
+    if (s) return s;
+    uint32_t *words = (uint32_t *)buf.p;
+    unsigned long long *clk = (unsigned long long *)((uint8_t *)buf.p + 256);
+    hipStream_t st = ctx->stream;
+    const cal_fn chains[2] = {k_lat_wave, k_lat_pair};
+    double *dst[2] = {&out->wave_perm_us, &out->pair_perm_us};
+    for (int k = 0; k < 2; ++k) {
+        double best = 1e30;
+        for (uint32_t r = 0; r < kReps; ++r) {
+            hipLaunchKernelGGL(chains[k], dim3(1), dim3(64), 0, st, words, kChain, clk);
+            HIP_TRY(hipGetLastError());
+            unsigned long long ticks = 0;
+            HIP_TRY(hipMemcpyAsync(&ticks, clk, 8, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            best = std::min(best, (double)ticks / 100.0 / kChain);  // 100 MHz ticks -> us
+        }
+        *dst[k] = best;
+    }
+    // back-to-back empty kernels in a captured graph
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (uint32_t i = 0; i < kNodes; ++i) hipLaunchKernelGGL(k_lat_empty, dim3(1), dim3(64), 0, st);
+    hipError_t e = hipStreamEndCapture(st, &gr);
+    if (e == hipSuccess) e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    if (gr) (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return fail(AZK_ERR_COMMAND_FAILED, "graph of empty kernels: %s", hipGetErrorString(e));
+    Timer tm;
+    s = tm.init();
+    double best = 1e30;
+    for (uint32_t r = 0; !s && r < kReps + 1; ++r) {
+        hipError_t le = hipEventRecord(tm.a, st);
+        if (le == hipSuccess) le = hipGraphLaunch(ge, st);
+        if (le == hipSuccess) le = hipEventRecord(tm.b, st);
+        if (le == hipSuccess) le = hipEventSynchronize(tm.b);
+        if (le != hipSuccess) {
+            s = fail(AZK_ERR_COMMAND_FAILED, "graph launch: %s", hipGetErrorString(le));
+            break;
+        }
+        if (r) best = std::min(best, (double)tm.elapsed_ns() / 1e3 / kNodes);  // the first launch warms
+    }
+    (void)hipGraphExecDestroy(ge);
+    if (s) return s;
+    out->graph_kernel_us = best;
     return AZK_OK;
 }

@@ -568,6 +568,15 @@ azk_status run(Runner &R, bool graph, azk_phase_times *phases, azk_stats *stats)
             if (slot[clk.tag[i]]) *slot[clk.tag[i]] += (uint64_t)(ms * 1e6);
         }
         ph.host_ns = h1 - h0;
+        auto count = [&](const std::vector<Launch<TreeDesc>> &steps) {
+            for (const auto &l : steps)
+                if (l.blocks) {
+                    ++ph.tree_launches;
+                    ph.tree_levels += l.v.empty() ? 0 : l.v[0].log_c;
+                }
+        };
+        count(R.tree0);
+        for (const auto &t : R.tree_r) count(t);
         float ms = 0;
         (void)hipEventElapsedTime(&ms, clk.ev.front(), clk.ev.back());
         ph.total_ns = (uint64_t)(ms * 1e6);

@@ -655,7 +655,20 @@ def main() -> int:
         prover.prove_fft_coefficients_m31_resident(buf)
         ph.append(prover.phase_times())
     prover.set_timing(False)
+    counts = {k: ph[0].pop(k) for k in ("tree_launches", "tree_levels")}
+    for x in ph[1:]:
+        for k in counts:
+            x.pop(k)
     phases = {k.replace("_ns", "_ms"): float(np.median([x[k] for x in ph])) / 1e6 for k in ph[0]}
+    # the upper trees are a serial chain: one dependent one-state-per-wave permutation per level and
+    # one graph kernel per launch, both measured on this chip in the same run (the round
+    # transcripts, 1-2 dependent permutations each inside the last launch of a round, come on top)
+    chain_lat = azk.calibrate_latency(ctx)
+    floor_ms = (counts["tree_levels"] * chain_lat["wave_perm_us"]
+                + counts["tree_launches"] * chain_lat["graph_kernel_us"]) / 1e3
+    serial_chain = {"bound": "latency", "kernel": "k_tree_reduce1 (upper Merkle levels, round transcripts)",
+                    "latency_calibration": chain_lat, **counts, "tree_ms": phases["tree_ms"], "floor_ms": floor_ms,
+                    "frac": floor_ms / phases["tree_ms"] if phases["tree_ms"] else None}
     # dominant kernel: first-layer commitment (final FFT stages fused with 2^21 leaf hashes and
     # 2^10 tile subtrees of 2^11 leaves); algorithmic work = 2^21 + (2^21 - 2^10) permutations
     n = 1 << LOG_N
@@ -691,6 +704,7 @@ def main() -> int:
                        "commits_per_s": len(lat_qm) / sum(lat_qm), "p50_ms": 1e3 * pct(lat_qm, 50)},
         "phases_ms": phases, "bit_exact_vs_oracle": checked, "device": caps["name"] or caps["arch"],
         "roofline": roofline,
+        "serial_chain": serial_chain,
         "valu_calibration": CALIBRATION,
     }
     if args.batched:

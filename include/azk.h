@@ -322,6 +322,8 @@ typedef struct azk_phase_times {
     uint64_t emit_ns;       /* readback of the tail, index upload, proof gather, proof readback */
     uint64_t host_ns;       /* host final-layer absorb + query squeeze */
     uint64_t total_ns;      /* first to last event, including the host step */
+    uint32_t tree_launches; /* upper-tree launches of the program */
+    uint32_t tree_levels;   /* upper Merkle levels they reduce */
 } azk_phase_times;
 azk_status azk_circle_prover_set_timing(azk_circle_prover *prover, int enable);
 azk_status azk_circle_prover_phase_times(const azk_circle_prover *prover, azk_phase_times *out);
@@ -631,6 +633,14 @@ typedef struct azk_valu_calibration {
     double cycles_per_instr[6], clock_mhz[6];
 } azk_valu_calibration;
 azk_status azk_calibrate_valu(azk_context *ctx, azk_valu_calibration *out);
+/* The serial chain's latency figures on this chip (no reference counterpart): microseconds per
+ * dependent Keccak-f[1600] on one wave in the one-state-per-wave form (241 in a row, the 100 MHz
+ * clock read in the kernel) and in the two-lanes-per-state form (32 states on the wave), and
+ * microseconds per kernel of a captured graph of 64 back-to-back empty kernels. */
+typedef struct azk_latency_calibration {
+    double wave_perm_us, pair_perm_us, graph_kernel_us;
+} azk_latency_calibration;
+azk_status azk_calibrate_latency(azk_context *ctx, azk_latency_calibration *out);
 
 #ifdef __cplusplus
 }

@@ -25,11 +25,13 @@ def test_binding_declares_every_header_symbol():
 
 
 def test_no_device_reports_status_not_crash():
-    if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
-        pytest.skip("a GPU is present")
-    with pytest.raises(azk.AZKError) as e:
-        azk.Context(0)
-    assert e.value.status == "noDevice"
+    try:
+        c = azk.Context(0)
+    except azk.AZKError as e:  # no GPU here: a status, not a crash
+        assert e.status == "noDevice"
+        return
+    c.close()
+    pytest.skip("a GPU is present")
 
 
 def test_splitmix_matches_oracle():
