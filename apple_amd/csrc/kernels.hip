@@ -200,13 +200,10 @@ __device__ __forceinline__ void ld_node(const gu32 *p, uint32_t (&d)[8])
     d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w; d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
 }
 
-// Thread index rotated by (block mod 4) waves, for loops over fewer items than threads: the
-// active waves of the blocks that share a CU then sit on different SIMDs instead of all on
-// the SIMDs of waves 0, 1, ...  A bijection on [0, blockDim.x) (a power of two).
-__device__ __forceinline__ uint32_t rotated_tid()
-{
-    return (threadIdx.x + ((blockIdx.x & 3u) << 6)) & (blockDim.x - 1);
-}
+// Loops over fewer items than threads use the low waves of a block.  Measured on MI355X (HW_ID
+// of every wave of 1024 blocks of 4 waves): blocks that share a CU are 256 apart in block
+// index and the dispatcher puts wave 0 of each on a different SIMD, so the active low waves of
+// co-resident blocks already spread over the CU's SIMDs.
 
 // In-place Merkle levels in LDS: the nodes of the current level sit `s` node slots (8s
 // words) apart; parent q of nodes 2q and 2q + 1 overwrites node 2q's slot, so the next
@@ -268,23 +265,28 @@ __device__ __forceinline__ void pair_parent(uint32_t *A, uint32_t q, uint32_t s,
 // Reduces `cnt` level-`l` nodes held contiguously in LDS (node q at words 8q..8q+7),
 // whose first node has global index `g` in that level, up to one node at A[0..7].  Every
 // computed level is written to the tree in HBM.  By the number of parents of a level:
-// more than half the threads, one lane per parent (~10 us a level); down to kWaveParents
-// per wave, two lanes per parent (5.4 us); the serial top, one wave per parent (2.5 us).
+// one lane per parent while the lane pairs would take more than 1/kPairDiv of the block
+// (~10 us a level alone on a SIMD, but half the issue of the pair form when the SIMDs are
+// full); down to kWaveParents per wave, two lanes per parent (5.4 us); the serial top, one
+// wave per parent (2.5 us).  kPairDiv 2 in the fold tiles (2^21 commit: the 2^17 and 2^18
+// latency-form folds 71 and 108 us instead of 77 and 112), 1 in the final FFT pass's tiles
+// (482 us with 2 against 475).
 #ifndef AZK_WAVE_PARENTS
 #define AZK_WAVE_PARENTS 2
 #endif
 constexpr uint32_t kWaveParents = AZK_WAVE_PARENTS;
+template <uint32_t kPairDiv = 1>
 __device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t cnt, int l, uint64_t g, uint32_t *tree, uint64_t m)
 {
     const uint32_t wave_max = kWaveParents * (blockDim.x >> 6);
     const WaveLane c = wave_lane();
-    const uint32_t tid = rotated_tid();
+    const uint32_t tid = threadIdx.x;
     uint32_t s = 1;
     while (cnt > 1) {
         uint32_t half = cnt >> 1;
         if (half <= wave_max) {
             wave_parents(A, half, s, l, g, tree, m, c);
-        } else if (2 * half <= blockDim.x) {
+        } else if (2 * half * kPairDiv <= blockDim.x) {
             if ((tid >> 1) < half) pair_parent(A, tid >> 1, s, tid & 1u, l, g, tree, m);  // pairs are whole
         } else {
             for (uint32_t q = tid; q < half; q += blockDim.x) {
@@ -310,6 +312,7 @@ __device__ __forceinline__ void reduce_lds(uint32_t *A, uint32_t cnt, int l, uin
 // across a permutation: the second leaf is read after the first is hashed, and the first
 // digest waits in the pair's LDS slot (the permutation's own state and temporaries take
 // 100 VGPRs, and 128 is the budget for four waves per SIMD).
+template <uint32_t kPairDiv = 1>
 __device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t T, uint64_t g0, uint32_t *tree, uint64_t m)
 {
     if (T == 1) {
@@ -336,7 +339,7 @@ __device__ __forceinline__ void tile_tree_qm31(uint32_t *A, uint32_t T, uint64_t
         st_node(A + 8 * p, par);
     }
     __syncthreads();
-    reduce_lds(A, T / 2, 1, g0 >> 1, tree, m);
+    reduce_lds<kPairDiv>(A, T / 2, 1, g0 >> 1, tree, m);
 }
 
 // Raw-leaf Merkle tile: leaves of `len` <= 136 bytes at `stride`, T leaves per
@@ -833,7 +836,7 @@ __device__ __forceinline__ void fold_tile(const FoldDesc &d, uint32_t tile)
     }
     if (!tree) return;
     __syncthreads();
-    tile_tree_qm31(lds, T, base, tree, d.n_out);
+    tile_tree_qm31<2>(lds, T, base, tree, d.n_out);
 }
 
 __global__ AZK_FOUR_WAVES __launch_bounds__(kFoldThreads) void k_fold(const FoldDesc *__restrict__ tab, uint32_t ntab)
@@ -877,6 +880,7 @@ __global__ __launch_bounds__(256) void k_fold_stream(const FoldDesc *__restrict_
 // Latency form for small layers: one output element per thread, T <= 256 outputs
 // per block; each thread folds its pair, hashes the new leaf and the block reduces
 // its leaf hashes in LDS (serial chain: one leaf permutation + log2 T levels).
+template <uint32_t kPairDiv>
 __device__ __forceinline__ void fold_leafwise_tile(const FoldDesc &d, uint32_t tile)
 {
     __shared__ uint32_t A[8 * 256];
@@ -890,7 +894,7 @@ __device__ __forceinline__ void fold_leafwise_tile(const FoldDesc &d, uint32_t t
     const qm31 c = {ch[0], ch[1], ch[2], ch[3]};
     uint32_t *tree = d.tree;
     const uint64_t n_out = d.n_out;
-    const uint32_t tid = rotated_tid();
+    const uint32_t tid = threadIdx.x;
     if (tree && 2 * T <= blockDim.x) {
         // two lanes per leaf: both fold the pair, the even lane stores the value, and the
         // pair hashes it with the two-lanes-per-state permutation (lo: a, c; hi: b, d)
@@ -929,21 +933,26 @@ __device__ __forceinline__ void fold_leafwise_tile(const FoldDesc &d, uint32_t t
     }
     if (!tree) return;
     __syncthreads();
-    reduce_lds(A, T, 0, base, tree, n_out);
+    reduce_lds<kPairDiv>(A, T, 0, base, tree, n_out);
 }
 
+// kPairDiv 2 when the launch holds two or more blocks per CU (2^21 commit: the
+// folds to 2^17 and 2^18 outputs 71 and 107 us instead of 77 and 112), 1 with one block per
+// CU, where the pair form's four waves take one SIMD each (to 2^16 outputs: 52 us against 55).
+template <uint32_t kPairDiv>
 __global__ __launch_bounds__(256) void k_fold_leafwise(const FoldDesc *__restrict__ tab, uint32_t ntab)
 {
     uint32_t tile;
     const FoldDesc &d = find_desc(tab, ntab, tile);
-    fold_leafwise_tile(d, tile);
+    fold_leafwise_tile<kPairDiv>(d, tile);
 }
 
+template <uint32_t kPairDiv>
 __global__ __launch_bounds__(256) void k_fold_leafwise_pair(const FoldPair p)
 {
     uint32_t tile;
     const FoldDesc d = pick(p, tile);
-    fold_leafwise_tile(d, tile);
+    fold_leafwise_tile<kPairDiv>(d, tile);
 }
 
 // ============================================================== transcript
@@ -1239,14 +1248,18 @@ void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t 
     hipLaunchKernelGGL(k_fold_stream, dim3(blocks), dim3(256), 0, st, t, n);
 }
 
-void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, const FoldDesc *host)
+void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t cus, const FoldDesc *host)
 {
     if (!blocks) return;
     FoldPair p;
-    if (fold_pair(host, n, p))
-        hipLaunchKernelGGL(k_fold_leafwise_pair, dim3(blocks), dim3(256), 0, st, p);
-    else
-        hipLaunchKernelGGL(k_fold_leafwise, dim3(blocks), dim3(256), 0, st, t, n);
+    const bool many = blocks >= 2 * cus;  // a transcript block beside one tile per CU does not count
+    if (fold_pair(host, n, p)) {
+        if (many) hipLaunchKernelGGL(k_fold_leafwise_pair<2>, dim3(blocks), dim3(256), 0, st, p);
+        else hipLaunchKernelGGL(k_fold_leafwise_pair<1>, dim3(blocks), dim3(256), 0, st, p);
+    } else {
+        if (many) hipLaunchKernelGGL(k_fold_leafwise<2>, dim3(blocks), dim3(256), 0, st, t, n);
+        else hipLaunchKernelGGL(k_fold_leafwise<1>, dim3(blocks), dim3(256), 0, st, t, n);
+    }
 }
 
 void launch_tree(hipStream_t st, const TreeDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_c, uint32_t waves,

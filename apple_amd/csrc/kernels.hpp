@@ -137,7 +137,9 @@ void launch_fft_final(hipStream_t st, const FftFinalDesc *t, uint32_t n, uint32_
 // `host`: host copy of the table; a launch of at most two entries passes them by value.
 void launch_fold(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t max_log_t,
                  const FoldDesc *host = nullptr);
-void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, const FoldDesc *host = nullptr);
+// cus: compute units of the device (the launch's blocks per CU set the Merkle levels' form)
+void launch_fold_leafwise(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks, uint32_t cus,
+                          const FoldDesc *host = nullptr);
 // No commitment: one output per thread, 256 outputs per block (log_t ignored).
 void launch_fold_stream(hipStream_t st, const FoldDesc *t, uint32_t n, uint32_t blocks);
 // waves per block: 4, 8 or 16

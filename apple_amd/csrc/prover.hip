@@ -345,7 +345,7 @@ azk_status enqueue_a(Runner &R, hipStream_t st, PhaseClock &clk)
         clk.mark(PH_TRANSCRIPT);
         auto &fb = R.fold_big[r], &fl = R.fold_leaf[r];
         launch_fold(st, tab(R, fb), fb.n(), fb.blocks, fb.max_log, fb.v.data());
-        launch_fold_leafwise(st, tab(R, fl), fl.n(), fl.blocks, fl.v.data());
+        launch_fold_leafwise(st, tab(R, fl), fl.n(), fl.blocks, (uint32_t)R.ctx->props.multiProcessorCount, fl.v.data());
         clk.mark(PH_FOLD);
         for (auto &l : R.tree_r[r]) launch_tree(st, tab(R, l), l.n(), l.blocks, l.max_log, R.tree_waves, l.v.data());
         clk.mark(PH_TREE);
