@@ -404,6 +404,8 @@ __global__ AZK_FOUR_WAVES __launch_bounds__(256) void k_merkle_raw_tiles1(const 
 }
 
 __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops);
+__device__ __forceinline__ void run_transcript_from(const TranscriptOp *ops, uint64_t *state, uint32_t nops, uint32_t lo,
+                                                    uint32_t hi);
 
 // Upper Merkle levels: each block loads 2^log_c nodes of `level` and reduces them
 // by log_c levels (MerkleCommitter.swift:853-893: parents per level + fused upper tree).
@@ -418,6 +420,16 @@ __device__ __forceinline__ void tree_reduce(const TreeDesc &d, uint32_t chunk)
     const int l = (int)d.level;
     uint32_t *A = lds;
     const uint64_t g = (uint64_t)chunk * C;
+    // the root's block loads the transcript state beside its nodes, not after the levels
+    uint32_t slo = 0, shi = 0;
+    if (d.t_ops && threadIdx.x < 64) {
+        const uint32_t w = wave_lane().word;
+        if (w < 25) {
+            const uint64_t v = glob(d.t_state)[w];
+            slo = (uint32_t)v;
+            shi = (uint32_t)(v >> 32);
+        }
+    }
     for (uint32_t q = threadIdx.x; q < C; q += blockDim.x) {
         uint32_t v[8];
         ld_node(node_ptr(tree, m, l, g + q), v);
@@ -427,7 +439,7 @@ __device__ __forceinline__ void tree_reduce(const TreeDesc &d, uint32_t chunk)
     reduce_lds(A, C, l, g, tree, m);
     if (d.t_ops) {  // uniform per block; the barrier makes the root's HBM words visible to wave 0
         __syncthreads();
-        if (threadIdx.x < 64) run_transcript(d.t_ops, d.t_state, d.t_nops);
+        if (threadIdx.x < 64) run_transcript_from(d.t_ops, d.t_state, d.t_nops, slo, shi);
     }
 }
 
@@ -1012,18 +1024,13 @@ __device__ __forceinline__ void t_squeeze(uint32_t lo, uint32_t hi, const WaveLa
 }
 
 // Runs a transcript program on its job's persistent 200-byte state with one whole wave
-// (the one-state-per-wave layout: every lane holding a state word loads it, the own lanes
-// store it back).
-__device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops)
+// (the one-state-per-wave layout: every lane holding a state word has it in lo, hi; the own
+// lanes store it back).
+__device__ __forceinline__ void run_transcript_from(const TranscriptOp *ops, uint64_t *state, uint32_t nops, uint32_t lo,
+                                                    uint32_t hi)
 {
     const WaveLane c = wave_lane();
     const bool holds = c.word < 25;
-    uint32_t lo = 0, hi = 0;
-    if (holds) {
-        const uint64_t v = glob(state)[c.word];
-        lo = (uint32_t)v;
-        hi = (uint32_t)(v >> 32);
-    }
     for (uint32_t k = 0; k < nops; ++k) {
         const TranscriptOp op = ops[k];
         if (op.kind == TOP_RESET) {
@@ -1041,6 +1048,19 @@ __device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t
         }
     }
     if (c.own) glob(state)[c.word] = (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// The same from the state in HBM.
+__device__ __forceinline__ void run_transcript(const TranscriptOp *ops, uint64_t *state, uint32_t nops)
+{
+    const uint32_t w = wave_lane().word;
+    uint32_t lo = 0, hi = 0;
+    if (w < 25) {
+        const uint64_t v = glob(state)[w];
+        lo = (uint32_t)v;
+        hi = (uint32_t)(v >> 32);
+    }
+    run_transcript_from(ops, state, nops, lo, hi);
 }
 
 __global__ __launch_bounds__(64) void k_transcript(const TranscriptDesc *__restrict__ tab, uint32_t ntab)
