@@ -268,9 +268,10 @@ __device__ __forceinline__ void pair_parent(uint32_t *A, uint32_t q, uint32_t s,
 // one lane per parent while the lane pairs would take more than 1/kPairDiv of the block
 // (~10 us a level alone on a SIMD, but half the issue of the pair form when the SIMDs are
 // full); down to kWaveParents per wave, two lanes per parent (5.4 us); the serial top, one
-// wave per parent (2.5 us).  kPairDiv 2 in the fold tiles (2^21 commit: the 2^17 and 2^18
-// latency-form folds 71 and 108 us instead of 77 and 112), 1 in the final FFT pass's tiles
-// (482 us with 2 against 475).
+// wave per parent (2.5 us).  kPairDiv 2 in the big-tile folds and in latency-form fold
+// launches of two or more blocks per CU (2^21 commit: the 2^17 and 2^18 latency-form folds
+// 71 and 108 us instead of 77 and 112); 1 in the final FFT pass's tiles (482 us with 2 against
+// 475) and in latency-form launches of one block per CU.
 #ifndef AZK_WAVE_PARENTS
 #define AZK_WAVE_PARENTS 2
 #endif
