@@ -660,11 +660,13 @@ def main() -> int:
         for k in counts:
             x.pop(k)
     phases = {k.replace("_ns", "_ms"): float(np.median([x[k] for x in ph])) / 1e6 for k in ph[0]}
-    # the upper trees are a serial chain: one dependent one-state-per-wave permutation per level and
-    # one graph kernel per launch, both measured on this chip in the same run (the round
-    # transcripts, 1-2 dependent permutations each inside the last launch of a round, come on top)
+    # the upper trees are a serial chain: one dependent one-state-per-wave permutation per level,
+    # one graph kernel per launch, and the round transcripts in the last launch of each round (two
+    # dependent permutations each: the root, then the challenge frame), all at latencies measured on
+    # this chip in the same run
     chain_lat = azk.calibrate_latency(ctx)
-    floor_ms = (counts["tree_levels"] * chain_lat["wave_perm_us"]
+    counts["transcript_perms"] = 2 * ROUNDS
+    floor_ms = ((counts["tree_levels"] + counts["transcript_perms"]) * chain_lat["wave_perm_us"]
                 + counts["tree_launches"] * chain_lat["graph_kernel_us"]) / 1e3
     serial_chain = {"bound": "latency", "kernel": "k_tree_reduce1 (upper Merkle levels, round transcripts)",
                     "latency_calibration": chain_lat, **counts, "tree_ms": phases["tree_ms"], "floor_ms": floor_ms,
